@@ -1,0 +1,280 @@
+"""apgar_amd — MI355X-native batched lattice search (ctypes view of libapgx.so).
+
+The compute path is the HIP library ``libapgx.so`` built in-tree by
+``apgar_amd.build``.  This module only mirrors its C-ABI records with ctypes and
+offers thin Python conveniences over them; it never computes a route itself.
+Loading fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libapgx.so"
+
+APGX_OK = 0
+ERROR_NAMES = {1: "unsupported", 2: "resource_exhausted", 3: "backend_failure", 4: "internal_invariant"}
+UNREACHABLE = 0xFFFFFFFF
+NET_REACHED, NET_DISCONNECTED, NET_BUDGET_EXHAUSTED, NET_INVALID = 0, 1, 2, 3
+LATTICE_SCHEMA_VERSION = 2
+
+
+class ApgxError(RuntimeError):
+    def __init__(self, code: int, detail: str):
+        super().__init__(f"{ERROR_NAMES.get(code, code)}: {detail}")
+        self.code = code
+        self.detail = detail
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("backend", C.c_char * 32), ("device_name", C.c_char * 256),
+                ("gcn_arch_name", C.c_char * 64), ("compute_units", C.c_uint32),
+                ("runtime_version", C.c_uint32), ("driver_version", C.c_uint32),
+                ("reserved", C.c_uint32), ("global_memory_bytes", C.c_uint64)]
+
+
+class LatticeDesc(C.Structure):
+    _fields_ = [("schema_version", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("layers", C.c_uint32), ("cost_bytes", C.c_uint32), ("via_cost", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
+
+
+class Window(C.Structure):
+    _fields_ = [("x0", C.c_uint32), ("y0", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class NetQuery(C.Structure):
+    _fields_ = [("net_id", C.c_uint64), ("pin_offset", C.c_uint32), ("pin_count", C.c_uint32),
+                ("window", Window)]
+
+
+class NetResult(C.Structure):
+    _fields_ = [("net_id", C.c_uint64), ("status", C.c_uint32), ("cost", C.c_uint32),
+                ("connection_count", C.c_uint32), ("connection_offset", C.c_uint32),
+                ("rounds", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class Connection(C.Structure):
+    _fields_ = [("pin_index", C.c_uint32), ("distance", C.c_uint32), ("cell_offset", C.c_uint64),
+                ("cell_count", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class RouteOptions(C.Structure):
+    _fields_ = [("max_rounds", C.c_uint32), ("tile_w", C.c_uint32), ("tile_h", C.c_uint32),
+                ("rounds_per_sync", C.c_uint32), ("max_device_bytes", C.c_uint64),
+                ("use_graph", C.c_uint32), ("profile_rounds", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
+
+
+class RouteStats(C.Structure):
+    _fields_ = [("device_ms", C.c_double), ("wall_ms", C.c_double), ("tile_visits", C.c_uint64),
+                ("cell_visits", C.c_uint64), ("algorithmic_bytes", C.c_uint64),
+                ("relaxations", C.c_uint64), ("inner_sweeps", C.c_uint64),
+                ("workspace_bytes", C.c_uint64), ("rounds", C.c_uint32), ("phases", C.c_uint32),
+                ("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("host_syncs", C.c_uint32),
+                ("graph_launches", C.c_uint32), ("round_kernel_ms", C.c_double)]
+
+
+class LatticeGenParams(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("layers", C.c_uint32),
+                ("cost_min", C.c_uint32), ("cost_max", C.c_uint32), ("blocked_ppm", C.c_uint32),
+                ("keepout_ppm", C.c_uint32), ("keepout_min_side", C.c_uint32),
+                ("keepout_max_side", C.c_uint32), ("reserved", C.c_uint32), ("seed", C.c_uint64)]
+
+
+_P = C.POINTER
+_SIGNATURES = {
+    "apgx_device_info_query": (C.c_int, [_P(DeviceInfo)]),
+    "apgx_last_error": (C.c_char_p, []),
+    "apgx_abi_version": (C.c_uint32, []),
+    "apgx_device_synchronize": (C.c_int, []),
+    "apgx_lattice_create": (C.c_int, [_P(LatticeDesc), C.c_void_p, _P(C.c_void_p)]),
+    "apgx_lattice_create_from_device": (C.c_int, [_P(LatticeDesc), C.c_void_p, _P(C.c_void_p)]),
+    "apgx_lattice_release": (None, [C.c_void_p]),
+    "apgx_lattice_device_costs": (C.c_void_p, [C.c_void_p]),
+    "apgx_route_nets": (C.c_int, [C.c_void_p, _P(NetQuery), C.c_uint32, _P(C.c_uint32), C.c_uint64,
+                                  _P(RouteOptions), _P(C.c_void_p)]),
+    "apgx_search_sizes": (C.c_int, [C.c_void_p, _P(C.c_uint64), _P(C.c_uint64)]),
+    "apgx_search_readback": (C.c_int, [C.c_void_p, _P(NetResult), _P(Connection), C.c_uint64,
+                                       _P(C.c_uint32), C.c_uint64]),
+    "apgx_search_stats": (C.c_int, [C.c_void_p, _P(RouteStats)]),
+    "apgx_search_release": (None, [C.c_void_p]),
+    "apgx_generate_lattice": (C.c_int, [_P(LatticeGenParams), _P(C.c_uint32)]),
+    "apgx_generate_pairs": (C.c_int, [_P(C.c_uint32), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                      C.c_uint64, _P(C.c_uint32)]),
+    "apgx_generate_nets": (C.c_int, [_P(C.c_uint32), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                     C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64,
+                                     _P(NetQuery), _P(C.c_uint32), _P(C.c_uint64)]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Loads libapgx.so (raises if it has not been built: there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m apgar_amd.build`")
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGNATURES)
+
+
+def check(code: int) -> None:
+    if code != APGX_OK:
+        raise ApgxError(code, lib().apgx_last_error().decode(errors="replace"))
+
+
+def u32p(a: np.ndarray):
+    assert a.dtype == np.uint32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_P(C.c_uint32))
+
+
+def synchronize() -> None:
+    check(lib().apgx_device_synchronize())
+
+
+def device_info() -> DeviceInfo:
+    info = DeviceInfo()
+    check(lib().apgx_device_info_query(C.byref(info)))
+    return info
+
+
+# ----------------------------------------------------------------- generators
+def generate_lattice(width, height, layers, cost_min, cost_max, blocked_ppm=0, keepout_ppm=0,
+                     keepout_min_side=0, keepout_max_side=0, seed=0) -> np.ndarray:
+    """Deterministic cost plane (uint32, 0 = blocked), shape (layers, height, width)."""
+    p = LatticeGenParams(width, height, layers, cost_min, cost_max, blocked_ppm, keepout_ppm,
+                         keepout_min_side, keepout_max_side, 0, seed)
+    out = np.empty(width * height * layers, dtype=np.uint32)
+    check(lib().apgx_generate_lattice(C.byref(p), u32p(out)))
+    return out.reshape(layers, height, width)
+
+
+def generate_pairs(cost: np.ndarray, count: int, seed: int) -> np.ndarray:
+    L, H, W = cost.shape
+    flat = np.ascontiguousarray(cost.reshape(-1), dtype=np.uint32)
+    out = np.empty(2 * count, dtype=np.uint32)
+    check(lib().apgx_generate_pairs(u32p(flat), W, H, L, count, seed, u32p(out)))
+    return out.reshape(count, 2)
+
+
+def generate_nets(cost: np.ndarray, count: int, min_pins: int, max_pins: int, min_side: int,
+                  max_side: int, seed: int):
+    L, H, W = cost.shape
+    flat = np.ascontiguousarray(cost.reshape(-1), dtype=np.uint32)
+    nets = (NetQuery * count)()
+    pins = np.empty(count * max_pins, dtype=np.uint32)
+    total = C.c_uint64(0)
+    check(lib().apgx_generate_nets(u32p(flat), W, H, L, count, min_pins, max_pins, min_side,
+                                   max_side, seed, nets, u32p(pins), C.byref(total)))
+    return nets, pins[: total.value].copy()
+
+
+# --------------------------------------------------------------------- search
+@dataclass
+class NetRoute:
+    net_id: int
+    status: int
+    cost: int
+    rounds: int
+    connections: list  # (pin_index, distance, cells ndarray)
+
+
+class Lattice:
+    """Owns one uploaded cost plane (uint16 or uint32) on the current device."""
+
+    def __init__(self, cost: np.ndarray, via_cost: int = 0, cost_bytes: int | None = None):
+        L, H, W = cost.shape
+        if cost_bytes is None:
+            cost_bytes = 2 if int(cost.max(initial=0)) <= 0xFFFF else 4
+        plane = np.ascontiguousarray(cost, dtype=np.uint16 if cost_bytes == 2 else np.uint32)
+        self.shape = (L, H, W)
+        self.via_cost = via_cost
+        self.cost_bytes = cost_bytes
+        self.desc = LatticeDesc(LATTICE_SCHEMA_VERSION, W, H, L, cost_bytes, via_cost, (C.c_uint32 * 2)())
+        handle = C.c_void_p()
+        check(lib().apgx_lattice_create(C.byref(self.desc), plane.ctypes.data, C.byref(handle)))
+        self.handle = handle
+
+    def close(self):
+        if self.handle:
+            lib().apgx_lattice_release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def route(self, nets, pins: np.ndarray, *, max_rounds=0, tile=(0, 0), rounds_per_sync=0,
+              max_device_bytes=0, use_graph=0, profile_rounds=0):
+        """Routes nets (a ctypes NetQuery array) and returns (routes, stats)."""
+        pins = np.ascontiguousarray(pins, dtype=np.uint32)
+        opts = RouteOptions(max_rounds, tile[0], tile[1], rounds_per_sync, max_device_bytes,
+                            use_graph, profile_rounds, (C.c_uint32 * 2)())
+        search = C.c_void_p()
+        check(lib().apgx_route_nets(self.handle, nets, len(nets), u32p(pins), pins.size,
+                                    C.byref(opts), C.byref(search)))
+        try:
+            nconn, ncell = C.c_uint64(0), C.c_uint64(0)
+            check(lib().apgx_search_sizes(search, C.byref(nconn), C.byref(ncell)))
+            results = (NetResult * max(len(nets), 1))()
+            conns = (Connection * max(nconn.value, 1))()
+            cells = np.empty(max(ncell.value, 1), dtype=np.uint32)
+            check(lib().apgx_search_readback(search, results, conns, nconn.value, u32p(cells),
+                                             ncell.value))
+            stats = RouteStats()
+            check(lib().apgx_search_stats(search, C.byref(stats)))
+        finally:
+            lib().apgx_search_release(search)
+        routes = []
+        for i in range(len(nets)):
+            r = results[i]
+            cs = []
+            for c in range(r.connection_count):
+                k = conns[r.connection_offset + c]
+                cs.append((k.pin_index, k.distance,
+                           cells[k.cell_offset:k.cell_offset + k.cell_count].copy()))
+            routes.append(NetRoute(r.net_id, r.status, r.cost, r.rounds, cs))
+        return routes, stats
+
+    def route_pairs(self, pairs: np.ndarray, window=None, **kw):
+        """Single-pair queries (pairs: (n, 2) source/target cells)."""
+        n = len(pairs)
+        nets = (NetQuery * n)()
+        for i in range(n):
+            nets[i].net_id = i + 1
+            nets[i].pin_offset = 2 * i
+            nets[i].pin_count = 2
+            if window is not None:
+                nets[i].window = Window(*window)
+        return self.route(nets, np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1), **kw)
+
+
+def nets_from_lists(pin_lists, windows=None):
+    """Builds a NetQuery array and flat pin array from python lists."""
+    nets = (NetQuery * len(pin_lists))()
+    flat = []
+    for i, pl in enumerate(pin_lists):
+        nets[i].net_id = i + 1
+        nets[i].pin_offset = len(flat)
+        nets[i].pin_count = len(pl)
+        if windows is not None and windows[i] is not None:
+            nets[i].window = Window(*windows[i])
+        flat.extend(int(p) for p in pl)
+    return nets, np.asarray(flat, dtype=np.uint32)

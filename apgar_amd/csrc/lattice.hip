@@ -1,0 +1,1057 @@
+// lattice.hip — batched exact shortest-path search over dense multi-layer
+// lattices on gfx950 (MI355X).
+//
+// Design (what the reference's heading-aware sweep becomes on CDNA4):
+//  * Every net owns a window-local label plane of packed 32-bit words
+//      word = (distance << 3) | predecessor_direction
+//    Unsigned min over these words is the lexicographic (distance, predecessor)
+//    min.  Direction codes are ordered like the predecessor's cell index
+//    (layer below < south < west < east < north < layer above), so at the fixed
+//    point the low bits name the lowest-index predecessor that achieves the
+//    distance: the canonical predecessor of gpu/cuda/planar_router.cu:1084-1165,
+//    restated for dense lattices.  Distance 0 with code 7 marks a source.
+//  * The window is cut into TX x TY x L tiles.  A workgroup stages one tile plus
+//    a one-cell halo of its four neighbours in LDS, then repeats directional
+//    sweeps (E, W, N, S serial min-plus run scans, then via scans up and down
+//    the layer stack) until the tile no longer changes.  Changed border cells
+//    activate the neighbouring tile for the next round.
+//  * Active tiles of all nets form one device work list per round.  A
+//    persistent round kernel pops items with an atomic head; pushes are deduped
+//    by a per-tile round stamp (atomicMax).  Rounds are separated by kernel
+//    boundaries, which make every label written in round r visible in round
+//    r+1 on all XCDs without fine-grained memory.
+//  * Tiles start uninitialised; a tile's labels are trusted by its neighbours
+//    only from the round after its first visit, so label planes never need a
+//    full-plane initialisation pass.
+//  * After each round a phase kernel takes every net with no queued work (its
+//    labels are the exact distances from its current tree), connects the
+//    nearest unconnected pin along the predecessor chain, re-seeds that path as
+//    sources and re-queues the touched tiles: multi-pin nets keep their labels
+//    between connection steps (labels only decrease), so later steps relax
+//    only what the new tree cells improve.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "apgx.h"
+#include "common.h"
+
+namespace apgx {
+namespace {
+
+constexpr uint32_t kInfD = 0x1FFFFFFFu;        // saturated distance
+constexpr uint32_t kInf = kInfD << 3;          // packed "unreached" word
+constexpr uint32_t kNever = 0xFFFFFFFFu;       // tile never initialised
+constexpr uint32_t kSourceWord = 7u;           // distance 0, no predecessor
+constexpr uint32_t kFromBelow = 0, kFromSouth = 1, kFromWest = 2, kFromEast = 3,
+                   kFromNorth = 4, kFromAbove = 5;
+constexpr uint32_t kSearching = 0xFFu;
+constexpr int kThreads = 256;
+constexpr int kMaxTileCells = 16384;
+constexpr int kKeep = kMaxTileCells / kThreads;  // labels kept in registers per thread
+
+struct NetStatic {
+  uint32_t x0, y0, w, h;
+  uint32_t tiles_x, tiles_y;
+  uint32_t tile_offset;
+  uint32_t pin_offset;
+  uint32_t pin_count;
+  uint32_t conn_offset;
+  uint64_t label_offset;
+};
+
+struct NetState {
+  uint32_t status;
+  uint32_t connected;    // bit per pin
+  uint32_t n_connected;  // connections made
+  uint32_t cost;
+  uint32_t last_push;    // highest round this net queued a tile for
+  uint32_t rounds;
+  uint32_t last_counted;
+  uint32_t pad;
+};
+
+struct DevConnection {
+  uint32_t pin_index;
+  uint32_t distance;
+  uint64_t cell_offset;
+  uint32_t cell_count;
+  uint32_t reserved;
+};
+
+struct SearchArgs {
+  const void* cost;
+  uint32_t W, H, L, via;
+  const NetStatic* nets;
+  NetState* state;
+  uint32_t net_count;
+  const uint32_t* tile_net;
+  const uint32_t* pins;
+  uint32_t* labels;
+  uint32_t* tile_stamp;
+  uint32_t* tile_init;
+  uint32_t* wl[2];
+  uint32_t* counters;  // count[r] at 2r, head[r] at 2r+1
+  unsigned long long* stats;
+  DevConnection* conns;
+  uint32_t* cells;
+  unsigned long long* cell_cursor;
+  unsigned long long cell_capacity;
+  uint32_t round;      // round being executed (round kernel) / next round (phase kernel)
+  uint32_t max_sweeps;
+};
+
+template <typename CostT>
+__device__ __forceinline__ uint32_t LoadCost(const void* cost, uint64_t index) {
+  return static_cast<const CostT*>(cost)[index];
+}
+
+__device__ __forceinline__ void Push(const SearchArgs& a, uint32_t tile, uint32_t net,
+                                     uint32_t round) {
+  if (atomicMax(&a.tile_stamp[tile], round) < round) {
+    const uint32_t slot = atomicAdd(&a.counters[2 * round], 1u);
+    a.wl[round & 1][slot] = tile;
+    atomicMax(&a.state[net].last_push, round);
+  }
+}
+
+__device__ __forceinline__ uint32_t Step(uint32_t carry_d, uint32_t cost, uint32_t code) {
+  return (min(carry_d + cost, kInfD) << 3) | code;
+}
+
+// One tile visit: stage, sweep to local convergence, write back, activate.
+template <int TX, int TY>
+__device__ void ProcessTile(const SearchArgs& a, uint32_t g, uint32_t* s_lab, uint32_t* s_cost,
+                            uint32_t* s_flags, uint32_t cost_bytes) {
+  constexpr int PX = TX + 3;   // odd LDS pitch: row scans are bank-conflict free
+  constexpr int CPX = TX + 1;
+  const int tid = threadIdx.x;
+  const int L = static_cast<int>(a.L);
+  const uint32_t n = a.tile_net[g];
+  const NetStatic ns = a.nets[n];
+  const uint32_t t = g - ns.tile_offset;
+  const uint32_t tx = t % ns.tiles_x, ty = t / ns.tiles_x;
+  const uint32_t bx = tx * TX, by = ty * TY;
+  const int nx = static_cast<int>(min(static_cast<uint32_t>(TX), ns.w - bx));
+  const int ny = static_cast<int>(min(static_cast<uint32_t>(TY), ns.h - by));
+  uint32_t* lab = a.labels + ns.label_offset;
+  const uint64_t w = ns.w, h = ns.h;
+  const bool initd = a.tile_init[g] != kNever;
+  const uint32_t r = a.round;
+  auto ready = [&](uint32_t ntx, uint32_t nty) {
+    const uint32_t v = a.tile_init[ns.tile_offset + nty * ns.tiles_x + ntx];
+    return v != kNever && v < r;
+  };
+  const bool has_w = tx > 0 && ready(tx - 1, ty);
+  const bool has_e = tx + 1 < ns.tiles_x && ready(tx + 1, ty);
+  const bool has_s = ty > 0 && ready(tx, ty - 1);
+  const bool has_n = ty + 1 < ns.tiles_y && ready(tx, ty + 1);
+
+  const int cells = L * TY * TX;
+  uint32_t orig[kKeep];
+#pragma unroll
+  for (int k = 0; k < kKeep; ++k) {
+    const int i = tid + k * kThreads;
+    orig[k] = kInf;
+    if (i < cells) {
+      const int x = i % TX, y = (i / TX) % TY, l = i / (TX * TY);
+      uint32_t c = 0, v = kInf;
+      if (x < nx && y < ny) {
+        const uint64_t gidx =
+            (static_cast<uint64_t>(l) * a.H + ns.y0 + by + y) * a.W + ns.x0 + bx + x;
+        c = cost_bytes == 2 ? LoadCost<uint16_t>(a.cost, gidx) : LoadCost<uint32_t>(a.cost, gidx);
+        if (initd) v = lab[(l * h + by + y) * w + bx + x];
+      }
+      s_cost[(l * TY + y) * CPX + x] = c ? c : kInfD;
+      s_lab[(l * (TY + 2) + y + 1) * PX + x + 1] = v;
+      orig[k] = v;
+    }
+  }
+  for (int i = tid; i < L * TY; i += kThreads) {
+    const int y = i % TY, l = i / TY;
+    uint32_t vw = kInf, ve = kInf;
+    if (y < ny) {
+      const uint64_t row = (l * h + by + y) * w;
+      if (has_w) vw = lab[row + bx - 1];
+      if (has_e) ve = lab[row + bx + TX];
+    }
+    s_lab[(l * (TY + 2) + y + 1) * PX] = vw;
+    s_lab[(l * (TY + 2) + y + 1) * PX + TX + 1] = ve;
+  }
+  for (int i = tid; i < L * TX; i += kThreads) {
+    const int x = i % TX, l = i / TX;
+    uint32_t vs = kInf, vn = kInf;
+    if (x < nx) {
+      if (has_s) vs = lab[(l * h + by - 1) * w + bx + x];
+      if (has_n) vn = lab[(l * h + by + TY) * w + bx + x];
+    }
+    s_lab[(l * (TY + 2)) * PX + x + 1] = vs;
+    s_lab[(l * (TY + 2) + TY + 1) * PX + x + 1] = vn;
+  }
+  __syncthreads();
+
+  uint32_t sweeps = 0;
+  bool converged = false;
+  while (sweeps < a.max_sweeps) {
+    bool changed = false;
+    // Row scans: east (+x) then west (-x), one row per thread.
+    for (int i = tid; i < L * TY; i += kThreads) {
+      const int y = i % TY, l = i / TY;
+      uint32_t* row = s_lab + (l * (TY + 2) + y + 1) * PX;
+      const uint32_t* crow = s_cost + (l * TY + y) * CPX;
+      uint32_t carry = row[0] >> 3;
+#pragma unroll 8
+      for (int x = 0; x < TX; ++x) {
+        const uint32_t v = row[x + 1];
+        const uint32_t nv = min(v, Step(carry, crow[x], kFromWest));
+        changed |= nv != v;
+        row[x + 1] = nv;
+        carry = nv >> 3;
+      }
+      carry = row[TX + 1] >> 3;
+#pragma unroll 8
+      for (int x = TX - 1; x >= 0; --x) {
+        const uint32_t v = row[x + 1];
+        const uint32_t nv = min(v, Step(carry, crow[x], kFromEast));
+        changed |= nv != v;
+        row[x + 1] = nv;
+        carry = nv >> 3;
+      }
+    }
+    __syncthreads();
+    // Column scans: north (+y) then south (-y), one column per thread.
+    for (int i = tid; i < L * TX; i += kThreads) {
+      const int x = i % TX, l = i / TX;
+      uint32_t* col = s_lab + l * (TY + 2) * PX + x + 1;
+      const uint32_t* ccol = s_cost + l * TY * CPX + x;
+      uint32_t carry = col[0] >> 3;
+#pragma unroll 8
+      for (int y = 0; y < TY; ++y) {
+        const uint32_t v = col[(y + 1) * PX];
+        const uint32_t nv = min(v, Step(carry, ccol[y * CPX], kFromSouth));
+        changed |= nv != v;
+        col[(y + 1) * PX] = nv;
+        carry = nv >> 3;
+      }
+      carry = col[(TY + 1) * PX] >> 3;
+#pragma unroll 8
+      for (int y = TY - 1; y >= 0; --y) {
+        const uint32_t v = col[(y + 1) * PX];
+        const uint32_t nv = min(v, Step(carry, ccol[y * CPX], kFromNorth));
+        changed |= nv != v;
+        col[(y + 1) * PX] = nv;
+        carry = nv >> 3;
+      }
+    }
+    // Via scans up and down each (x, y) stack.
+    if (L > 1 && a.via != 0) {
+      __syncthreads();
+      for (int i = tid; i < TX * TY; i += kThreads) {
+        const int x = i % TX, y = i / TX;
+        uint32_t* stack = s_lab + (y + 1) * PX + x + 1;
+        const uint32_t* cstack = s_cost + y * CPX + x;
+        constexpr int kLab = (TY + 2) * PX, kCost = TY * CPX;
+        uint32_t carry = stack[0] >> 3;
+        for (int l = 1; l < L; ++l) {
+          const uint32_t v = stack[l * kLab];
+          const uint32_t c = cstack[l * kCost] >= kInfD ? kInfD : a.via;
+          const uint32_t nv = min(v, Step(carry, c, kFromBelow));
+          changed |= nv != v;
+          stack[l * kLab] = nv;
+          carry = nv >> 3;
+        }
+        for (int l = L - 2; l >= 0; --l) {
+          const uint32_t v = stack[l * kLab];
+          const uint32_t c = cstack[l * kCost] >= kInfD ? kInfD : a.via;
+          const uint32_t nv = min(v, Step(carry, c, kFromAbove));
+          changed |= nv != v;
+          stack[l * kLab] = nv;
+          carry = nv >> 3;
+        }
+      }
+    }
+    ++sweeps;
+    if (!__syncthreads_or(changed)) {
+      converged = true;
+      break;
+    }
+  }
+
+  uint32_t flags = 0;
+#pragma unroll
+  for (int k = 0; k < kKeep; ++k) {
+    const int i = tid + k * kThreads;
+    if (i < cells) {
+      const int x = i % TX, y = (i / TX) % TY, l = i / (TX * TY);
+      if (x < nx && y < ny) {
+        const uint32_t v = s_lab[(l * (TY + 2) + y + 1) * PX + x + 1];
+        if (v != orig[k] || !initd) lab[(l * h + by + y) * w + bx + x] = v;
+        if ((v >> 3) != (orig[k] >> 3)) {
+          flags |= (x == 0 ? 1u : 0u) | (x == nx - 1 ? 2u : 0u) | (y == 0 ? 4u : 0u) |
+                   (y == ny - 1 ? 8u : 0u);
+        }
+      }
+    }
+  }
+  if (flags) atomicOr(s_flags, flags);
+  __syncthreads();
+  if (tid == 0) {
+    flags = *s_flags;
+    if (!initd) a.tile_init[g] = r;
+    const uint32_t next = r + 1;
+    if ((flags & 1u) && tx > 0) Push(a, g - 1, n, next);
+    if ((flags & 2u) && tx + 1 < ns.tiles_x) Push(a, g + 1, n, next);
+    if ((flags & 4u) && ty > 0) Push(a, g - ns.tiles_x, n, next);
+    if ((flags & 8u) && ty + 1 < ns.tiles_y) Push(a, g + ns.tiles_x, n, next);
+    if (!converged) Push(a, g, n, next);
+    if (atomicMax(&a.state[n].last_counted, r) < r) atomicAdd(&a.state[n].rounds, 1u);
+    atomicAdd(&a.stats[0], 1ull);
+    atomicAdd(&a.stats[1], static_cast<unsigned long long>(nx) * ny * L);
+    atomicAdd(&a.stats[2], static_cast<unsigned long long>(sweeps));
+    atomicAdd(&a.stats[3], static_cast<unsigned long long>(sweeps) * nx * ny * L);
+  }
+}
+
+template <int TX, int TY>
+__global__ __launch_bounds__(kThreads) void RoundKernel(SearchArgs a, uint32_t cost_bytes) {
+  extern __shared__ uint32_t smem[];
+  constexpr int PX = TX + 3;
+  uint32_t* s_lab = smem;
+  uint32_t* s_cost = smem + static_cast<int>(a.L) * (TY + 2) * PX;
+  __shared__ uint32_t s_item;
+  __shared__ uint32_t s_flags;
+  const uint32_t count = a.counters[2 * a.round];
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const uint32_t i = atomicAdd(&a.counters[2 * a.round + 1], 1u);
+      s_item = i < count ? a.wl[a.round & 1][i] : kNever;
+      s_flags = 0;
+    }
+    __syncthreads();
+    const uint32_t g = s_item;
+    if (g == kNever) break;
+    ProcessTile<TX, TY>(a, g, s_lab, s_cost, &s_flags, cost_bytes);
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ uint64_t LocalIndex(const NetStatic& ns, uint32_t W, uint32_t H,
+                                               uint32_t cell) {
+  const uint32_t x = cell % W, y = (cell / W) % H, l = cell / (W * H);
+  return (static_cast<uint64_t>(l) * ns.h + (y - ns.y0)) * ns.w + (x - ns.x0);
+}
+
+template <int TX, int TY>
+__device__ __forceinline__ uint32_t TileOf(const NetStatic& ns, uint32_t W, uint32_t H,
+                                           uint32_t cell) {
+  const uint32_t x = cell % W - ns.x0, y = (cell / W) % H - ns.y0;
+  return ns.tile_offset + (y / TY) * ns.tiles_x + x / TX;
+}
+
+// Seeds pin 0 of every searchable net: initialises its tile (all unreached,
+// the pin a source) and queues it for round 1.  One workgroup per net.
+template <int TX, int TY>
+__global__ __launch_bounds__(kThreads) void SeedKernel(SearchArgs a) {
+  const uint32_t n = blockIdx.x;
+  if (n >= a.net_count || a.state[n].status != kSearching) return;
+  const NetStatic ns = a.nets[n];
+  const uint32_t pin = a.pins[ns.pin_offset];
+  const uint32_t g = TileOf<TX, TY>(ns, a.W, a.H, pin);
+  const uint32_t t = g - ns.tile_offset;
+  const uint32_t bx = (t % ns.tiles_x) * TX, by = (t / ns.tiles_x) * TY;
+  const uint32_t nx = min(static_cast<uint32_t>(TX), ns.w - bx);
+  const uint32_t ny = min(static_cast<uint32_t>(TY), ns.h - by);
+  const uint64_t pin_local = LocalIndex(ns, a.W, a.H, pin);
+  uint32_t* lab = a.labels + ns.label_offset;
+  for (uint32_t i = threadIdx.x; i < a.L * TY * TX; i += kThreads) {
+    const uint32_t x = i % TX, y = (i / TX) % TY, l = i / (TX * TY);
+    if (x < nx && y < ny) {
+      const uint64_t idx = (static_cast<uint64_t>(l) * ns.h + by + y) * ns.w + bx + x;
+      lab[idx] = idx == pin_local ? kSourceWord : kInf;
+    }
+  }
+  if (threadIdx.x == 0) {
+    a.tile_init[g] = 0;
+    a.state[n].connected = 1u;
+    Push(a, g, n, 1u);
+  }
+}
+
+__device__ __forceinline__ uint32_t Neighbour(uint32_t cell, uint32_t code, uint32_t W,
+                                              uint32_t plane) {
+  switch (code) {
+    case kFromBelow: return cell - plane;
+    case kFromSouth: return cell - W;
+    case kFromWest: return cell - 1;
+    case kFromEast: return cell + 1;
+    case kFromNorth: return cell + W;
+    default: return cell + plane;
+  }
+}
+
+// Runs after every round.  Nets without queued work hold exact distances from
+// their current tree: connect the nearest unconnected pin (ties: lowest pin
+// index) along the predecessor chain, make that path part of the tree and
+// queue its tiles for the next round.  One thread per net.
+template <int TX, int TY>
+__global__ void PhaseKernel(SearchArgs a) {
+  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.net_count) return;
+  NetState& st = a.state[n];
+  if (st.status != kSearching || st.last_push >= a.round) return;
+  const NetStatic ns = a.nets[n];
+  uint32_t* lab = a.labels + ns.label_offset;
+  const uint32_t plane = a.W * a.H;
+  uint32_t best = kInfD, best_pin = kNever;
+  for (uint32_t p = 0; p < ns.pin_count; ++p) {
+    if (st.connected & (1u << p)) continue;
+    const uint32_t cell = a.pins[ns.pin_offset + p];
+    const uint32_t g = TileOf<TX, TY>(ns, a.W, a.H, cell);
+    const uint32_t d = a.tile_init[g] != kNever ? lab[LocalIndex(ns, a.W, a.H, cell)] >> 3 : kInfD;
+    if (d < best) {
+      best = d;
+      best_pin = p;
+    }
+  }
+  if (best_pin == kNever) {
+    st.status = APGX_NET_DISCONNECTED;
+    return;
+  }
+  const uint32_t start = a.pins[ns.pin_offset + best_pin];
+  // Pass 1: length of the chain back to a tree cell (distance 0).  Distances
+  // strictly decrease along the chain; the step bound and window check only
+  // guard against a corrupted word.
+  const uint64_t max_len = static_cast<uint64_t>(ns.w) * ns.h * a.L;
+  uint32_t len = 1;
+  for (uint32_t v = start;;) {
+    const uint32_t word = lab[LocalIndex(ns, a.W, a.H, v)];
+    if ((word >> 3) == 0) break;
+    const uint32_t code = word & 7u;
+    const uint32_t x = v % a.W, y = (v / a.W) % a.H, l = v / plane;
+    const bool ok = code <= kFromAbove && len < max_len &&
+                    (code != kFromBelow || l > 0) && (code != kFromAbove || l + 1 < a.L) &&
+                    (code != kFromWest || x > ns.x0) && (code != kFromEast || x + 1 < ns.x0 + ns.w) &&
+                    (code != kFromSouth || y > ns.y0) && (code != kFromNorth || y + 1 < ns.y0 + ns.h);
+    if (!ok) {
+      st.status = APGX_NET_INVALID;
+      return;
+    }
+    v = Neighbour(v, code, a.W, plane);
+    ++len;
+  }
+  const unsigned long long offset = atomicAdd(a.cell_cursor, static_cast<unsigned long long>(len));
+  if (offset + len > a.cell_capacity) {
+    st.status = APGX_NET_INVALID;
+    return;
+  }
+  // Pass 2: record the path and turn it into sources for the next step.
+  const bool last = st.n_connected + 2 == ns.pin_count;
+  uint32_t pushed = kNever;
+  uint32_t v = start;
+  for (uint32_t i = 0; i < len; ++i) {
+    const uint64_t li = LocalIndex(ns, a.W, a.H, v);
+    const uint32_t word = lab[li];
+    a.cells[offset + i] = v;
+    if (!last && (word >> 3) != 0) {
+      lab[li] = kSourceWord;
+      const uint32_t g = TileOf<TX, TY>(ns, a.W, a.H, v);
+      if (g != pushed) {
+        Push(a, g, n, a.round);
+        pushed = g;
+      }
+    }
+    v = Neighbour(v, word & 7u, a.W, plane);
+  }
+  DevConnection& c = a.conns[ns.conn_offset + st.n_connected];
+  c.pin_index = best_pin;
+  c.distance = best;
+  c.cell_offset = offset;
+  c.cell_count = len;
+  c.reserved = 0;
+  st.cost += best;
+  st.connected |= 1u << best_pin;
+  st.n_connected += 1;
+  if (last) st.status = APGX_NET_REACHED;
+}
+
+struct TileShape {
+  int tx, ty;
+};
+
+size_t TileLdsBytes(TileShape s, uint32_t layers) {
+  return static_cast<size_t>(layers) * ((s.ty + 2) * (s.tx + 3) + s.ty * (s.tx + 1)) * 4;
+}
+
+struct Kernels {
+  const void* round;
+  const void* seed;
+  const void* phase;
+};
+
+template <int TX, int TY>
+Kernels KernelsFor() {
+  return Kernels{reinterpret_cast<const void*>(&RoundKernel<TX, TY>),
+                 reinterpret_cast<const void*>(&SeedKernel<TX, TY>),
+                 reinterpret_cast<const void*>(&PhaseKernel<TX, TY>)};
+}
+
+bool KernelsForShape(TileShape s, Kernels* k) {
+  if (s.tx == 64 && s.ty == 64) { *k = KernelsFor<64, 64>(); return true; }
+  if (s.tx == 64 && s.ty == 32) { *k = KernelsFor<64, 32>(); return true; }
+  if (s.tx == 32 && s.ty == 32) { *k = KernelsFor<32, 32>(); return true; }
+  if (s.tx == 32 && s.ty == 16) { *k = KernelsFor<32, 16>(); return true; }
+  return false;
+}
+
+}  // namespace
+}  // namespace apgx
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+struct apgx_lattice {
+  apgx_lattice_desc desc{};
+  void* d_cost = nullptr;
+  int device = 0;
+  uint32_t min_edge = 0;      // smallest edge weight (free cell cost or via)
+  uint64_t free_cells = 0;
+  std::mutex cache_mutex;
+  void* cached_labels = nullptr;  // reusable label pool (returned by searches)
+  size_t cached_label_bytes = 0;
+  std::vector<uint8_t> host_costs;  // host copy for request preflight
+  uint32_t HostCost(uint64_t cell) const {
+    return desc.cost_bytes == 2 ? reinterpret_cast<const uint16_t*>(host_costs.data())[cell]
+                                : reinterpret_cast<const uint32_t*>(host_costs.data())[cell];
+  }
+};
+
+struct apgx_search {
+  apgx_lattice* lattice = nullptr;
+  uint32_t net_count = 0;
+  std::vector<apgx_net_query> nets;
+  std::vector<uint32_t> conn_offset;
+  apgx::TileShape shape{};
+  void* d_labels = nullptr;
+  size_t label_bytes = 0;
+  void* d_block = nullptr;   // every other device buffer, one allocation
+  apgx::NetState* d_state = nullptr;
+  apgx::DevConnection* d_conns = nullptr;
+  uint32_t* d_cells = nullptr;
+  unsigned long long* d_cursor = nullptr;
+  uint64_t total_connections = 0;
+  apgx_route_stats stats{};
+  // host copies filled at the end of the search
+  std::vector<apgx::NetState> state;
+  std::vector<apgx::DevConnection> conns;
+  std::vector<uint32_t> cells;
+  bool read = false;
+};
+
+namespace apgx {
+namespace {
+
+int ValidateDesc(const apgx_lattice_desc* d) {
+  if (d == nullptr) return Fail(APGX_E_INTERNAL_INVARIANT, "lattice descriptor is null");
+  if (d->schema_version != APGX_LATTICE_SCHEMA_VERSION)
+    return Fail(APGX_E_UNSUPPORTED, "lattice schema version is not supported");
+  if (d->width == 0 || d->height == 0 || d->layers == 0 || d->layers > APGX_MAX_LAYERS)
+    return Fail(APGX_E_INTERNAL_INVARIANT, "lattice dimensions are out of range");
+  if (d->cost_bytes != 2 && d->cost_bytes != 4)
+    return Fail(APGX_E_INTERNAL_INVARIANT, "lattice cost_bytes must be 2 or 4");
+  if (d->via_cost > APGX_MAX_VIA_COST)
+    return Fail(APGX_E_INTERNAL_INVARIANT, "via cost exceeds APGX_MAX_VIA_COST");
+  const unsigned __int128 cells = static_cast<unsigned __int128>(d->width) * d->height * d->layers;
+  if (cells >= 0xFFFFFFFFull)
+    return Fail(APGX_E_RESOURCE_EXHAUSTED, "lattice cell count exceeds the uint32 index space");
+  return APGX_OK;
+}
+
+int FinishLattice(const apgx_lattice_desc* desc, const void* host_costs, apgx_lattice* lat) {
+  const uint64_t cells = static_cast<uint64_t>(desc->width) * desc->height * desc->layers;
+  uint32_t min_cost = 0xFFFFFFFFu, max_cost = 0;
+  uint64_t free_cells = 0;
+  for (uint64_t c = 0; c < cells; ++c) {
+    const uint32_t v = desc->cost_bytes == 2 ? static_cast<const uint16_t*>(host_costs)[c]
+                                             : static_cast<const uint32_t*>(host_costs)[c];
+    if (v == 0) continue;
+    if (v > APGX_MAX_CELL_COST)
+      return Fail(APGX_E_INTERNAL_INVARIANT, "cell cost exceeds APGX_MAX_CELL_COST");
+    ++free_cells;
+    min_cost = std::min(min_cost, v);
+    max_cost = std::max(max_cost, v);
+  }
+  lat->desc = *desc;
+  lat->free_cells = free_cells;
+  const uint8_t* bytes = static_cast<const uint8_t*>(host_costs);
+  lat->host_costs.assign(bytes, bytes + cells * desc->cost_bytes);
+  lat->min_edge = min_cost;
+  if (desc->via_cost != 0 && desc->layers > 1) lat->min_edge = std::min(min_cost, desc->via_cost);
+  return APGX_OK;
+}
+
+}  // namespace
+}  // namespace apgx
+
+extern "C" int apgx_lattice_create(const apgx_lattice_desc* desc, const void* host_costs,
+                                   apgx_lattice** out) {
+  using namespace apgx;
+  if (out == nullptr || host_costs == nullptr)
+    return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_lattice_create: null argument");
+  *out = nullptr;
+  if (int s = ValidateDesc(desc)) return s;
+  auto* lat = new (std::nothrow) apgx_lattice();
+  if (lat == nullptr) return Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_lattice_create: host allocation");
+  if (int s = FinishLattice(desc, host_costs, lat)) {
+    delete lat;
+    return s;
+  }
+  const size_t bytes = static_cast<size_t>(desc->width) * desc->height * desc->layers * desc->cost_bytes;
+  hipError_t e = hipGetDevice(&lat->device);
+  if (e == hipSuccess) e = hipMalloc(&lat->d_cost, bytes);
+  if (e == hipSuccess) e = hipMemcpy(lat->d_cost, host_costs, bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (lat->d_cost) (void)hipFree(lat->d_cost);
+    delete lat;
+    return HipFail(e, "lattice upload");
+  }
+  *out = lat;
+  ClearError();
+  return APGX_OK;
+}
+
+extern "C" int apgx_lattice_create_from_device(const apgx_lattice_desc* desc,
+                                               const void* device_costs, apgx_lattice** out) {
+  using namespace apgx;
+  if (out == nullptr || device_costs == nullptr)
+    return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_lattice_create_from_device: null argument");
+  if (int s = ValidateDesc(desc)) return s;
+  const size_t bytes = static_cast<size_t>(desc->width) * desc->height * desc->layers * desc->cost_bytes;
+  std::vector<uint8_t> host(bytes);
+  APGX_HIP(hipMemcpy(host.data(), device_costs, bytes, hipMemcpyDeviceToHost), "lattice validation copy");
+  return apgx_lattice_create(desc, host.data(), out);
+}
+
+extern "C" void apgx_lattice_release(apgx_lattice* lat) {
+  if (lat == nullptr) return;
+  if (lat->d_cost) (void)hipFree(lat->d_cost);
+  if (lat->cached_labels) (void)hipFree(lat->cached_labels);
+  delete lat;
+}
+
+extern "C" const void* apgx_lattice_device_costs(const apgx_lattice* lat) {
+  return lat ? lat->d_cost : nullptr;
+}
+
+namespace apgx {
+namespace {
+
+TileShape DefaultShape(const apgx_lattice_desc& d, size_t lds_limit) {
+  const TileShape candidates[] = {{64, 64}, {64, 32}, {32, 32}, {32, 16}};
+  // Prefer the largest tile that fits the register-resident cell budget and
+  // leaves room for two workgroups per CU.
+  for (const TileShape& s : candidates) {
+    if (static_cast<uint64_t>(s.tx) * s.ty * d.layers > kMaxTileCells) continue;
+    if (TileLdsBytes(s, d.layers) * 2 > lds_limit && !(s.tx == 32 && s.ty == 16)) continue;
+    return s;
+  }
+  return TileShape{32, 16};
+}
+
+int Launch(const void* kernel, dim3 grid, dim3 block, size_t lds, hipStream_t stream,
+           void** args, const char* op) {
+  APGX_HIP(hipLaunchKernel(kernel, grid, block, args, lds, stream), op);
+  return APGX_OK;
+}
+
+}  // namespace
+}  // namespace apgx
+
+extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, uint32_t net_count,
+                               const uint32_t* pins, uint64_t pin_count,
+                               const apgx_route_options* options, apgx_search** out) {
+  using namespace apgx;
+  const auto wall_start = std::chrono::steady_clock::now();
+  if (lat == nullptr || out == nullptr || (net_count != 0 && (nets == nullptr || pins == nullptr)))
+    return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_route_nets: null argument");
+  *out = nullptr;
+  int device = 0;
+  APGX_HIP(hipGetDevice(&device), "hipGetDevice");
+  if (device != lat->device)
+    return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_route_nets: current device differs from the upload device");
+  apgx_route_options opt{};
+  if (options) opt = *options;
+  const uint32_t max_rounds = opt.max_rounds ? opt.max_rounds : 100000u;
+  const apgx_lattice_desc& d = lat->desc;
+  hipDeviceProp_t prop;
+  APGX_HIP(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+  const size_t lds_limit = prop.sharedMemPerBlock;
+  TileShape shape = DefaultShape(d, lds_limit);
+  if (opt.tile_w || opt.tile_h) shape = TileShape{static_cast<int>(opt.tile_w), static_cast<int>(opt.tile_h)};
+  Kernels k{};
+  if (!KernelsForShape(shape, &k))
+    return Fail(APGX_E_UNSUPPORTED, "apgx_route_nets: unsupported tile shape");
+  if (static_cast<uint64_t>(shape.tx) * shape.ty * d.layers > kMaxTileCells)
+    return Fail(APGX_E_UNSUPPORTED, "apgx_route_nets: tile holds more cells than a workgroup keeps");
+  const size_t lds = TileLdsBytes(shape, d.layers);
+  if (lds > lds_limit)
+    return Fail(APGX_E_UNSUPPORTED, "apgx_route_nets: tile does not fit in LDS");
+
+  auto* s = new (std::nothrow) apgx_search();
+  if (s == nullptr) return Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_route_nets: host allocation");
+  s->lattice = lat;
+  s->net_count = net_count;
+  s->nets.assign(nets, nets + net_count);
+  s->shape = shape;
+
+  // Host preflight: windows, pins, offsets.  Invalid nets are reported, not searched.
+  std::vector<NetStatic> ns(net_count);
+  std::vector<NetState> st(net_count);
+  std::vector<uint32_t> tile_net;
+  uint64_t label_words = 0, tiles = 0, conns = 0, window_cells_total = 0;
+  const uint64_t plane = static_cast<uint64_t>(d.width) * d.height;
+  for (uint32_t i = 0; i < net_count; ++i) {
+    const apgx_net_query& q = nets[i];
+    NetStatic& n = ns[i];
+    std::memset(&n, 0, sizeof(n));
+    std::memset(&st[i], 0, sizeof(st[i]));
+    st[i].status = kSearching;
+    apgx_window win = q.window;
+    if (win.width == 0 && win.height == 0) win = apgx_window{0, 0, d.width, d.height};
+    bool ok = win.width > 0 && win.height > 0 && win.x0 + static_cast<uint64_t>(win.width) <= d.width &&
+              win.y0 + static_cast<uint64_t>(win.height) <= d.height && q.pin_count >= 2 &&
+              q.pin_count <= APGX_MAX_NET_PINS &&
+              static_cast<uint64_t>(q.pin_offset) + q.pin_count <= pin_count;
+    if (ok) {
+      for (uint32_t p = 0; p < q.pin_count && ok; ++p) {
+        const uint32_t c = pins[q.pin_offset + p];
+        if (c >= plane * d.layers) { ok = false; break; }
+        const uint32_t x = static_cast<uint32_t>(c % d.width), y = static_cast<uint32_t>((c / d.width) % d.height);
+        ok = x >= win.x0 && x < win.x0 + win.width && y >= win.y0 && y < win.y0 + win.height;
+        for (uint32_t p2 = 0; p2 < p && ok; ++p2) ok = pins[q.pin_offset + p2] != c;
+      }
+    }
+    n.x0 = win.x0; n.y0 = win.y0; n.w = win.width; n.h = win.height;
+    n.pin_offset = q.pin_offset; n.pin_count = q.pin_count;
+    n.conn_offset = static_cast<uint32_t>(conns);
+    if (!ok) {
+      st[i].status = APGX_NET_INVALID;
+      continue;
+    }
+    n.tiles_x = (n.w + shape.tx - 1) / shape.tx;
+    n.tiles_y = (n.h + shape.ty - 1) / shape.ty;
+    n.tile_offset = static_cast<uint32_t>(tiles);
+    n.label_offset = label_words;
+    const uint64_t nt = static_cast<uint64_t>(n.tiles_x) * n.tiles_y;
+    tile_net.insert(tile_net.end(), nt, i);
+    tiles += nt;
+    label_words += static_cast<uint64_t>(n.w) * n.h * d.layers;
+    window_cells_total += static_cast<uint64_t>(n.w) * n.h * d.layers;
+    conns += q.pin_count - 1;
+  }
+  if (tiles >= 0xFFFFFFF0ull) {
+    delete s;
+    return Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_route_nets: tile count exceeds the uint32 index space");
+  }
+  s->total_connections = conns;
+  const uint64_t cell_capacity = std::max<uint64_t>(1u << 20, std::min<uint64_t>(window_cells_total, 1ull << 28));
+  // Device layout: one block for everything except the label pool.
+  const uint64_t counter_words = 2ull * (static_cast<uint64_t>(max_rounds) + 2);
+  struct Part { size_t bytes; size_t offset; };
+  auto align = [](size_t v) { return (v + 255) & ~static_cast<size_t>(255); };
+  Part p_nets{sizeof(NetStatic) * std::max<uint32_t>(net_count, 1), 0};
+  Part p_state{sizeof(NetState) * std::max<uint32_t>(net_count, 1), 0};
+  Part p_tile_net{4 * std::max<uint64_t>(tiles, 1), 0};
+  Part p_stamp{4 * std::max<uint64_t>(tiles, 1), 0};
+  Part p_init{4 * std::max<uint64_t>(tiles, 1), 0};
+  Part p_wl0{4 * std::max<uint64_t>(tiles, 1), 0};
+  Part p_wl1{4 * std::max<uint64_t>(tiles, 1), 0};
+  Part p_counters{4 * counter_words, 0};
+  Part p_stats{8 * 8, 0};
+  Part p_pins{4 * std::max<uint64_t>(pin_count, 1), 0};
+  Part p_conns{sizeof(DevConnection) * std::max<uint64_t>(conns, 1), 0};
+  Part p_cursor{8, 0};
+  Part p_cells{4 * cell_capacity, 0};
+  Part* parts[] = {&p_nets, &p_state, &p_tile_net, &p_stamp, &p_init, &p_wl0, &p_wl1,
+                   &p_counters, &p_stats, &p_pins, &p_conns, &p_cursor, &p_cells};
+  size_t total = 0;
+  for (Part* p : parts) { p->offset = total; total += align(p->bytes); }
+  const size_t label_bytes = static_cast<size_t>(std::max<uint64_t>(label_words, 1)) * 4;
+  if (opt.max_device_bytes && total + label_bytes > opt.max_device_bytes) {
+    delete s;
+    return Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_route_nets: workspace exceeds max_device_bytes");
+  }
+  hipError_t e = hipMalloc(&s->d_block, total);
+  if (e != hipSuccess) { delete s; return HipFail(e, "search workspace allocation"); }
+  {
+    std::lock_guard<std::mutex> lock(lat->cache_mutex);
+    if (lat->cached_labels && lat->cached_label_bytes >= label_bytes) {
+      s->d_labels = lat->cached_labels;
+      s->label_bytes = lat->cached_label_bytes;
+      lat->cached_labels = nullptr;
+      lat->cached_label_bytes = 0;
+    }
+  }
+  if (s->d_labels == nullptr) {
+    e = hipMalloc(&s->d_labels, label_bytes);
+    if (e != hipSuccess) { (void)hipFree(s->d_block); s->d_block = nullptr; delete s; return HipFail(e, "label pool allocation"); }
+    s->label_bytes = label_bytes;
+  }
+  auto at = [&](const Part& p) { return static_cast<char*>(s->d_block) + p.offset; };
+  SearchArgs a{};
+  a.cost = lat->d_cost;
+  a.W = d.width; a.H = d.height; a.L = d.layers; a.via = d.layers > 1 ? d.via_cost : 0;
+  a.nets = reinterpret_cast<NetStatic*>(at(p_nets));
+  a.state = reinterpret_cast<NetState*>(at(p_state));
+  a.net_count = net_count;
+  a.tile_net = reinterpret_cast<uint32_t*>(at(p_tile_net));
+  a.pins = reinterpret_cast<uint32_t*>(at(p_pins));
+  a.labels = static_cast<uint32_t*>(s->d_labels);
+  a.tile_stamp = reinterpret_cast<uint32_t*>(at(p_stamp));
+  a.tile_init = reinterpret_cast<uint32_t*>(at(p_init));
+  a.wl[0] = reinterpret_cast<uint32_t*>(at(p_wl0));
+  a.wl[1] = reinterpret_cast<uint32_t*>(at(p_wl1));
+  a.counters = reinterpret_cast<uint32_t*>(at(p_counters));
+  a.stats = reinterpret_cast<unsigned long long*>(at(p_stats));
+  a.conns = reinterpret_cast<DevConnection*>(at(p_conns));
+  a.cells = reinterpret_cast<uint32_t*>(at(p_cells));
+  a.cell_cursor = reinterpret_cast<unsigned long long*>(at(p_cursor));
+  a.cell_capacity = cell_capacity;
+  a.max_sweeps = 64;
+  s->d_state = a.state;
+  s->d_conns = a.conns;
+  s->d_cells = a.cells;
+  s->d_cursor = a.cell_cursor;
+
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int status = APGX_OK;
+  auto check = [&](hipError_t err, const char* op) {
+    if (status == APGX_OK && err != hipSuccess) status = HipFail(err, op);
+    return status == APGX_OK;
+  };
+  check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+  check(hipEventCreate(&ev0), "hipEventCreate");
+  check(hipEventCreate(&ev1), "hipEventCreate");
+  check(hipMemcpyAsync(const_cast<NetStatic*>(a.nets), ns.data(), sizeof(NetStatic) * net_count, hipMemcpyHostToDevice, stream), "net upload");
+  check(hipMemcpyAsync(a.state, st.data(), sizeof(NetState) * net_count, hipMemcpyHostToDevice, stream), "net state upload");
+  if (!tile_net.empty())
+    check(hipMemcpyAsync(const_cast<uint32_t*>(a.tile_net), tile_net.data(), 4 * tile_net.size(), hipMemcpyHostToDevice, stream), "tile map upload");
+  if (pin_count)
+    check(hipMemcpyAsync(const_cast<uint32_t*>(a.pins), pins, 4 * pin_count, hipMemcpyHostToDevice, stream), "pin upload");
+  check(hipMemsetAsync(a.tile_stamp, 0, 4 * std::max<uint64_t>(tiles, 1), stream), "stamp clear");
+  check(hipMemsetAsync(a.tile_init, 0xFF, 4 * std::max<uint64_t>(tiles, 1), stream), "init clear");
+  check(hipMemsetAsync(a.counters, 0, 4 * counter_words, stream), "counter clear");
+  check(hipMemsetAsync(a.stats, 0, 64, stream), "stats clear");
+  check(hipMemsetAsync(a.cell_cursor, 0, 8, stream), "cursor clear");
+  if (status == APGX_OK && lds > 65536)
+    check(hipFuncSetAttribute(k.round, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)), "hipFuncSetAttribute");
+
+  // Pins must be free cells (checked on the host copy of the plane).
+  if (status == APGX_OK && pin_count) {
+    for (uint32_t i = 0; i < net_count; ++i) {
+      if (st[i].status != kSearching) continue;
+      for (uint32_t p = 0; p < ns[i].pin_count; ++p) {
+        if (lat->HostCost(pins[ns[i].pin_offset + p]) == 0) { st[i].status = APGX_NET_INVALID; break; }
+      }
+    }
+    check(hipMemcpyAsync(a.state, st.data(), sizeof(NetState) * net_count, hipMemcpyHostToDevice, stream), "net state upload");
+  }
+
+  int occupancy = 1;
+  if (status == APGX_OK)
+    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occupancy, k.round, kThreads, lds), "occupancy query");
+  occupancy = std::max(occupancy, 1);
+  const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(
+      static_cast<uint64_t>(prop.multiProcessorCount) * occupancy, std::max<uint64_t>(tiles, 1))));
+  const uint32_t chunk = opt.rounds_per_sync ? opt.rounds_per_sync : 8;
+  uint32_t r = 1, syncs = 0;
+  std::vector<hipEvent_t> round_events;
+  if (status == APGX_OK && net_count) {
+    check(hipEventRecord(ev0, stream), "hipEventRecord");
+    a.round = 0;
+    {
+      void* args[] = {&a};
+      if (status == APGX_OK) status = Launch(k.seed, dim3(net_count), dim3(kThreads), 0, stream, args, "seed launch");
+    }
+    bool done = false;
+    while (status == APGX_OK && !done) {
+      // Rounds 1..max_rounds only: the counter array holds max_rounds + 2 rounds.
+      const uint32_t this_chunk = std::min(chunk, max_rounds + 1 - r);
+      for (uint32_t c = 0; c < this_chunk && status == APGX_OK; ++c, ++r) {
+        a.round = r;
+        uint32_t cb = d.cost_bytes;
+        void* rargs[] = {&a, &cb};
+        if (opt.profile_rounds) {
+          hipEvent_t e0 = nullptr, e1 = nullptr;
+          if (!check(hipEventCreate(&e0), "hipEventCreate") || !check(hipEventCreate(&e1), "hipEventCreate")) break;
+          round_events.push_back(e0);
+          round_events.push_back(e1);
+          check(hipEventRecord(e0, stream), "hipEventRecord");
+        }
+        status = Launch(k.round, dim3(grid), dim3(kThreads), lds, stream, rargs, "round launch");
+        if (status != APGX_OK) break;
+        if (opt.profile_rounds) check(hipEventRecord(round_events.back(), stream), "hipEventRecord");
+        SearchArgs pa = a;
+        pa.round = r + 1;
+        void* pargs[] = {&pa};
+        status = Launch(k.phase, dim3((net_count + 63) / 64), dim3(64), 0, stream, pargs, "phase launch");
+      }
+      uint32_t next_count = 0;
+      if (!check(hipMemcpyAsync(&next_count, a.counters + 2ull * r, 4, hipMemcpyDeviceToHost, stream), "status readback")) break;
+      if (!check(hipStreamSynchronize(stream), "hipStreamSynchronize")) break;
+      ++syncs;
+      if (next_count == 0) done = true;
+      else if (r > max_rounds) done = true;  // budget: remaining searching nets stop
+    }
+    check(hipEventRecord(ev1, stream), "hipEventRecord");
+    check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  }
+  // Results to host.
+  s->state.resize(net_count);
+  if (status == APGX_OK && net_count)
+    check(hipMemcpy(s->state.data(), a.state, sizeof(NetState) * net_count, hipMemcpyDeviceToHost), "state readback");
+  s->conns.resize(conns);
+  if (status == APGX_OK && conns)
+    check(hipMemcpy(s->conns.data(), a.conns, sizeof(DevConnection) * conns, hipMemcpyDeviceToHost), "connection readback");
+  unsigned long long used = 0, stats[8] = {0};
+  if (status == APGX_OK) {
+    check(hipMemcpy(&used, a.cell_cursor, 8, hipMemcpyDeviceToHost), "cursor readback");
+    check(hipMemcpy(stats, a.stats, 64, hipMemcpyDeviceToHost), "stats readback");
+  }
+  if (status == APGX_OK) {
+    used = std::min<unsigned long long>(used, cell_capacity);
+    s->cells.resize(used);
+    if (used) check(hipMemcpy(s->cells.data(), a.cells, 4 * used, hipMemcpyDeviceToHost), "cell readback");
+  }
+  float ms = 0.f;
+  if (status == APGX_OK && net_count) check(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
+  double round_ms = 0.0;
+  for (size_t i = 0; i + 1 < round_events.size(); i += 2) {
+    float x = 0.f;
+    if (status == APGX_OK && check(hipEventElapsedTime(&x, round_events[i], round_events[i + 1]), "hipEventElapsedTime"))
+      round_ms += x;
+  }
+  for (hipEvent_t ev : round_events) (void)hipEventDestroy(ev);
+  if (ev0) (void)hipEventDestroy(ev0);
+  if (ev1) (void)hipEventDestroy(ev1);
+  if (stream) (void)hipStreamDestroy(stream);
+  if (status != APGX_OK) {
+    apgx_search_release(s);
+    return status;
+  }
+  for (auto& x : s->state)
+    if (x.status == kSearching) x.status = APGX_NET_BUDGET_EXHAUSTED;
+  apgx_route_stats& rs = s->stats;
+  rs.device_ms = ms;
+  rs.tile_visits = stats[0];
+  rs.cell_visits = stats[1];
+  rs.inner_sweeps = stats[2];
+  rs.relaxations = stats[3] * (d.layers > 1 && d.via_cost ? 6ull : 4ull);
+  rs.algorithmic_bytes = stats[1] * (static_cast<uint64_t>(d.cost_bytes) + 8ull);
+  rs.workspace_bytes = total + s->label_bytes;
+  rs.rounds = r - 1;
+  rs.tile_w = static_cast<uint32_t>(shape.tx);
+  rs.tile_h = static_cast<uint32_t>(shape.ty);
+  rs.host_syncs = syncs;
+  rs.round_kernel_ms = round_ms;
+  rs.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall_start).count();
+  *out = s;
+  ClearError();
+  return APGX_OK;
+}
+
+extern "C" int apgx_search_sizes(const apgx_search* s, uint64_t* connection_count, uint64_t* cell_count) {
+  using namespace apgx;
+  if (s == nullptr || connection_count == nullptr || cell_count == nullptr)
+    return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_search_sizes: null argument");
+  // Connections of net i sit at its conn_offset (pin_count - 1 slots each).
+  uint64_t nc = 0, cells = 0, offset = 0;
+  for (uint32_t i = 0; i < s->net_count; ++i) {
+    const NetState& st = s->state[i];
+    if (st.status == APGX_NET_INVALID) continue;
+    for (uint32_t c = 0; c < st.n_connected; ++c) cells += s->conns[offset + c].cell_count;
+    nc += st.n_connected;
+    offset += s->nets[i].pin_count - 1;
+  }
+  *connection_count = nc;
+  *cell_count = cells;
+  ClearError();
+  return APGX_OK;
+}
+
+extern "C" int apgx_search_readback(apgx_search* s, apgx_net_result* results,
+                                    apgx_connection* connections, uint64_t connection_capacity,
+                                    uint32_t* cells, uint64_t cell_capacity) {
+  using namespace apgx;
+  if (s == nullptr || (s->net_count && results == nullptr))
+    return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_search_readback: null argument");
+  uint64_t nc = 0, ncell = 0, offset = 0;
+  for (uint32_t i = 0; i < s->net_count; ++i) {
+    const NetState& st = s->state[i];
+    apgx_net_result& r = results[i];
+    std::memset(&r, 0, sizeof(r));
+    r.net_id = s->nets[i].net_id;
+    r.status = st.status;
+    r.rounds = st.rounds;
+    r.connection_offset = static_cast<uint32_t>(nc);
+    const bool valid = st.status != APGX_NET_INVALID;
+    r.cost = st.status == APGX_NET_REACHED ? st.cost : APGX_UNREACHABLE;
+    if (!valid) continue;
+    r.connection_count = st.n_connected;
+    for (uint32_t c = 0; c < st.n_connected; ++c) {
+      const DevConnection& dc = s->conns[offset + c];
+      if (nc >= connection_capacity || (connections == nullptr))
+        return Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_search_readback: connection capacity too small");
+      if (ncell + dc.cell_count > cell_capacity || (cells == nullptr && dc.cell_count))
+        return Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_search_readback: cell capacity too small");
+      if (dc.cell_offset + dc.cell_count > s->cells.size())
+        return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_search_readback: device path outside the cell buffer");
+      apgx_connection& out = connections[nc++];
+      out.pin_index = dc.pin_index;
+      out.distance = dc.distance;
+      out.cell_offset = ncell;
+      out.cell_count = dc.cell_count;
+      out.reserved = 0;
+      std::memcpy(cells + ncell, s->cells.data() + dc.cell_offset, 4ull * dc.cell_count);
+      ncell += dc.cell_count;
+    }
+    offset += s->nets[i].pin_count - 1;
+  }
+  s->read = true;
+  ClearError();
+  return APGX_OK;
+}
+
+extern "C" int apgx_search_stats(const apgx_search* s, apgx_route_stats* out) {
+  using namespace apgx;
+  if (s == nullptr || out == nullptr) return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_search_stats: null argument");
+  *out = s->stats;
+  ClearError();
+  return APGX_OK;
+}
+
+extern "C" void apgx_search_release(apgx_search* s) {
+  if (s == nullptr) return;
+  if (s->d_block) (void)hipFree(s->d_block);
+  if (s->d_labels) {
+    apgx_lattice* lat = s->lattice;
+    std::lock_guard<std::mutex> lock(lat->cache_mutex);
+    if (lat->cached_labels == nullptr) {
+      lat->cached_labels = s->d_labels;
+      lat->cached_label_bytes = s->label_bytes;
+    } else if (lat->cached_label_bytes < s->label_bytes) {
+      (void)hipFree(lat->cached_labels);
+      lat->cached_labels = s->d_labels;
+      lat->cached_label_bytes = s->label_bytes;
+    } else {
+      (void)hipFree(s->d_labels);
+    }
+  }
+  delete s;
+}

@@ -1,0 +1,249 @@
+/*
+ * apgx.h — C-ABI of the MI355X (gfx950) batched lattice search library.
+ *
+ * Everything that crosses this boundary is a plain C type: fixed-size POD
+ * records, pointers and sizes.  No C++ or torch types appear in signatures.
+ * Every entry point returns an int status:
+ *
+ *   0 APGX_OK
+ *   1 APGX_E_UNSUPPORTED         BackendErrorCode::kUnsupported
+ *   2 APGX_E_RESOURCE_EXHAUSTED  BackendErrorCode::kResourceExhausted
+ *   3 APGX_E_BACKEND_FAILURE     BackendErrorCode::kBackendFailure
+ *   4 APGX_E_INTERNAL_INVARIANT  BackendErrorCode::kInternalInvariant
+ *
+ * The four error codes keep the meaning of the reference's BackendErrorCode
+ * (include/apgar/gpu/planar_router.h:475-485): malformed requests and foreign
+ * handles are internal invariants, allocation failure and byte budgets are
+ * resource exhaustion, other HIP runtime failures are backend failures whose
+ * detail reads "<operation> failed: <hip message>" (gpu/cuda/planar_router.cu:67-78).
+ * apgx_last_error() returns the calling thread's last detail string.
+ *
+ * Two record families cross the boundary:
+ *
+ *  - V1: byte-identical restatements of the reference's DeviceCompiledBoardV1 and
+ *    DeviceCandidateBatch records (include/apgar/gpu/planar_router.h:170-423),
+ *    so a host that already builds them can hand them over unchanged.
+ *  - V2 ("lattice"): dense multi-layer planes with per-cell costs, vias, search
+ *    windows and multi-pin nets, which V1 cannot express (V1 is planar, has
+ *    uniform step costs, no windows and two-terminal queries only).
+ */
+#ifndef APGX_H_
+#define APGX_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define APGX_OK 0
+#define APGX_E_UNSUPPORTED 1
+#define APGX_E_RESOURCE_EXHAUSTED 2
+#define APGX_E_BACKEND_FAILURE 3
+#define APGX_E_INTERNAL_INVARIANT 4
+
+/* ------------------------------------------------------------------------ */
+/* Library / device                                                         */
+/* ------------------------------------------------------------------------ */
+
+typedef struct apgx_device_info {
+  char backend[32];         /* "hip" */
+  char device_name[256];
+  char gcn_arch_name[64];   /* e.g. "gfx950:sramecc+:xnack-" */
+  uint32_t compute_units;
+  uint32_t runtime_version;
+  uint32_t driver_version;
+  uint32_t reserved;
+  uint64_t global_memory_bytes;
+} apgx_device_info;
+
+/* Replaces IPlanarRouteBackend::QueryMetadata (include/apgar/gpu/planar_router.h:550). */
+int apgx_device_info_query(apgx_device_info* out);
+
+/* Thread-local detail of the last failing call on this thread ("" if none). */
+const char* apgx_last_error(void);
+
+/* Library ABI version (major << 16 | minor). */
+uint32_t apgx_abi_version(void);
+
+/* Blocks until all work the library queued on the current device is done. */
+int apgx_device_synchronize(void);
+
+/* ------------------------------------------------------------------------ */
+/* V2 dense lattice                                                          */
+/* ------------------------------------------------------------------------ */
+/*
+ * Cells are indexed c = (layer * height + y) * width + x.  cost[c] == 0 marks a
+ * blocked cell; free cells have cost in [1, APGX_MAX_CELL_COST].  Edges:
+ *   in-layer 4-neighbour move u -> v     weight cost[v]  (cost of entering v)
+ *   via (x,y,l) <-> (x,y,l+1)            weight via_cost (both cells free)
+ * Blocked cells have no edges.  A search window [x0,x0+w) x [y0,y0+h) applies
+ * to every layer; cells outside it do not exist for that net.
+ *
+ * Canonical path rule (the dense restatement of the reference's lowest
+ * predecessor rule, gpu/cuda/planar_router.cu:1084-1165): walking back from a
+ * reached cell v, the predecessor is the in-window free neighbour u with the
+ * lowest cell index such that dist[u] + w(u->v) == dist[v].
+ *
+ * Multi-pin nets: the tree starts as {pin 0}; each connection step runs an
+ * exact multi-source search from every tree cell, connects the unconnected pin
+ * with the smallest distance (ties: lowest pin index) along its canonical path
+ * back to the tree, and adds that path to the tree.  A two-pin net is one
+ * source/target query.
+ */
+#define APGX_LATTICE_SCHEMA_VERSION 2u
+#define APGX_MAX_CELL_COST 1048575u      /* 2^20 - 1 */
+#define APGX_MAX_VIA_COST 1048575u
+#define APGX_MAX_LAYERS 16u
+#define APGX_MAX_NET_PINS 32u
+#define APGX_UNREACHABLE 0xFFFFFFFFu
+
+typedef struct apgx_lattice_desc {
+  uint32_t schema_version;  /* APGX_LATTICE_SCHEMA_VERSION */
+  uint32_t width;
+  uint32_t height;
+  uint32_t layers;
+  uint32_t cost_bytes;      /* 2 (uint16 plane) or 4 (uint32 plane) */
+  uint32_t via_cost;        /* 0 disables vias */
+  uint32_t reserved[2];
+} apgx_lattice_desc;
+
+typedef struct apgx_lattice apgx_lattice;
+
+/* Validates and uploads one immutable cost plane (host memory, cost_bytes per
+ * cell).  The handle owns the device copy until apgx_lattice_release. */
+int apgx_lattice_create(const apgx_lattice_desc* desc, const void* host_cost_plane,
+                        apgx_lattice** out);
+/* Same, from a device-resident plane that the library copies (no PCIe). */
+int apgx_lattice_create_from_device(const apgx_lattice_desc* desc, const void* device_cost_plane,
+                                    apgx_lattice** out);
+void apgx_lattice_release(apgx_lattice* lattice);
+/* Device address of the lattice's cost plane (read-only for callers). */
+const void* apgx_lattice_device_costs(const apgx_lattice* lattice);
+
+typedef struct apgx_window {
+  uint32_t x0;
+  uint32_t y0;
+  uint32_t width;   /* 0 with height 0 selects the whole lattice */
+  uint32_t height;
+} apgx_window;
+
+typedef struct apgx_net_query {
+  uint64_t net_id;
+  uint32_t pin_offset;      /* into the pin cell array */
+  uint32_t pin_count;       /* 2 .. APGX_MAX_NET_PINS */
+  apgx_window window;
+} apgx_net_query;
+
+#define APGX_NET_REACHED 0u
+#define APGX_NET_DISCONNECTED 1u
+#define APGX_NET_BUDGET_EXHAUSTED 2u
+#define APGX_NET_INVALID 3u
+
+typedef struct apgx_net_result {
+  uint64_t net_id;
+  uint32_t status;            /* APGX_NET_* */
+  uint32_t cost;              /* sum of connection distances; APGX_UNREACHABLE unless reached */
+  uint32_t connection_count;  /* pin_count - 1 when reached */
+  uint32_t connection_offset; /* into the connection array (result order) */
+  uint32_t rounds;            /* sweep rounds in which this net had work */
+  uint32_t reserved;
+} apgx_net_result;
+
+typedef struct apgx_connection {
+  uint32_t pin_index;     /* pin joined by this connection */
+  uint32_t distance;      /* its exact multi-source distance from the tree */
+  uint64_t cell_offset;   /* into the cell array */
+  uint32_t cell_count;    /* cells from the pin back to the tree cell it joins, inclusive */
+  uint32_t reserved;
+} apgx_connection;
+
+typedef struct apgx_route_options {
+  uint32_t max_rounds;        /* 0: 100,000 (PlanarRoutePolicy::maximum_rounds) */
+  uint32_t tile_w;            /* 0: autotuned (also tile_h) */
+  uint32_t tile_h;
+  uint32_t rounds_per_sync;   /* rounds launched between host convergence checks; 0: auto */
+  uint64_t max_device_bytes;  /* 0: unlimited (workspace budget) */
+  uint32_t use_graph;         /* 1: launch round chunks as a captured hipGraph */
+  uint32_t profile_rounds;    /* 1: time every round kernel with hipEvents (round_kernel_ms) */
+  uint32_t reserved[2];
+} apgx_route_options;
+
+typedef struct apgx_route_stats {
+  double device_ms;              /* hipEvent envelope: first search kernel to last */
+  double wall_ms;                /* host wall time of apgx_route_nets */
+  uint64_t tile_visits;          /* tiles processed over all rounds */
+  uint64_t cell_visits;          /* tile_visits * cells per tile (window clipped) */
+  uint64_t algorithmic_bytes;    /* cell_visits * (cost bytes + 4 label read + 4 label write) */
+  uint64_t relaxations;          /* directed edges examined, summed over inner sweeps */
+  uint64_t inner_sweeps;         /* LDS sweeps summed over tile visits */
+  uint64_t workspace_bytes;      /* device bytes owned by the search */
+  uint32_t rounds;               /* global rounds executed */
+  uint32_t phases;               /* connection phases executed */
+  uint32_t tile_w;
+  uint32_t tile_h;
+  uint32_t host_syncs;
+  uint32_t graph_launches;
+  double round_kernel_ms;        /* summed hipEvent time of the round kernels */
+} apgx_route_stats;
+
+typedef struct apgx_search apgx_search;
+
+/* Routes a batch of nets (host arrays) on one lattice and keeps the results on
+ * the device until readback.  Replaces the execution half of
+ * IPlanarRouteBackend::ExecuteCandidateBatch for V2 lattices. */
+int apgx_route_nets(apgx_lattice* lattice, const apgx_net_query* nets, uint32_t net_count,
+                    const uint32_t* pins, uint64_t pin_count, const apgx_route_options* options,
+                    apgx_search** out);
+/* Sizes needed by apgx_search_readback. */
+int apgx_search_sizes(const apgx_search* search, uint64_t* connection_count,
+                      uint64_t* cell_count);
+/* Copies results in net order; connections and cells are packed in net order. */
+int apgx_search_readback(apgx_search* search, apgx_net_result* results,
+                         apgx_connection* connections, uint64_t connection_capacity,
+                         uint32_t* cells, uint64_t cell_capacity);
+int apgx_search_stats(const apgx_search* search, apgx_route_stats* out);
+void apgx_search_release(apgx_search* search);
+
+/* ------------------------------------------------------------------------ */
+/* Synthetic lattice generators (deterministic xoshiro256** streams)         */
+/* ------------------------------------------------------------------------ */
+
+typedef struct apgx_lattice_gen_params {
+  uint32_t width;
+  uint32_t height;
+  uint32_t layers;
+  uint32_t cost_min;            /* uniform integer cost in [cost_min, cost_max] */
+  uint32_t cost_max;
+  uint32_t blocked_ppm;         /* i.i.d. blocked probability, parts per million */
+  uint32_t keepout_ppm;         /* per-layer rectangular keepout coverage target */
+  uint32_t keepout_min_side;
+  uint32_t keepout_max_side;
+  uint32_t reserved;
+  uint64_t seed;
+} apgx_lattice_gen_params;
+
+/* Writes width*height*layers uint32 costs (0 = blocked). */
+int apgx_generate_lattice(const apgx_lattice_gen_params* params, uint32_t* cost_out);
+
+/* Random single-pair queries: pins_out gets 2*count cells (source, target),
+ * both free and distinct, uniform over all free cells. */
+int apgx_generate_pairs(const uint32_t* cost, uint32_t width, uint32_t height, uint32_t layers,
+                        uint32_t count, uint64_t seed, uint32_t* pins_out);
+
+/* Random multi-pin nets with log-uniform square windows of side
+ * [min_side, max_side]; pin counts uniform in [min_pins, max_pins]; pins are
+ * distinct free cells inside the window.  Writes count net records (pin
+ * offsets into pins_out) and returns the pin total in *pin_total.
+ * pins_out must hold count*max_pins cells. */
+int apgx_generate_nets(const uint32_t* cost, uint32_t width, uint32_t height, uint32_t layers,
+                       uint32_t count, uint32_t min_pins, uint32_t max_pins, uint32_t min_side,
+                       uint32_t max_side, uint64_t seed, apgx_net_query* nets_out,
+                       uint32_t* pins_out, uint64_t* pin_total);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* APGX_H_ */

@@ -1,0 +1,27 @@
+"""Workload definitions shared by tests, the smoke check and the benchmark.
+
+The five lattice workloads of this project (sizes, cost models, seeds):
+  single_layer_256   1 x 256 x 256, uint32 cost U[1,16], 10% i.i.d. blocked, 64 pairs, seed 1
+  four_layer_1024    4 x 1024 x 1024, uint16 cost U[1,8], 15% rectangular keepouts
+                     (sides U[4,64]), vias cost 20, 1024 pairs, seed 2
+  eight_layer_2048   8 x 2048 x 2048, same model, 4096 nets of 2-8 pins, windows
+                     log-uniform 32..2048, seed 3
+"""
+from __future__ import annotations
+
+SINGLE_LAYER_256 = dict(width=256, height=256, layers=1, cost_min=1, cost_max=16,
+                        blocked_ppm=100_000, seed=1, via=0, queries=64, cost_bytes=4)
+FOUR_LAYER_1024 = dict(width=1024, height=1024, layers=4, cost_min=1, cost_max=8,
+                       keepout_ppm=150_000, keepout_min_side=4, keepout_max_side=64, seed=2,
+                       via=20, queries=1024, cost_bytes=2)
+EIGHT_LAYER_2048 = dict(width=2048, height=2048, layers=8, cost_min=1, cost_max=8,
+                        keepout_ppm=150_000, keepout_min_side=4, keepout_max_side=64, seed=3,
+                        via=20, nets=4096, min_pins=2, max_pins=8, min_side=32, max_side=2048,
+                        cost_bytes=2)
+
+
+def make_lattice(apgx, spec):
+    return apgx.generate_lattice(spec["width"], spec["height"], spec["layers"], spec["cost_min"],
+                                 spec["cost_max"], spec.get("blocked_ppm", 0),
+                                 spec.get("keepout_ppm", 0), spec.get("keepout_min_side", 0),
+                                 spec.get("keepout_max_side", 0), spec["seed"])

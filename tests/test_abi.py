@@ -1,0 +1,39 @@
+"""CPU checks of the C-ABI library: it loads and exports every declared symbol.
+
+No compute call is made here (there may be no GPU)."""
+import ctypes
+import re
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_functions():
+    names = set()
+    for header in (ROOT / "include").glob("*.h"):
+        text = header.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\*?\s*(apgx_[a-z0-9_]+)\s*\(",
+                             text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol(apgx):
+    names = declared_functions()
+    assert len(names) >= 15
+    handle = ctypes.CDLL(str(apgx.LIB_PATH))
+    missing = [n for n in sorted(names) if not hasattr(handle, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_the_header(apgx):
+    assert declared_functions() <= set(apgx.exported_symbols()) | set()
+
+
+def test_last_error_reports_invalid_arguments(apgx):
+    lib = apgx.lib()
+    assert lib.apgx_abi_version() >> 16 == 1
+    rc = lib.apgx_generate_lattice(None, None)
+    assert rc == 4
+    assert b"invalid" in lib.apgx_last_error()

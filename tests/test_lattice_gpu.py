@@ -1,0 +1,171 @@
+"""GPU parity of the dense lattice search against the CPU oracle.
+
+Bar: distances bit-exact and every connection path identical to the oracle's
+canonical path; at full size, size-independent properties (adjacency, exact
+path cost, canonical-predecessor condition) for every query.
+"""
+import numpy as np
+import pytest
+
+from tests.lattice_cases import FOUR_LAYER_1024, SINGLE_LAYER_256, make_lattice
+from tests.test_oracle_dense import check_canonical, random_lattice
+
+pytestmark = pytest.mark.gpu
+UNR = 0xFFFFFFFF
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def test_single_layer_256_full_workload(apgx, oracle, gpu):
+    spec = SINGLE_LAYER_256
+    _say("[case] single_layer_256: 64 pairs, seed 1")
+    cost = make_lattice(apgx, spec)
+    pairs = apgx.generate_pairs(cost, spec["queries"], 101)
+    lat = apgx.Lattice(cost, via_cost=0, cost_bytes=4)
+    routes, stats = lat.route_pairs(pairs)
+    o = oracle.Dense(cost, 0)
+    for (s, t), r in zip(pairs, routes):
+        d_astar, _, _ = o.astar(int(s), int(t))
+        dist = o.dijkstra([int(s)])
+        assert d_astar == dist[t]
+        if d_astar == UNR:
+            assert r.status == apgx.NET_DISCONNECTED
+            continue
+        assert r.status == apgx.NET_REACHED
+        assert r.cost == d_astar
+        pin, d, cells = r.connections[0]
+        assert pin == 1 and d == d_astar
+        assert np.array_equal(cells, o.canonical_path(dist, int(t)))
+    assert stats.tile_visits > 0 and stats.device_ms > 0
+    lat.close()
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_small_lattices(apgx, oracle, gpu, seed):
+    rng = np.random.default_rng(1000 + seed)
+    L = [1, 2, 4, 3][seed % 4]
+    H, W = [(37, 53), (64, 64), (70, 129), (100, 33)][seed % 4]
+    cost = random_lattice(rng, L, H, W, blocked=0.2, cmax=[3, 8, 16, 2][seed % 4])
+    via = 0 if L == 1 else [20, 1, 7, 3][seed % 4]
+    _say(f"[case] random lattice seed={seed} shape={(L, H, W)} via={via}")
+    free = np.flatnonzero(cost.reshape(-1))
+    pairs = rng.choice(free, size=(12, 2), replace=True)
+    pairs = pairs[pairs[:, 0] != pairs[:, 1]]
+    tile = [(0, 0), (32, 16), (32, 32), (64, 32)][seed % 4]
+    lat = apgx.Lattice(cost, via_cost=via)
+    routes, _ = lat.route_pairs(pairs, tile=tile)
+    o = oracle.Dense(cost, via)
+    for (s, t), r in zip(pairs, routes):
+        dist = o.dijkstra([int(s)])
+        if dist[t] == UNR:
+            assert r.status == apgx.NET_DISCONNECTED
+            continue
+        assert r.status == apgx.NET_REACHED and r.cost == dist[t]
+        assert np.array_equal(r.connections[0][2], o.canonical_path(dist, int(t)))
+    lat.close()
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_windows_and_multi_pin_nets(apgx, oracle, gpu, seed):
+    rng = np.random.default_rng(2000 + seed)
+    L, H, W = 3, 90, 110
+    cost = random_lattice(rng, L, H, W, blocked=0.15, cmax=8)
+    via = 20
+    _say(f"[case] multi-pin nets seed={seed}")
+    pin_lists, windows = [], []
+    for _ in range(10):
+        w, h = int(rng.integers(8, W)), int(rng.integers(8, H))
+        x0, y0 = int(rng.integers(0, W - w + 1)), int(rng.integers(0, H - h + 1))
+        ll, yy, xx = np.meshgrid(np.arange(L), np.arange(y0, y0 + h), np.arange(x0, x0 + w), indexing="ij")
+        cells = ((ll * H + yy) * W + xx).reshape(-1)
+        cells = cells[cost.reshape(-1)[cells] > 0]
+        k = int(rng.integers(2, 7))
+        pin_lists.append(rng.choice(cells, size=k, replace=False))
+        windows.append((x0, y0, w, h))
+    nets, pins = apgx.nets_from_lists(pin_lists, windows)
+    lat = apgx.Lattice(cost, via_cost=via)
+    routes, _ = lat.route(nets, pins, tile=[(0, 0), (32, 16), (32, 32), (64, 32)][seed])
+    for pl, win, r in zip(pin_lists, windows, routes):
+        rc, conns = oracle.Dense(cost, via, win).route_net(pl)
+        if rc == 1:
+            assert r.status == apgx.NET_DISCONNECTED
+            continue
+        assert r.status == apgx.NET_REACHED
+        assert len(r.connections) == len(conns)
+        for (gp, gd, gc), (op, od, oc) in zip(r.connections, conns):
+            assert (gp, gd) == (op, od)
+            assert np.array_equal(gc, oc)
+        assert r.cost == sum(d for _, d, _ in conns)
+    lat.close()
+
+
+def test_edge_cases(apgx, gpu):
+    _say("[case] edge cases")
+    cost = np.ones((1, 16, 16), dtype=np.uint32)
+    cost[0, :, 8] = 0  # wall: left and right halves disconnected
+    lat = apgx.Lattice(cost, via_cost=0)
+    # empty batch
+    routes, stats = lat.route_pairs(np.zeros((0, 2), dtype=np.uint32))
+    assert routes == []
+    # disconnected, blocked pin (invalid), identical pins (invalid), neighbour pair
+    pairs = np.array([[0, 15], [8, 0], [3, 3], [0, 1]], dtype=np.uint32)
+    routes, _ = lat.route_pairs(pairs)
+    assert [r.status for r in routes] == [apgx.NET_DISCONNECTED, apgx.NET_INVALID,
+                                          apgx.NET_INVALID, apgx.NET_REACHED]
+    assert routes[3].cost == 1 and list(routes[3].connections[0][2]) == [1, 0]
+    # a one-round budget cannot finish a long route
+    pairs = np.array([[0, 16 * 15 + 7]], dtype=np.uint32)
+    routes, _ = lat.route_pairs(pairs, max_rounds=1, rounds_per_sync=1, tile=(32, 16))
+    assert routes[0].status in (apgx.NET_BUDGET_EXHAUSTED, apgx.NET_REACHED)
+    lat.close()
+
+
+def test_repeatable(apgx, gpu):
+    _say("[case] repeatability")
+    rng = np.random.default_rng(5)
+    cost = random_lattice(rng, 2, 80, 80, blocked=0.1, cmax=4)
+    free = np.flatnonzero(cost.reshape(-1))
+    pairs = rng.choice(free, size=(20, 2), replace=False)
+    lat = apgx.Lattice(cost, via_cost=3)
+    a, _ = lat.route_pairs(pairs)
+    b, _ = lat.route_pairs(pairs)
+    for x, y in zip(a, b):
+        assert (x.status, x.cost) == (y.status, y.cost)
+        assert all(np.array_equal(p[2], q[2]) for p, q in zip(x.connections, y.connections))
+    lat.close()
+
+
+def test_four_layer_1024_full_size(apgx, oracle, gpu):
+    spec = FOUR_LAYER_1024
+    _say("[case] four_layer_1024: 1024 pairs, seed 2 (oracle on a sample, properties on all)")
+    cost = make_lattice(apgx, spec)
+    pairs = apgx.generate_pairs(cost, spec["queries"], 202)
+    lat = apgx.Lattice(cost, via_cost=spec["via"], cost_bytes=2)
+    routes, stats = lat.route_pairs(pairs)
+    _say(f"[info] device_ms={stats.device_ms:.2f} rounds={stats.rounds} tiles={stats.tile_visits}")
+    L, H, W = cost.shape
+    plane = H * W
+    flat = cost.reshape(-1)
+    for (s, t), r in zip(pairs, routes):
+        assert r.status in (apgx.NET_REACHED, apgx.NET_DISCONNECTED)
+        if r.status != apgx.NET_REACHED:
+            continue
+        cells = r.connections[0][2].astype(np.int64)
+        assert cells[0] == t and cells[-1] == s
+        steps = np.abs(np.diff(cells))
+        assert np.isin(steps, [1, W, plane]).all()
+        w = np.where(steps == plane, spec["via"], flat[cells[:-1]].astype(np.int64))
+        assert int(w.sum()) == r.cost
+    o = oracle.Dense(cost, spec["via"])
+    for i in range(0, spec["queries"], 128):
+        s, t = (int(v) for v in pairs[i])
+        dist = o.dijkstra([s])
+        r = routes[i]
+        if dist[t] == UNR:
+            assert r.status == apgx.NET_DISCONNECTED
+            continue
+        assert r.cost == dist[t]
+        assert np.array_equal(r.connections[0][2], o.canonical_path(dist, t))
+    lat.close()

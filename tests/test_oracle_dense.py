@@ -1,0 +1,228 @@
+"""Pins the dense-lattice oracle (CPU only).
+
+Dijkstra is checked against an independent brute-force Bellman-Ford written
+here on the explicit edge list, against closed forms, and A* against Dijkstra.
+The canonical path is checked by properties a plausible mistake would break
+(adjacency, exact cost, the lowest-index-predecessor condition at every step)
+and on a hand-worked tie.
+"""
+import numpy as np
+import pytest
+
+from tests.lattice_cases import SINGLE_LAYER_256, make_lattice
+
+UNR = 0xFFFFFFFF
+
+
+def edges(cost, via, window=None):
+    """Explicit directed edge list (u, v, w) of the lattice model."""
+    L, H, W = cost.shape
+    out = []
+
+    def ok(l, y, x):
+        if not (0 <= l < L and 0 <= y < H and 0 <= x < W):
+            return False
+        if window is not None:
+            x0, y0, ww, wh = window
+            if not (x0 <= x < x0 + ww and y0 <= y < y0 + wh):
+                return False
+        return cost[l, y, x] != 0
+
+    for l in range(L):
+        for y in range(H):
+            for x in range(W):
+                if not ok(l, y, x):
+                    continue
+                u = (l * H + y) * W + x
+                for dl, dy, dx in ((0, 0, 1), (0, 0, -1), (0, 1, 0), (0, -1, 0), (1, 0, 0), (-1, 0, 0)):
+                    if dl and not via:
+                        continue
+                    l2, y2, x2 = l + dl, y + dy, x + dx
+                    if ok(l2, y2, x2):
+                        v = (l2 * H + y2) * W + x2
+                        out.append((u, v, via if dl else int(cost[l2, y2, x2])))
+    return out
+
+
+def bellman_ford(n, es, sources):
+    d = [None] * n
+    for s in sources:
+        d[s] = 0
+    for _ in range(n):
+        changed = False
+        for u, v, w in es:
+            if d[u] is not None and (d[v] is None or d[u] + w < d[v]):
+                d[v] = d[u] + w
+                changed = True
+        if not changed:
+            break
+    return np.array([UNR if x is None else x for x in d], dtype=np.uint64)
+
+
+def random_lattice(rng, L, H, W, blocked=0.2, cmax=9):
+    cost = rng.integers(1, cmax + 1, size=(L, H, W)).astype(np.uint32)
+    cost[rng.random((L, H, W)) < blocked] = 0
+    return cost
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_dijkstra_matches_bellman_ford(oracle, seed):
+    rng = np.random.default_rng(seed)
+    L, H, W = [(1, 7, 9), (2, 6, 5), (3, 5, 6)][seed % 3]
+    cost = random_lattice(rng, L, H, W)
+    via = [0, 20, 3][seed % 3] if L > 1 else 0
+    window = None if seed < 3 else (1, 1, W - 2, H - 2)
+    free = np.flatnonzero(cost.reshape(-1))
+    if window is not None:
+        xs, ys = free % W, (free // W) % H
+        free = free[(xs >= 1) & (xs < W - 1) & (ys >= 1) & (ys < H - 1)]
+    srcs = rng.choice(free, size=1 + seed % 2, replace=False)
+    o = oracle.Dense(cost, via, window)
+    got = o.dijkstra(srcs).astype(np.uint64)
+    want = bellman_ford(L * H * W, edges(cost, via, window), [int(s) for s in srcs])
+    assert np.array_equal(got, want)
+
+
+def test_closed_form_uniform_costs(oracle):
+    L, H, W, c, via = 3, 9, 11, 5, 7
+    cost = np.full((L, H, W), c, dtype=np.uint32)
+    o = oracle.Dense(cost, via)
+    s = (1 * H + 4) * W + 2
+    d = o.dijkstra([s]).reshape(L, H, W)
+    ll, yy, xx = np.meshgrid(np.arange(L), np.arange(H), np.arange(W), indexing="ij")
+    want = c * (np.abs(yy - 4) + np.abs(xx - 2)) + via * np.abs(ll - 1)
+    assert np.array_equal(d.astype(np.int64), want)
+
+
+def test_corridor_and_disconnection(oracle):
+    # one free row: distance is the sum of entered costs; a cut row disconnects
+    cost = np.zeros((1, 3, 8), dtype=np.uint32)
+    cost[0, 1, :] = [3, 1, 4, 1, 5, 9, 2, 6]
+    o = oracle.Dense(cost, 0)
+    d = o.dijkstra([8]).reshape(3, 8)
+    assert list(d[1]) == [0, 1, 5, 6, 11, 20, 22, 28]
+    assert (d[0] == UNR).all() and (d[2] == UNR).all()
+    cost[0, 1, 4] = 0
+    d = oracle.Dense(cost, 0).dijkstra([8]).reshape(3, 8)
+    assert d[1, 5] == UNR
+
+
+def check_canonical(cost, via, dist, path, src_set):
+    L, H, W = cost.shape
+    plane = H * W
+    t = int(path[0])
+    assert dist[t] != UNR
+    total = 0
+    for a, b in zip(path[:-1], path[1:]):
+        a, b = int(a), int(b)  # a is reached from predecessor b
+        diff = abs(a - b)
+        assert diff in (1, W, plane)
+        w = via if diff == plane else int(cost.reshape(-1)[a])
+        assert int(dist[b]) + w == int(dist[a])
+        total += w
+        # no lower-index predecessor achieves the same distance
+        x, y, l = a % W, (a // W) % H, a // plane
+        cands = []
+        if via and l > 0: cands.append((a - plane, via))
+        if y > 0: cands.append((a - W, int(cost.reshape(-1)[a])))
+        if x > 0: cands.append((a - 1, int(cost.reshape(-1)[a])))
+        if x + 1 < W: cands.append((a + 1, int(cost.reshape(-1)[a])))
+        if y + 1 < H: cands.append((a + W, int(cost.reshape(-1)[a])))
+        if via and l + 1 < L: cands.append((a + plane, via))
+        for u, w2 in cands:
+            if u < b and cost.reshape(-1)[u] != 0 and dist[u] != UNR:
+                assert int(dist[u]) + w2 != int(dist[a]), "a lower-index predecessor ties"
+    assert int(path[-1]) in src_set and dist[int(path[-1])] == 0
+    assert total == int(dist[t])
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_canonical_path_properties(oracle, seed):
+    rng = np.random.default_rng(100 + seed)
+    cost = random_lattice(rng, 2, 12, 14, blocked=0.15, cmax=3)  # small costs: many ties
+    o = oracle.Dense(cost, 2)
+    free = np.flatnonzero(cost.reshape(-1))
+    s, t = rng.choice(free, size=2, replace=False)
+    dist = o.dijkstra([s])
+    if dist[t] == UNR:
+        pytest.skip("disconnected draw")
+    path = o.canonical_path(dist, int(t))
+    check_canonical(cost, 2, dist, path, {int(s)})
+
+
+def test_canonical_tie_hand_worked(oracle):
+    # 2x2 grid, all costs 1: from (0,0) to (1,1) both routes cost 2.  Walking
+    # back from cell 3, predecessors 1 (west of 3? no: x-1 -> cell 2) ... the
+    # lower-index predecessor is cell 1 (y-1), so the path is 3 <- 1 <- 0.
+    cost = np.ones((1, 2, 2), dtype=np.uint32)
+    o = oracle.Dense(cost, 0)
+    d = o.dijkstra([0])
+    assert list(o.canonical_path(d, 3)) == [3, 1, 0]
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_astar_matches_dijkstra(oracle, seed):
+    rng = np.random.default_rng(200 + seed)
+    cost = random_lattice(rng, 1 + seed % 3, 20, 24, blocked=0.25, cmax=16)
+    via = 20 if cost.shape[0] > 1 else 0
+    o = oracle.Dense(cost, via)
+    free = np.flatnonzero(cost.reshape(-1))
+    for _ in range(5):
+        s, t = rng.choice(free, size=2, replace=False)
+        dist = o.dijkstra([s])
+        d, path, _ = o.astar(int(s), int(t), want_path=True)
+        assert d == dist[t]
+        if d != UNR:
+            assert int(path[0]) == t and int(path[-1]) == s
+
+
+def test_route_net_two_pins_is_pair(oracle):
+    rng = np.random.default_rng(7)
+    cost = random_lattice(rng, 2, 10, 10, blocked=0.1)
+    o = oracle.Dense(cost, 5)
+    free = np.flatnonzero(cost.reshape(-1))
+    s, t = (int(v) for v in rng.choice(free, size=2, replace=False))
+    dist = o.dijkstra([s])
+    rc, conns = o.route_net([s, t])
+    assert rc == 0
+    assert conns[0][1] == dist[t]
+    assert np.array_equal(conns[0][2], o.canonical_path(dist, t))
+
+
+def test_route_net_hand_worked(oracle):
+    # single row of unit costs with pins at 0, 9 and 3: the tree grows from
+    # pin 0 to the nearest pin (3, distance 3), then from {0..3} to 9 (6).
+    cost = np.ones((1, 1, 10), dtype=np.uint32)
+    rc, conns = oracle.Dense(cost, 0).route_net([0, 9, 3])
+    assert rc == 0
+    assert [(p, d) for p, d, _ in conns] == [(2, 3), (1, 6)]
+    assert list(conns[0][2]) == [3, 2, 1, 0]
+    assert list(conns[1][2]) == [9, 8, 7, 6, 5, 4, 3]
+
+
+def test_route_net_tie_takes_lowest_pin(oracle):
+    cost = np.ones((1, 1, 7), dtype=np.uint32)
+    rc, conns = oracle.Dense(cost, 0).route_net([3, 6, 0])  # pins 1 and 2 both at distance 3
+    assert rc == 0 and conns[0][0] == 1 and conns[1][0] == 2
+
+
+def test_single_layer_generator_statistics(apgx):
+    cost = make_lattice(apgx, SINGLE_LAYER_256)
+    assert cost.shape == (1, 256, 256)
+    free = cost[cost > 0]
+    assert free.min() >= 1 and free.max() <= 16
+    frac = (cost == 0).mean()
+    assert 0.09 < frac < 0.11
+    again = make_lattice(apgx, SINGLE_LAYER_256)
+    assert np.array_equal(cost, again)
+    pairs = apgx.generate_pairs(cost, 64, 11)
+    flat = cost.reshape(-1)
+    assert (flat[pairs[:, 0]] > 0).all() and (flat[pairs[:, 1]] > 0).all()
+    assert (pairs[:, 0] != pairs[:, 1]).all()
+
+
+def test_keepout_generator_coverage(apgx):
+    cost = apgx.generate_lattice(256, 256, 2, 1, 8, 0, 150_000, 4, 64, 9)
+    for l in range(2):
+        assert (cost[l] == 0).mean() >= 0.15
+    assert cost.max() <= 8
