@@ -759,6 +759,8 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
     return Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_route_nets: tile count exceeds the uint32 index space");
   }
   s->total_connections = conns;
+  s->conn_offset.resize(net_count);
+  for (uint32_t i = 0; i < net_count; ++i) s->conn_offset[i] = ns[i].conn_offset;
   const uint64_t cell_capacity = std::max<uint64_t>(1u << 20, std::min<uint64_t>(window_cells_total, 1ull << 28));
   // Device layout: one block for everything except the label pool.
   const uint64_t counter_words = 2ull * (static_cast<uint64_t>(max_rounds) + 2);
@@ -971,13 +973,12 @@ extern "C" int apgx_search_sizes(const apgx_search* s, uint64_t* connection_coun
   if (s == nullptr || connection_count == nullptr || cell_count == nullptr)
     return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_search_sizes: null argument");
   // Connections of net i sit at its conn_offset (pin_count - 1 slots each).
-  uint64_t nc = 0, cells = 0, offset = 0;
+  uint64_t nc = 0, cells = 0;
   for (uint32_t i = 0; i < s->net_count; ++i) {
     const NetState& st = s->state[i];
     if (st.status == APGX_NET_INVALID) continue;
-    for (uint32_t c = 0; c < st.n_connected; ++c) cells += s->conns[offset + c].cell_count;
+    for (uint32_t c = 0; c < st.n_connected; ++c) cells += s->conns[s->conn_offset[i] + c].cell_count;
     nc += st.n_connected;
-    offset += s->nets[i].pin_count - 1;
   }
   *connection_count = nc;
   *cell_count = cells;
@@ -991,7 +992,7 @@ extern "C" int apgx_search_readback(apgx_search* s, apgx_net_result* results,
   using namespace apgx;
   if (s == nullptr || (s->net_count && results == nullptr))
     return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_search_readback: null argument");
-  uint64_t nc = 0, ncell = 0, offset = 0;
+  uint64_t nc = 0, ncell = 0;
   for (uint32_t i = 0; i < s->net_count; ++i) {
     const NetState& st = s->state[i];
     apgx_net_result& r = results[i];
@@ -1005,7 +1006,7 @@ extern "C" int apgx_search_readback(apgx_search* s, apgx_net_result* results,
     if (!valid) continue;
     r.connection_count = st.n_connected;
     for (uint32_t c = 0; c < st.n_connected; ++c) {
-      const DevConnection& dc = s->conns[offset + c];
+      const DevConnection& dc = s->conns[s->conn_offset[i] + c];
       if (nc >= connection_capacity || (connections == nullptr))
         return Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_search_readback: connection capacity too small");
       if (ncell + dc.cell_count > cell_capacity || (cells == nullptr && dc.cell_count))
@@ -1021,7 +1022,6 @@ extern "C" int apgx_search_readback(apgx_search* s, apgx_net_result* results,
       std::memcpy(cells + ncell, s->cells.data() + dc.cell_offset, 4ull * dc.cell_count);
       ncell += dc.cell_count;
     }
-    offset += s->nets[i].pin_count - 1;
   }
   s->read = true;
   ClearError();

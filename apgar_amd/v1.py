@@ -1,0 +1,258 @@
+"""ctypes mirror of the V1 (reference record) C-ABI of libapgx.so.
+
+Records are byte-identical to include/apgar/gpu/planar_router.h:170-423.  The
+helpers here only marshal records; all flattening, validation and search runs
+in the library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+from . import check, lib
+
+INVALID = 0xFFFFFFFF
+BANNED = 0xFFFFFFFFFFFFFFFF
+GENERATOR_FRONTIER, GENERATOR_SWEEP = 0, 1
+REACHED, DISCONNECTED, BUDGET_EXHAUSTED, CANCELLED = 0, 1, 2, 3
+U8, U32, U64, I64, P = C.c_uint8, C.c_uint32, C.c_uint64, C.c_int64, C.POINTER
+
+
+class Header(C.Structure):
+    _fields_ = [("schema_version", U32), ("compiler_version", U32),
+                ("source_board_content_hash", U64), ("compiler_profile_fingerprint", U64),
+                ("rule_bucket_identity", U64), ("device_view_fingerprint", U64),
+                ("represented_nodes", U64), ("represented_states", U64),
+                ("estimated_persistent_device_bytes", U64), ("lattice_origin_x", I64),
+                ("lattice_origin_y", I64), ("lattice_step", I64), ("orthogonal_step", U32),
+                ("diagonal_step", U32), ("bend", U32), ("heading_mask", U8), ("reserved", U8 * 7)]
+
+
+class LayerRange(C.Structure):
+    _fields_ = [("layer", U32), ("node_offset", U32), ("node_count", U32), ("pad", U32),
+                ("minimum_lattice_x", I64), ("maximum_lattice_x", I64),
+                ("minimum_lattice_y", I64), ("maximum_lattice_y", I64)]
+
+
+class Node(C.Structure):
+    _fields_ = [("lattice_x", I64), ("lattice_y", I64), ("layer", U32), ("legal_edges", U8),
+                ("reserved", U8 * 3), ("neighbors", U32 * 8)]
+
+
+class Run(C.Structure):
+    _fields_ = [("node_offset", U32), ("node_count", U32), ("direction", U8), ("reserved", U8 * 3)]
+
+
+class PolicyEdge(C.Structure):
+    _fields_ = [("physical_edge_index", U32), ("reserved", U32), ("adjustment", U64)]
+
+
+class Query(C.Structure):
+    _fields_ = [("schema_version", U32), ("input_ordinal", U32), ("batch_id", U64),
+                ("query_id", U64), ("workspace_owner", U64), ("policy_identity", U64),
+                ("routing_profile_fingerprint", U64), ("start_node", U32), ("goal_node", U32),
+                ("policy_offset", U32), ("policy_count", U32), ("workspace_offset", U64),
+                ("workspace_state_count", U64), ("frontier_state_capacity", U64),
+                ("orthogonal_step_surcharge", U64), ("diagonal_step_surcharge", U64),
+                ("bend_surcharge", U64), ("maximum_rounds", U32), ("generator", U8),
+                ("reserved", U8 * 3)]
+
+
+class Result(C.Structure):
+    _fields_ = [("schema_version", U32), ("compiler_version", U32), ("input_ordinal", U32),
+                ("start_node", U32), ("goal_node", U32), ("goal_state", U32), ("rounds", U32),
+                ("generator", U8), ("completion", U8), ("reserved", U8 * 2), ("batch_id", U64),
+                ("query_id", U64), ("workspace_owner", U64), ("policy_identity", U64),
+                ("routing_profile_fingerprint", U64), ("source_board_content_hash", U64),
+                ("compiler_profile_fingerprint", U64), ("rule_bucket_identity", U64),
+                ("device_view_fingerprint", U64), ("examined_work", U64),
+                ("heading_turn_relaxations", U64)]
+
+
+class CompactPath(C.Structure):
+    _fields_ = [("schema_version", U32), ("state_count", U32), ("query_id", U64),
+                ("workspace_owner", U64), ("state_offset", U64)]
+
+
+class BatchRequest(C.Structure):
+    _fields_ = [("schema_version", U32), ("generator", U32), ("batch_id", U64),
+                ("maximum_rounds", U32), ("reserved", U32),
+                ("maximum_workspace_states_per_query", U64),
+                ("maximum_frontier_states_per_query", U64), ("maximum_device_bytes", U64),
+                ("maximum_host_bytes", U64)]
+
+
+class BatchTelemetry(C.Structure):
+    _fields_ = [("persistent_device_bytes", U64), ("batch_device_bytes", U64),
+                ("workspace_capacity_device_bytes", U64), ("peak_device_bytes", U64),
+                ("batch_host_bytes", U64), ("device_to_host_readback_bytes", U64),
+                ("kernel_launch_count", U64), ("blocking_status_readback_count", U64),
+                ("dispatched_rounds", U32), ("finalization_launch_count", U32),
+                ("chunk_rounds", U32), ("reserved", U32), ("kernel_milliseconds", C.c_double)]
+
+
+class CompiledProfile(C.Structure):
+    _fields_ = [("compiler_version", U32), ("heading_mask", U32),
+                ("source_board_content_hash", U64), ("compiler_profile_fingerprint", U64),
+                ("rule_bucket_identity", U64), ("lattice_origin_x", I64), ("lattice_origin_y", I64),
+                ("lattice_step", I64), ("tile_width_nodes", U32), ("tile_height_nodes", U32),
+                ("orthogonal_step", U32), ("diagonal_step", U32), ("bend", U32), ("reserved", U32)]
+
+
+class CompiledNode(C.Structure):
+    _fields_ = [("lattice_x", I64), ("lattice_y", I64), ("layer", U32), ("legal_edges", U8),
+                ("reserved", U8 * 3)]
+
+
+_SIGS = {
+    "apgx_v1_build_board": (C.c_int, [P(CompiledProfile), P(CompiledNode), U64, P(C.c_void_p)]),
+    "apgx_v1_board_sizes": (C.c_int, [C.c_void_p, P(U64), P(U64), P(U64), P(U64)]),
+    "apgx_v1_board_copy": (C.c_int, [C.c_void_p, P(Header), P(LayerRange), P(Node), P(Run), P(U32)]),
+    "apgx_v1_board_release": (None, [C.c_void_p]),
+    "apgx_v1_view_fingerprint": (U64, [P(Header), P(LayerRange), U64, P(Node), U64, P(Run), U64,
+                                       P(U32), U64]),
+    "apgx_v1_view_upload": (C.c_int, [P(Header), P(LayerRange), U64, P(Node), U64, P(Run), U64,
+                                      P(U32), U64, P(C.c_void_p)]),
+    "apgx_v1_view_release": (None, [C.c_void_p]),
+    "apgx_v1_batch_execute": (C.c_int, [C.c_void_p, P(BatchRequest), P(Query), U32, P(PolicyEdge),
+                                        U64, P(U32), P(C.c_void_p)]),
+    "apgx_v1_batch_readback": (C.c_int, [C.c_void_p, P(Result), P(CompactPath), P(U32), U64,
+                                         P(U64), P(BatchTelemetry)]),
+    "apgx_v1_pending_release": (None, [C.c_void_p]),
+}
+
+
+def _lib():
+    h = lib()
+    if not getattr(h, "_v1_sigs", False):
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        h._v1_sigs = True
+    return h
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+@dataclass
+class Board:
+    header: Header
+    layers: C.Array
+    nodes: C.Array
+    runs: C.Array
+    run_nodes: C.Array
+
+    @property
+    def counts(self):
+        return len(self.layers), len(self.nodes), len(self.runs), len(self.run_nodes)
+
+    def node_index(self, x, y, layer=0):
+        for i, n in enumerate(self.nodes):
+            if n.lattice_x == x and n.lattice_y == y and n.layer == layer:
+                return i
+        raise KeyError((x, y, layer))
+
+    def fingerprint(self) -> int:
+        nl, nn, nr, nrn = self.counts
+        return _lib().apgx_v1_view_fingerprint(C.byref(self.header), self.layers, nl, self.nodes,
+                                               nn, self.runs, nr, self.run_nodes, nrn)
+
+
+def build_board(nodes, *, tile_w, tile_h, costs, heading_mask, step=10, origin=(0, 0),
+                compiler_version=1, source_hash=0, profile_fingerprint=0, rule_bucket=0) -> Board:
+    """Flattens compiled nodes [(x, y, layer, mask)] through apgx_v1_build_board."""
+    prof = CompiledProfile(compiler_version, heading_mask, source_hash, profile_fingerprint,
+                           rule_bucket, origin[0], origin[1], step, tile_w, tile_h, costs[0],
+                           costs[1], costs[2], 0)
+    arr = (CompiledNode * max(len(nodes), 1))()
+    for i, (x, y, l, m) in enumerate(nodes):
+        arr[i].lattice_x, arr[i].lattice_y, arr[i].layer, arr[i].legal_edges = x, y, l, m
+    h = _lib()
+    handle = C.c_void_p()
+    check(h.apgx_v1_build_board(C.byref(prof), arr, len(nodes), C.byref(handle)))
+    try:
+        nl, nn, nr, nrn = U64(), U64(), U64(), U64()
+        check(h.apgx_v1_board_sizes(handle, C.byref(nl), C.byref(nn), C.byref(nr), C.byref(nrn)))
+        header = Header()
+        layers = (LayerRange * nl.value)()
+        nodes_out = (Node * nn.value)()
+        runs = (Run * nr.value)()
+        run_nodes = (U32 * nrn.value)()
+        check(h.apgx_v1_board_copy(handle, C.byref(header), layers, nodes_out, runs, run_nodes))
+    finally:
+        h.apgx_v1_board_release(handle)
+    return Board(header, layers, nodes_out, runs, run_nodes)
+
+
+class View:
+    def __init__(self, board: Board):
+        nl, nn, nr, nrn = board.counts
+        self.board = board
+        self.handle = C.c_void_p()
+        check(_lib().apgx_v1_view_upload(C.byref(board.header), board.layers, nl, board.nodes, nn,
+                                         board.runs, nr, board.run_nodes, nrn, C.byref(self.handle)))
+
+    def close(self):
+        if self.handle:
+            _lib().apgx_v1_view_release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def execute(self, queries, policy_edges=(), *, batch_id=1, maximum_rounds=100_000,
+                generator=GENERATOR_SWEEP, maximum_device_bytes=(1 << 63), cancel=None):
+        """Runs a batch; returns (results, compact paths, packed states, telemetry)."""
+        S = len(self.board.nodes) * 9
+        req = BatchRequest(1, generator, batch_id, maximum_rounds, 0, S, S, maximum_device_bytes,
+                           1 << 63)
+        qarr = (Query * max(len(queries), 1))(*queries)
+        parr = (PolicyEdge * max(len(policy_edges), 1))(*policy_edges)
+        pending = C.c_void_p()
+        flag = (U32 * 1)(1 if cancel else 0)
+        check(_lib().apgx_v1_batch_execute(self.handle, C.byref(req), qarr, len(queries), parr,
+                                           len(policy_edges), flag if cancel is not None else None,
+                                           C.byref(pending)))
+        try:
+            used = U64()
+            results = (Result * max(len(queries), 1))()
+            paths = (CompactPath * max(len(queries), 1))()
+            check(_lib().apgx_v1_batch_readback(pending, results, paths, None, 0, C.byref(used), None))
+            states = (U32 * max(used.value, 1))()
+            tel = BatchTelemetry()
+            check(_lib().apgx_v1_batch_readback(pending, results, paths, states, used.value,
+                                                C.byref(used), C.byref(tel)))
+        finally:
+            _lib().apgx_v1_pending_release(pending)
+        n = len(queries)
+        return list(results)[:n], list(paths)[:n], list(states)[: used.value], tel
+
+
+def make_query(i, start, goal, board: Board, *, batch_id=1, policy_offset=0, policy_count=0,
+               surcharges=(0, 0, 0), maximum_rounds=100_000, generator=GENERATOR_SWEEP):
+    S = len(board.nodes) * 9
+    q = Query()
+    q.schema_version = 1
+    q.input_ordinal = i
+    q.batch_id = batch_id
+    q.query_id = i + 1
+    q.workspace_owner = i + 1
+    q.start_node, q.goal_node = start, goal
+    q.policy_offset, q.policy_count = policy_offset, policy_count
+    q.workspace_offset = i * S
+    q.workspace_state_count = S
+    q.orthogonal_step_surcharge, q.diagonal_step_surcharge, q.bend_surcharge = surcharges
+    q.maximum_rounds = maximum_rounds
+    q.generator = generator
+    return q
+
+
+def physical_edge_index(board: Board, x, y, direction, layer=0):
+    """Canonical physical edge id node*8 + dir (< 4) of a canonical (x, y, dir) resource."""
+    return board.node_index(x, y, layer) * 8 + direction

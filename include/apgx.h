@@ -207,6 +207,231 @@ int apgx_search_stats(const apgx_search* search, apgx_route_stats* out);
 void apgx_search_release(apgx_search* search);
 
 /* ------------------------------------------------------------------------ */
+/* V1: reference DeviceCompiledBoardV1 / DeviceCandidateBatch records        */
+/* ------------------------------------------------------------------------ */
+/* Byte-identical to include/apgar/gpu/planar_router.h:170-423 (sizes checked
+ * with static asserts in the library).  State index = node * 9 + incoming
+ * heading (0..7 = E, NE, N, NW, W, SW, S, SE; 8 = none, start only);
+ * UINT64_MAX labels are unreachable. */
+#define APGX_V1_SCHEMA_VERSION 1u
+#define APGX_V1_INVALID_INDEX 0xFFFFFFFFu
+#define APGX_V1_GENERATOR_BUCKETED_FRONTIER 0u
+#define APGX_V1_GENERATOR_HEADING_AWARE_SWEEP 1u
+#define APGX_V1_REACHED 0u
+#define APGX_V1_DISCONNECTED 1u
+#define APGX_V1_BUDGET_EXHAUSTED 2u
+#define APGX_V1_CANCELLED 3u
+#define APGX_V1_BANNED_ADJUSTMENT 0xFFFFFFFFFFFFFFFFull
+
+typedef struct apgx_v1_header {            /* DeviceCompiledHeaderV1, 112 bytes */
+  uint32_t schema_version;
+  uint32_t compiler_version;
+  uint64_t source_board_content_hash;
+  uint64_t compiler_profile_fingerprint;
+  uint64_t rule_bucket_identity;
+  uint64_t device_view_fingerprint;
+  uint64_t represented_nodes;
+  uint64_t represented_states;
+  uint64_t estimated_persistent_device_bytes;
+  int64_t lattice_origin_x;
+  int64_t lattice_origin_y;
+  int64_t lattice_step;
+  uint32_t orthogonal_step;
+  uint32_t diagonal_step;
+  uint32_t bend;
+  uint8_t heading_mask;                    /* 1 horizontal | 2 vertical | 4 diagonal45 */
+  uint8_t reserved[7];
+} apgx_v1_header;
+
+typedef struct apgx_v1_layer_range {       /* DeviceLayerRangeV1, 48 bytes */
+  uint32_t layer;
+  uint32_t node_offset;
+  uint32_t node_count;
+  uint32_t pad;
+  int64_t minimum_lattice_x;
+  int64_t maximum_lattice_x;
+  int64_t minimum_lattice_y;
+  int64_t maximum_lattice_y;
+} apgx_v1_layer_range;
+
+typedef struct apgx_v1_node {              /* DeviceNodeV1, 56 bytes */
+  int64_t lattice_x;
+  int64_t lattice_y;
+  uint32_t layer;
+  uint8_t legal_edges;
+  uint8_t reserved[3];
+  uint32_t neighbors[8];
+} apgx_v1_node;
+
+typedef struct apgx_v1_run {               /* DeviceRunV1, 12 bytes */
+  uint32_t node_offset;
+  uint32_t node_count;
+  uint8_t direction;
+  uint8_t reserved[3];
+} apgx_v1_run;
+
+typedef struct apgx_v1_policy_edge {       /* DeviceCandidatePolicyEdgeV1, 16 bytes */
+  uint32_t physical_edge_index;            /* canonical source node * 8 + direction (< 4) */
+  uint32_t reserved;
+  uint64_t adjustment;                     /* APGX_V1_BANNED_ADJUSTMENT bans the edge */
+} apgx_v1_policy_edge;
+
+typedef struct apgx_v1_query {             /* DeviceCandidateBatchQueryV1, 120 bytes */
+  uint32_t schema_version;
+  uint32_t input_ordinal;
+  uint64_t batch_id;
+  uint64_t query_id;
+  uint64_t workspace_owner;
+  uint64_t policy_identity;
+  uint64_t routing_profile_fingerprint;
+  uint32_t start_node;
+  uint32_t goal_node;
+  uint32_t policy_offset;
+  uint32_t policy_count;
+  uint64_t workspace_offset;
+  uint64_t workspace_state_count;
+  uint64_t frontier_state_capacity;
+  uint64_t orthogonal_step_surcharge;
+  uint64_t diagonal_step_surcharge;
+  uint64_t bend_surcharge;
+  uint32_t maximum_rounds;
+  uint8_t generator;
+  uint8_t reserved[3];
+} apgx_v1_query;
+
+typedef struct apgx_v1_result {            /* DeviceCandidateBatchResultV1, 120 bytes */
+  uint32_t schema_version;
+  uint32_t compiler_version;
+  uint32_t input_ordinal;
+  uint32_t start_node;
+  uint32_t goal_node;
+  uint32_t goal_state;
+  uint32_t rounds;
+  uint8_t generator;
+  uint8_t completion;                      /* APGX_V1_REACHED .. APGX_V1_CANCELLED */
+  uint8_t reserved[2];
+  uint64_t batch_id;
+  uint64_t query_id;
+  uint64_t workspace_owner;
+  uint64_t policy_identity;
+  uint64_t routing_profile_fingerprint;
+  uint64_t source_board_content_hash;
+  uint64_t compiler_profile_fingerprint;
+  uint64_t rule_bucket_identity;
+  uint64_t device_view_fingerprint;
+  uint64_t examined_work;
+  uint64_t heading_turn_relaxations;
+} apgx_v1_result;
+
+typedef struct apgx_v1_compact_path {      /* DeviceCandidateCompactPathV1, 32 bytes */
+  uint32_t schema_version;
+  uint32_t state_count;                    /* reverse (goal..start) states; 0 unless reached */
+  uint64_t query_id;
+  uint64_t workspace_owner;
+  uint64_t state_offset;
+} apgx_v1_compact_path;
+
+typedef struct apgx_v1_batch_request {     /* BackendCandidateBatchExecutionRequest scalars */
+  uint32_t schema_version;
+  uint32_t generator;
+  uint64_t batch_id;
+  uint32_t maximum_rounds;
+  uint32_t reserved;
+  uint64_t maximum_workspace_states_per_query;
+  uint64_t maximum_frontier_states_per_query;
+  uint64_t maximum_device_bytes;
+  uint64_t maximum_host_bytes;
+} apgx_v1_batch_request;
+
+typedef struct apgx_v1_batch_telemetry {   /* CandidateBatchTelemetry */
+  uint64_t persistent_device_bytes;
+  uint64_t batch_device_bytes;
+  uint64_t workspace_capacity_device_bytes;
+  uint64_t peak_device_bytes;
+  uint64_t batch_host_bytes;
+  uint64_t device_to_host_readback_bytes;
+  uint64_t kernel_launch_count;
+  uint64_t blocking_status_readback_count;
+  uint32_t dispatched_rounds;
+  uint32_t finalization_launch_count;
+  uint32_t chunk_rounds;
+  uint32_t reserved;
+  double kernel_milliseconds;
+} apgx_v1_batch_telemetry;
+
+/* Host-side inputs of BuildDeviceCompiledBoardV1: the compiled profile scalars
+ * and every represented node with its legal-direction mask, in any order. */
+typedef struct apgx_v1_compiled_profile {
+  uint32_t compiler_version;
+  uint32_t heading_mask;
+  uint64_t source_board_content_hash;
+  uint64_t compiler_profile_fingerprint;
+  uint64_t rule_bucket_identity;
+  int64_t lattice_origin_x;
+  int64_t lattice_origin_y;
+  int64_t lattice_step;
+  uint32_t tile_width_nodes;
+  uint32_t tile_height_nodes;
+  uint32_t orthogonal_step;
+  uint32_t diagonal_step;
+  uint32_t bend;
+  uint32_t reserved;
+} apgx_v1_compiled_profile;
+
+typedef struct apgx_v1_compiled_node {
+  int64_t lattice_x;
+  int64_t lattice_y;
+  uint32_t layer;
+  uint8_t legal_edges;
+  uint8_t reserved[3];
+} apgx_v1_compiled_node;
+
+typedef struct apgx_v1_board apgx_v1_board;        /* flattened view on the host */
+typedef struct apgx_v1_view apgx_v1_view;          /* uploaded immutable view */
+typedef struct apgx_v1_pending apgx_v1_pending;    /* executed batch awaiting readback */
+
+/* Replaces BuildDeviceCompiledBoardV1 (src/gpu/device_board.cc:136-325):
+ * tile-order flattening, neighbours, layer ranges, maximal directed runs,
+ * persistent bytes and the FNV-1a view fingerprint. */
+int apgx_v1_build_board(const apgx_v1_compiled_profile* profile, const apgx_v1_compiled_node* nodes,
+                        uint64_t node_count, apgx_v1_board** out);
+int apgx_v1_board_sizes(const apgx_v1_board* board, uint64_t* layer_count, uint64_t* node_count,
+                        uint64_t* run_count, uint64_t* run_node_count);
+int apgx_v1_board_copy(const apgx_v1_board* board, apgx_v1_header* header,
+                       apgx_v1_layer_range* layers, apgx_v1_node* nodes, apgx_v1_run* runs,
+                       uint32_t* run_nodes);
+void apgx_v1_board_release(apgx_v1_board* board);
+
+/* Replaces ComputeDeviceCompiledBoardFingerprintV1 (src/gpu/device_board.cc:82-134). */
+uint64_t apgx_v1_view_fingerprint(const apgx_v1_header* header, const apgx_v1_layer_range* layers,
+                                  uint64_t layer_count, const apgx_v1_node* nodes,
+                                  uint64_t node_count, const apgx_v1_run* runs, uint64_t run_count,
+                                  const uint32_t* run_nodes, uint64_t run_node_count);
+
+/* Replaces IPlanarRouteBackend::UploadCompiledView (include/apgar/gpu/planar_router.h:551):
+ * validates every structural invariant and the fingerprint, then uploads. */
+int apgx_v1_view_upload(const apgx_v1_header* header, const apgx_v1_layer_range* layers,
+                        uint64_t layer_count, const apgx_v1_node* nodes, uint64_t node_count,
+                        const apgx_v1_run* runs, uint64_t run_count, const uint32_t* run_nodes,
+                        uint64_t run_node_count, apgx_v1_view** out);
+void apgx_v1_view_release(apgx_v1_view* view);
+
+/* Replaces IPlanarRouteBackend::ExecuteCandidateBatch (include/apgar/gpu/planar_router.h:556-557)
+ * for the heading-aware sweep generator.  `cancel` (may be NULL) is sampled
+ * between round chunks; a nonzero value cancels the unfinished queries. */
+int apgx_v1_batch_execute(apgx_v1_view* view, const apgx_v1_batch_request* request,
+                          const apgx_v1_query* queries, uint32_t query_count,
+                          const apgx_v1_policy_edge* policy_edges, uint64_t policy_edge_count,
+                          const volatile uint32_t* cancel, apgx_v1_pending** out);
+/* Replaces IPlanarRouteBackend::ReadbackCandidateBatch (include/apgar/gpu/planar_router.h:558-559),
+ * compact-path form: results and compact headers in query order, reverse state
+ * lists packed into `states`. */
+int apgx_v1_batch_readback(apgx_v1_pending* pending, apgx_v1_result* results,
+                           apgx_v1_compact_path* paths, uint32_t* states, uint64_t state_capacity,
+                           uint64_t* states_used, apgx_v1_batch_telemetry* telemetry);
+void apgx_v1_pending_release(apgx_v1_pending* pending);
+
+/* ------------------------------------------------------------------------ */
 /* Synthetic lattice generators (deterministic xoshiro256** streams)         */
 /* ------------------------------------------------------------------------ */
 

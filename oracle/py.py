@@ -114,3 +114,91 @@ class Dense:
                 conns.append((int(cp[i]), int(cd[i]), cells[off:off + cl[i]].copy()))
                 off += int(cl[i])
         return rc, conns
+
+
+# ------------------------------------------------------------------ V1 model
+class V1Node(C.Structure):
+    _fields_ = [("x", C.c_int64), ("y", C.c_int64), ("layer", C.c_uint32), ("mask", C.c_uint8),
+                ("pad", C.c_uint8 * 3)]
+
+
+class V1Profile(C.Structure):
+    _fields_ = [("orth", C.c_uint32), ("diag", C.c_uint32), ("bend", C.c_uint32),
+                ("heading_mask", C.c_uint32), ("tile_w", C.c_uint32), ("tile_h", C.c_uint32)]
+
+
+class V1Policy(C.Structure):
+    _fields_ = [("orth_s", C.c_uint64), ("diag_s", C.c_uint64), ("bend_s", C.c_uint64),
+                ("ban_keys", C.POINTER(C.c_int64)), ("n_bans", C.c_uint32), ("n_pens", C.c_uint32),
+                ("pen_keys", C.POINTER(C.c_int64)), ("pen_vals", C.POINTER(C.c_uint64))]
+
+
+def _v1_setup():
+    h = lib()
+    if getattr(h, "_v1_ready", False):
+        return h
+    P = C.POINTER
+    h.orc_v1_astar.restype = C.c_int
+    h.orc_v1_astar.argtypes = [P(V1Node), C.c_uint64, P(V1Profile), P(V1Policy), C.c_uint32,
+                               C.c_int64, C.c_int64, C.c_int64, C.c_int64, P(C.c_uint64),
+                               P(C.c_int64), C.c_uint64, P(C.c_uint64), P(C.c_uint64)]
+    h.orc_v1_flatten_order.restype = C.c_int
+    h.orc_v1_flatten_order.argtypes = [P(V1Node), C.c_uint64, C.c_uint32, C.c_uint32, P(C.c_uint32)]
+    h.orc_v1_canonical.restype = C.c_int
+    h.orc_v1_canonical.argtypes = [P(V1Node), C.c_uint64, P(V1Profile), P(V1Policy), C.c_uint32,
+                                   C.c_int64, C.c_int64, C.c_int64, C.c_int64, P(C.c_uint64),
+                                   P(C.c_uint32), C.c_uint64, P(C.c_uint64)]
+    h._v1_ready = True
+    return h
+
+
+class V1:
+    """Planar heading-state oracle over compiled nodes [(x, y, layer, mask)]."""
+
+    def __init__(self, nodes, costs, heading_mask, tile_w, tile_h):
+        self.n = len(nodes)
+        self.nodes = (V1Node * max(self.n, 1))()
+        for i, (x, y, l, m) in enumerate(nodes):
+            self.nodes[i].x, self.nodes[i].y, self.nodes[i].layer, self.nodes[i].mask = x, y, l, m
+        self.profile = V1Profile(costs[0], costs[1], costs[2], heading_mask, tile_w, tile_h)
+
+    @staticmethod
+    def policy(orth_s=0, diag_s=0, bend_s=0, bans=(), penalties=()):
+        bk = (C.c_int64 * max(3 * len(bans), 1))(*[v for b in bans for v in b])
+        pk = (C.c_int64 * max(3 * len(penalties), 1))(*[v for (k, _) in penalties for v in k])
+        pv = (C.c_uint64 * max(len(penalties), 1))(*[v for (_, v) in penalties])
+        pol = V1Policy(orth_s, diag_s, bend_s, bk, len(bans), len(penalties), pk, pv)
+        pol._keep = (bk, pk, pv)
+        return pol
+
+    def astar(self, start, goal, policy=None, layer=0):
+        h = _v1_setup()
+        cost, plen, exp = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+        path = (C.c_int64 * (2 * (9 * self.n + 2)))()
+        rc = h.orc_v1_astar(self.nodes, self.n, C.byref(self.profile),
+                            C.byref(policy) if policy is not None else None, layer, start[0],
+                            start[1], goal[0], goal[1], C.byref(cost), path, 9 * self.n + 2,
+                            C.byref(plen), C.byref(exp))
+        if rc < 0:
+            raise RuntimeError("oracle v1 astar error")
+        pts = [(path[2 * i], path[2 * i + 1]) for i in range(plen.value)][::-1]
+        return (None if rc == 1 else cost.value), pts, exp.value
+
+    def flatten_order(self, tile_w, tile_h):
+        h = _v1_setup()
+        order = (C.c_uint32 * max(self.n, 1))()
+        if h.orc_v1_flatten_order(self.nodes, self.n, tile_w, tile_h, order):
+            raise RuntimeError("flatten order failed")
+        return list(order)[: self.n]
+
+    def canonical(self, start, goal, policy=None, layer=0):
+        h = _v1_setup()
+        cost, plen = C.c_uint64(0), C.c_uint64(0)
+        states = (C.c_uint32 * (9 * self.n + 2))()
+        rc = h.orc_v1_canonical(self.nodes, self.n, C.byref(self.profile),
+                                C.byref(policy) if policy is not None else None, layer, start[0],
+                                start[1], goal[0], goal[1], C.byref(cost), states, 9 * self.n + 2,
+                                C.byref(plen))
+        if rc < 0:
+            raise RuntimeError("oracle v1 canonical error")
+        return (None if rc == 1 else cost.value), list(states)[: plen.value]

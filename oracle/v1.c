@@ -1,0 +1,421 @@
+/*
+ * v1.c — CPU oracle for the reference's planar heading-state model (V1
+ * DeviceCompiledBoard semantics).  TEST INFRASTRUCTURE ONLY: loaded by tests/
+ * and the smoke check, never by the product library.
+ *
+ * Model (include/apgar/geometry_compiler/compiled_board.h:23-78,
+ * src/routing/candidate_policy.cc:419-438):
+ *   directions 0..7 = E, NE, N, NW, W, SW, S, SE; deltas as DeltaFor;
+ *   state = (node, incoming heading 0..7 or 8 = none);
+ *   w(u->v, d | h) = (diag ? diag_step + diag_surcharge : orth_step + orth_surcharge)
+ *                    + [h not in {8, d}] * (bend + bend_surcharge)
+ *                    + penalty(canonical undirected edge);
+ *   banned edges are absent; sums reaching UINT64_MAX are rejected.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle_v1.h"
+
+static const int DX[8] = {1, 1, 0, -1, -1, -1, 0, 1};
+static const int DY[8] = {0, 1, 1, 1, 0, -1, -1, -1};
+#define INF64 UINT64_MAX
+
+/* --------------------------------------------------------- node lookup -- */
+typedef struct {
+  int64_t x, y;
+  uint32_t layer;
+  uint32_t idx;
+} keyed_t;
+
+static int key_cmp(const void* a, const void* b) {
+  const keyed_t* p = (const keyed_t*)a;
+  const keyed_t* q = (const keyed_t*)b;
+  if (p->layer != q->layer) return p->layer < q->layer ? -1 : 1;
+  if (p->x != q->x) return p->x < q->x ? -1 : 1;
+  if (p->y != q->y) return p->y < q->y ? -1 : 1;
+  return 0;
+}
+
+typedef struct {
+  keyed_t* keys;
+  uint64_t n;
+} lookup_t;
+
+static int lookup_init(lookup_t* lk, const orc_v1_node* nodes, uint64_t n) {
+  lk->keys = (keyed_t*)malloc(sizeof(keyed_t) * (n ? n : 1));
+  if (!lk->keys) return -1;
+  lk->n = n;
+  for (uint64_t i = 0; i < n; ++i) {
+    lk->keys[i].x = nodes[i].x;
+    lk->keys[i].y = nodes[i].y;
+    lk->keys[i].layer = nodes[i].layer;
+    lk->keys[i].idx = (uint32_t)i;
+  }
+  qsort(lk->keys, n, sizeof(keyed_t), key_cmp);
+  return 0;
+}
+
+static int64_t lookup(const lookup_t* lk, uint32_t layer, int64_t x, int64_t y) {
+  keyed_t k = {x, y, layer, 0};
+  uint64_t lo = 0, hi = lk->n;
+  while (lo < hi) {
+    uint64_t mid = lo + (hi - lo) / 2;
+    int c = key_cmp(&lk->keys[mid], &k);
+    if (c == 0) return lk->keys[mid].idx;
+    if (c < 0) lo = mid + 1; else hi = mid;
+  }
+  return -1;
+}
+
+/* ------------------------------------------------------- policy lookup -- */
+/* canonical physical edge (CanonicalPhysicalEdgeResource, candidate_policy.cc:72-116) */
+static void canonical_edge(int64_t x, int64_t y, int dir, int64_t* cx, int64_t* cy, int* cd) {
+  *cx = x; *cy = y; *cd = dir;
+  switch (dir) {
+    case 4: *cx = x - 1; *cd = 0; break;
+    case 5: *cx = x - 1; *cy = y - 1; *cd = 1; break;
+    case 6: *cy = y - 1; *cd = 2; break;
+    case 7: *cx = x + 1; *cy = y - 1; *cd = 3; break;
+    default: break;
+  }
+}
+
+static int policy_find(const int64_t* keys, uint32_t n, int64_t x, int64_t y, int d) {
+  for (uint32_t i = 0; i < n; ++i)
+    if (keys[3 * i] == x && keys[3 * i + 1] == y && keys[3 * i + 2] == d) return (int)i;
+  return -1;
+}
+
+/* Returns 0 and the transition cost, or -1 if the edge is banned/overflows. */
+static int transition(const orc_v1_profile* pr, const orc_v1_policy* po, int64_t x, int64_t y,
+                      int dir, int incoming, uint64_t* out) {
+  int64_t cx, cy;
+  int cd;
+  canonical_edge(x, y, dir, &cx, &cy, &cd);
+  if (po && policy_find(po->ban_keys, po->n_bans, cx, cy, cd) >= 0) return -1;
+  unsigned __int128 c = (dir & 1) ? (unsigned __int128)pr->diag + (po ? po->diag_s : 0)
+                                  : (unsigned __int128)pr->orth + (po ? po->orth_s : 0);
+  if (incoming != 8 && incoming != dir) c += (unsigned __int128)pr->bend + (po ? po->bend_s : 0);
+  if (po) {
+    /* duplicate penalty entries merge by summation (NormalizeCandidateGenerationPolicy) */
+    for (uint32_t i = 0; i < po->n_pens; ++i)
+      if (po->pen_keys[3 * i] == cx && po->pen_keys[3 * i + 1] == cy && po->pen_keys[3 * i + 2] == cd)
+        c += po->pen_vals[i];
+  }
+  if (c >= (unsigned __int128)INF64) return -1;
+  *out = (uint64_t)c;
+  return 0;
+}
+
+static int checked_add(uint64_t a, uint64_t b, uint64_t* out) {
+  const uint64_t maxf = INF64 - 1;
+  if (a > maxf || b > maxf - a) return -1;
+  *out = a + b;
+  return 0;
+}
+
+static int heading_allowed(const orc_v1_profile* pr, int dir) {
+  int h = (DY[dir] == 0) ? 1 : (DX[dir] == 0 ? 2 : 4);
+  return (pr->heading_mask & h) != 0;
+}
+
+/* ------------------------------------------------------------- heap -- */
+typedef struct {
+  uint64_t k[7];
+} item_t;
+typedef struct {
+  item_t* a;
+  uint64_t n, cap;
+} pq_t;
+
+static int item_less(const item_t* p, const item_t* q) {
+  for (int i = 0; i < 7; ++i)
+    if (p->k[i] != q->k[i]) return p->k[i] < q->k[i];
+  return 0;
+}
+static int pq_push(pq_t* h, item_t it) {
+  if (h->n == h->cap) {
+    uint64_t nc = h->cap ? h->cap * 2 : 64;
+    item_t* na = (item_t*)realloc(h->a, sizeof(item_t) * nc);
+    if (!na) return -1;
+    h->a = na;
+    h->cap = nc;
+  }
+  uint64_t i = h->n++;
+  h->a[i] = it;
+  while (i) {
+    uint64_t p = (i - 1) / 2;
+    if (!item_less(&h->a[i], &h->a[p])) break;
+    item_t t = h->a[i]; h->a[i] = h->a[p]; h->a[p] = t;
+    i = p;
+  }
+  return 0;
+}
+static item_t pq_pop(pq_t* h) {
+  item_t top = h->a[0];
+  h->a[0] = h->a[--h->n];
+  uint64_t i = 0;
+  for (;;) {
+    uint64_t l = 2 * i + 1, r = l + 1, m = i;
+    if (l < h->n && item_less(&h->a[l], &h->a[m])) m = l;
+    if (r < h->n && item_less(&h->a[r], &h->a[m])) m = r;
+    if (m == i) break;
+    item_t t = h->a[i]; h->a[i] = h->a[m]; h->a[m] = t;
+    i = m;
+  }
+  return top;
+}
+
+/* Heuristic (cpu_astar.cc:159-212) */
+static uint64_t heuristic(const orc_v1_profile* pr, const orc_v1_policy* po, int64_t x, int64_t y,
+                          int64_t gx, int64_t gy) {
+  uint64_t dx = x > gx ? (uint64_t)(x - gx) : (uint64_t)(gx - x);
+  uint64_t dy = y > gy ? (uint64_t)(y - gy) : (uint64_t)(gy - y);
+  int hz = pr->heading_mask & 1, vt = pr->heading_mask & 2, dg = pr->heading_mask & 4;
+  uint64_t orth = (uint64_t)pr->orth + (po ? po->orth_s : 0);
+  uint64_t diag = (uint64_t)pr->diag + (po ? po->diag_s : 0);
+  unsigned __int128 e = 0;
+  uint64_t mx = dx > dy ? dx : dy, mn = dx < dy ? dx : dy;
+  if (hz && vt) {
+    if (dg) {
+      if (diag < orth) {
+        e = (unsigned __int128)mx * diag;
+      } else {
+        uint64_t eff = (unsigned __int128)diag < (unsigned __int128)orth * 2 ? diag : orth * 2;
+        e = (unsigned __int128)mn * eff + (unsigned __int128)(mx - mn) * orth;
+      }
+    } else {
+      e = ((unsigned __int128)dx + dy) * orth;
+    }
+  } else if (hz && !vt && !dg) {
+    e = (unsigned __int128)dx * orth;
+  } else if (vt && !hz && !dg) {
+    e = (unsigned __int128)dy * orth;
+  } else if (dg) {
+    uint64_t m = diag;
+    if ((hz || vt) && orth < m) m = orth;
+    e = (unsigned __int128)mx * m;
+  } else {
+    return 0;
+  }
+  if (e > (unsigned __int128)INF64) return 0;
+  return (uint64_t)e;
+}
+
+int orc_v1_astar(const orc_v1_node* nodes, uint64_t n, const orc_v1_profile* pr,
+                 const orc_v1_policy* po, uint32_t layer, int64_t sx, int64_t sy, int64_t gx,
+                 int64_t gy, uint64_t* cost_out, int64_t* path_xy, uint64_t path_cap,
+                 uint64_t* path_len, uint64_t* expanded_out) {
+  lookup_t lk;
+  if (lookup_init(&lk, nodes, n)) return -1;
+  int64_t s = lookup(&lk, layer, sx, sy), g = lookup(&lk, layer, gx, gy);
+  if (s < 0 || g < 0) { free(lk.keys); return -1; }
+  uint64_t S = n * 9;
+  uint64_t* best = (uint64_t*)malloc(sizeof(uint64_t) * S);
+  int64_t* pred = (int64_t*)malloc(sizeof(int64_t) * S);
+  if (!best || !pred) { free(best); free(pred); free(lk.keys); return -1; }
+  for (uint64_t i = 0; i < S; ++i) { best[i] = INF64; pred[i] = -1; }
+  pq_t q = {0, 0, 0};
+  uint64_t seq = 0, expanded = 0;
+  uint64_t h0 = heuristic(pr, po, sx, sy, gx, gy);
+  best[s * 9 + 8] = 0;
+  item_t it0 = {{h0, h0, 0, (uint64_t)sy, (uint64_t)sx, 8, seq++}};
+  pq_push(&q, it0);
+  int64_t goal_state = -1;
+  int err = 0;
+  while (q.n) {
+    item_t cur = pq_pop(&q);
+    int64_t cx = (int64_t)cur.k[4], cy = (int64_t)cur.k[3];
+    int inc = (int)cur.k[5];
+    int64_t u = lookup(&lk, layer, cx, cy);
+    uint64_t st = (uint64_t)u * 9 + inc;
+    if (best[st] != cur.k[2]) continue;               /* lazy deletion */
+    if (cx == gx && cy == gy) { goal_state = (int64_t)st; break; }  /* goal on pop */
+    ++expanded;
+    for (int d = 0; d < 8; ++d) {                      /* stable direction order */
+      if (!heading_allowed(pr, d)) continue;
+      if (!(nodes[u].mask & (1u << d))) continue;
+      int64_t vx = cx + DX[d], vy = cy + DY[d];
+      int64_t v = lookup(&lk, layer, vx, vy);
+      if (v < 0) { err = 1; break; }
+      uint64_t w;
+      if (transition(pr, po, cx, cy, d, inc, &w)) continue;   /* banned */
+      uint64_t nc;
+      if (checked_add(cur.k[2], w, &nc)) { err = 1; break; }
+      uint64_t vs = (uint64_t)v * 9 + d;
+      if (best[vs] <= nc) continue;                    /* strict improvement */
+      best[vs] = nc;
+      pred[vs] = (int64_t)st;
+      uint64_t hv = heuristic(pr, po, vx, vy, gx, gy);
+      uint64_t f;
+      if (checked_add(nc, hv, &f)) f = INF64;
+      item_t it = {{f, hv, nc, (uint64_t)vy, (uint64_t)vx, (uint64_t)d, seq++}};
+      if (pq_push(&q, it)) { err = 1; break; }
+    }
+    if (err) break;
+  }
+  if (expanded_out) *expanded_out = expanded;
+  int rc = 0;
+  if (err) rc = -1;
+  else if (goal_state < 0) { rc = 1; if (cost_out) *cost_out = INF64; }
+  else {
+    if (cost_out) *cost_out = best[goal_state];
+    uint64_t k = 0;
+    for (int64_t c = goal_state; c >= 0 && k < path_cap; c = pred[c]) {
+      if (path_xy) {
+        path_xy[2 * k] = nodes[c / 9].x;
+        path_xy[2 * k + 1] = nodes[c / 9].y;
+      }
+      ++k;
+    }
+    if (path_len) *path_len = k;
+  }
+  free(q.a);
+  free(best);
+  free(pred);
+  free(lk.keys);
+  return rc;
+}
+
+/* -------------------------------------------- flattened order & labels -- */
+static int64_t floordiv(int64_t a, int64_t b) {
+  int64_t q = a / b;
+  if ((a % b) < 0) --q;
+  return q;
+}
+
+typedef struct {
+  uint32_t layer;
+  int64_t tx, ty;
+  uint64_t local;
+  uint32_t idx;
+} tileord_t;
+
+static int tileord_cmp(const void* a, const void* b) {
+  const tileord_t* p = (const tileord_t*)a;
+  const tileord_t* q = (const tileord_t*)b;
+  if (p->layer != q->layer) return p->layer < q->layer ? -1 : 1;
+  if (p->tx != q->tx) return p->tx < q->tx ? -1 : 1;
+  if (p->ty != q->ty) return p->ty < q->ty ? -1 : 1;
+  if (p->local != q->local) return p->local < q->local ? -1 : 1;
+  return 0;
+}
+
+int orc_v1_flatten_order(const orc_v1_node* nodes, uint64_t n, uint32_t tile_w, uint32_t tile_h,
+                         uint32_t* order) {
+  /* Sparse-tile order: TileKey (layer, tile x, tile y), then row-major local
+   * index inside the tile (compiled_board.cc:240-266, device_board.cc:167-193). */
+  tileord_t* t = (tileord_t*)malloc(sizeof(tileord_t) * (n ? n : 1));
+  if (!t) return -1;
+  for (uint64_t i = 0; i < n; ++i) {
+    int64_t tx = floordiv(nodes[i].x, tile_w), ty = floordiv(nodes[i].y, tile_h);
+    uint64_t lx = (uint64_t)(nodes[i].x - tx * (int64_t)tile_w);
+    uint64_t ly = (uint64_t)(nodes[i].y - ty * (int64_t)tile_h);
+    t[i].layer = nodes[i].layer;
+    t[i].tx = tx;
+    t[i].ty = ty;
+    t[i].local = ly * tile_w + lx;
+    t[i].idx = (uint32_t)i;
+  }
+  qsort(t, n, sizeof(tileord_t), tileord_cmp);
+  for (uint64_t i = 0; i < n; ++i) order[i] = t[i].idx;   /* flattened position -> input node */
+  free(t);
+  return 0;
+}
+
+/* Exact labels over (flattened node, heading) states by Dijkstra, then the
+ * lowest-predecessor-state walk of gpu/cuda/planar_router.cu:1084-1165.
+ * states_out gets the reverse state path (goal..start) in flattened numbering. */
+int orc_v1_canonical(const orc_v1_node* nodes, uint64_t n, const orc_v1_profile* pr,
+                     const orc_v1_policy* po, uint32_t layer, int64_t sx, int64_t sy, int64_t gx,
+                     int64_t gy, uint64_t* cost_out, uint32_t* states_out, uint64_t cap,
+                     uint64_t* len_out) {
+  uint32_t* order = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+  uint32_t* pos = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+  orc_v1_node* flat = (orc_v1_node*)malloc(sizeof(orc_v1_node) * (n ? n : 1));
+  if (!order || !pos || !flat || orc_v1_flatten_order(nodes, n, pr->tile_w, pr->tile_h, order)) {
+    free(order); free(pos); free(flat); return -1;
+  }
+  for (uint64_t i = 0; i < n; ++i) { flat[i] = nodes[order[i]]; pos[order[i]] = (uint32_t)i; }
+  lookup_t lk;
+  if (lookup_init(&lk, flat, n)) { free(order); free(pos); free(flat); return -1; }
+  int64_t s = lookup(&lk, layer, sx, sy), g = lookup(&lk, layer, gx, gy);
+  uint64_t S = n * 9;
+  uint64_t* L = (uint64_t*)malloc(sizeof(uint64_t) * (S ? S : 1));
+  int rc = 0;
+  if (s < 0 || g < 0 || !L) { rc = -1; goto done; }
+  for (uint64_t i = 0; i < S; ++i) L[i] = INF64;
+  {
+    pq_t q = {0, 0, 0};
+    L[s * 9 + 8] = 0;
+    item_t it0 = {{0, (uint64_t)s * 9 + 8, 0, 0, 0, 0, 0}};
+    pq_push(&q, it0);
+    while (q.n) {
+      item_t cur = pq_pop(&q);
+      uint64_t st = cur.k[1];
+      if (L[st] != cur.k[0]) continue;
+      uint64_t u = st / 9;
+      int inc = (int)(st % 9);
+      for (int d = 0; d < 8; ++d) {
+        if (!heading_allowed(pr, d) || !(flat[u].mask & (1u << d))) continue;
+        int64_t v = lookup(&lk, layer, flat[u].x + DX[d], flat[u].y + DY[d]);
+        if (v < 0) { rc = -1; break; }
+        uint64_t w, nc;
+        if (transition(pr, po, flat[u].x, flat[u].y, d, inc, &w)) continue;
+        if (checked_add(cur.k[0], w, &nc)) continue;
+        uint64_t vs = (uint64_t)v * 9 + d;
+        if (nc < L[vs]) {
+          L[vs] = nc;
+          item_t it = {{nc, vs, 0, 0, 0, 0, 0}};
+          if (pq_push(&q, it)) { rc = -1; break; }
+        }
+      }
+      if (rc) break;
+    }
+    free(q.a);
+  }
+  if (rc) goto done;
+  {
+    /* goal state: lowest state index among the minimal goal headings 0..7 */
+    uint64_t best = INF64, bs = 0;
+    for (int h = 0; h < 8; ++h) {
+      uint64_t st = (uint64_t)g * 9 + h;
+      if (L[st] < best) { best = L[st]; bs = st; }
+    }
+    if (best == INF64) { rc = 1; if (cost_out) *cost_out = INF64; goto done; }
+    if (cost_out) *cost_out = best;
+    uint64_t k = 0, cur = bs, start_state = (uint64_t)s * 9 + 8;
+    for (;;) {
+      if (k >= cap || k > S) { rc = -1; break; }
+      states_out[k++] = (uint32_t)cur;
+      if (cur == start_state) break;
+      uint64_t v = cur / 9;
+      int inc = (int)(cur % 9);
+      if (inc == 8) { rc = -1; break; }
+      int opp = (inc + 4) % 8;
+      int64_t p = lookup(&lk, layer, flat[v].x + DX[opp], flat[v].y + DY[opp]);
+      if (p < 0 || !(flat[p].mask & (1u << inc))) { rc = -1; break; }
+      uint64_t chosen = INF64;
+      for (int ph = 0; ph < 9; ++ph) {
+        if (ph == 8 && p != s) continue;
+        uint64_t ps = (uint64_t)p * 9 + ph;
+        if (L[ps] == INF64) continue;
+        uint64_t w, c;
+        if (transition(pr, po, flat[p].x, flat[p].y, inc, ph, &w)) continue;
+        if (checked_add(L[ps], w, &c)) continue;
+        if (c == L[cur] && ps < chosen) chosen = ps;
+      }
+      if (chosen == INF64) { rc = -1; break; }
+      cur = chosen;
+    }
+    if (len_out) *len_out = k;
+  }
+done:
+  free(L);
+  free(lk.keys);
+  free(order);
+  free(pos);
+  free(flat);
+  return rc;
+}

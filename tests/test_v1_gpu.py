@@ -1,0 +1,161 @@
+"""GPU parity of the V1 heading-aware sweep on the reference corpus.
+
+Checks every query's reverse compact path against the oracle's
+lowest-predecessor path, recomputes the path cost and compares it with the
+reference's published optimal cost, and compares the round count and examined
+work with the reference's published sweep telemetry.
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from apgar_amd import v1
+from tests import v1_corpus as vc
+from tests.test_v1_board import board_for
+
+pytestmark = pytest.mark.gpu
+GOLDEN = json.loads((Path(__file__).parent / "golden" / "corpus_golden.json").read_text())
+
+
+def path_cost(board, states, costs, surcharges=(0, 0, 0), adjustments=None):
+    """Cost of a reverse state path under the V1 transition model."""
+    adjustments = adjustments or {}
+    orth, diag, bend = costs
+    total = 0
+    seq = list(reversed(states))  # start .. goal
+    for p, s in zip(seq[:-1], seq[1:]):
+        pn, ph = divmod(p, 9)
+        sn, sh = divmod(s, 9)
+        assert board.nodes[pn].neighbors[sh] == sn
+        w = (diag + surcharges[1]) if sh % 2 else (orth + surcharges[0])
+        if ph != 8 and ph != sh:
+            w += bend + surcharges[2]
+        if sh < 4:
+            edge = pn * 8 + sh
+        else:
+            edge = board.nodes[pn].neighbors[sh] * 8 + sh - 4
+        adj = adjustments.get(edge, 0)
+        assert adj != v1.BANNED
+        total += w + adj
+    return total
+
+
+@pytest.mark.parametrize("name", sorted(vc.corpus()))
+def test_corpus_case(apgx, oracle, gpu, name):
+    print(f"[case] v1 corpus {name}", flush=True)
+    profile, start, goal = vc.corpus()[name]
+    nodes, b = board_for(profile)
+    s_xy, g_xy = vc.lattice(profile, start), vc.lattice(profile, goal)
+    s, g = b.node_index(*s_xy), b.node_index(*g_xy)
+    view = v1.View(b)
+    results, paths, states, tel = view.execute([v1.make_query(0, s, g, b)])
+    r, p = results[0], paths[0]
+    gold = GOLDEN[name]
+    o = oracle.V1(nodes, profile.costs, profile.heading, profile.tile_w, profile.tile_h)
+    ocost, ostates = o.canonical(s_xy, g_xy)
+    if name == "disconnected_fields":
+        assert r.completion == v1.DISCONNECTED and p.state_count == 0 and ocost is None
+    else:
+        assert r.completion == v1.REACHED
+        got = states[p.state_offset:p.state_offset + p.state_count]
+        assert got == ostates
+        want = gold.get("cuda_sweep_path_cost", gold.get("base_policy_scalar_cost"))
+        assert path_cost(b, got, profile.costs) == want == ocost
+        assert r.goal_state == got[0]
+    if "cuda_sweep_rounds" in gold:
+        assert r.rounds == gold["cuda_sweep_rounds"]
+        assert r.examined_work == gold["cuda_sweep_examined_work"]
+    assert r.query_id == 1 and r.workspace_owner == 1
+    assert r.device_view_fingerprint == b.header.device_view_fingerprint
+    view.close()
+
+
+def test_policy_known_answers(apgx, gpu):
+    print("[case] v1 policy known answers", flush=True)
+    profile, start, goal = vc.known_answer_boards()["horizontal_detour"]
+    _, b = board_for(profile)
+    s, g = b.node_index(*vc.lattice(profile, start)), b.node_index(*vc.lattice(profile, goal))
+    middle = v1.physical_edge_index(b, 4, 0, 0)
+    view = v1.View(b)
+    edges = [v1.PolicyEdge(middle, 0, v1.BANNED), v1.PolicyEdge(middle, 0, 200)]
+    qs = [v1.make_query(0, s, g, b),
+          v1.make_query(1, s, g, b, policy_offset=0, policy_count=1),
+          v1.make_query(2, s, g, b, policy_offset=1, policy_count=1)]
+    results, paths, states, _ = view.execute(qs, edges)
+    costs = []
+    for i, (r, p) in enumerate(zip(results, paths)):
+        assert r.completion == v1.REACHED
+        got = states[p.state_offset:p.state_offset + p.state_count]
+        adj = {middle: edges[i - 1].adjustment} if i else {}
+        costs.append(path_cost(b, got, profile.costs, adjustments=adj))
+    assert costs == [100, 126, 126]
+    view.close()
+
+    profile, start, goal = vc.known_answer_boards()["horizontal_corridor"]
+    _, b = board_for(profile)
+    s, g = b.node_index(*vc.lattice(profile, start)), b.node_index(*vc.lattice(profile, goal))
+    middle = v1.physical_edge_index(b, 4, 0, 0)
+    view = v1.View(b)
+    edges = [v1.PolicyEdge(middle, 0, 17), v1.PolicyEdge(middle, 0, v1.BANNED)]
+    qs = [v1.make_query(0, s, g, b, policy_count=1, surcharges=(2, 3, 5)),
+          v1.make_query(1, g, s, b, policy_count=1, surcharges=(2, 3, 5)),
+          v1.make_query(2, s, g, b, policy_offset=1, policy_count=1)]
+    results, paths, states, _ = view.execute(qs, edges)
+    for i in (0, 1):
+        p = paths[i]
+        got = states[p.state_offset:p.state_offset + p.state_count]
+        assert path_cost(b, got, profile.costs, (2, 3, 5), {middle: 17}) == 137
+    assert results[2].completion == v1.DISCONNECTED
+    view.close()
+
+
+def test_cheap_diagonal_cost(apgx, oracle, gpu):
+    profile, start, goal = vc.known_answer_boards()["cheap_diagonal"]
+    nodes, b = board_for(profile)
+    s, g = b.node_index(*vc.lattice(profile, start)), b.node_index(*vc.lattice(profile, goal))
+    view = v1.View(b)
+    results, paths, states, _ = view.execute([v1.make_query(0, s, g, b)])
+    p = paths[0]
+    got = states[p.state_offset:p.state_offset + p.state_count]
+    assert path_cost(b, got, profile.costs) == 37
+    o = oracle.V1(nodes, profile.costs, profile.heading, profile.tile_w, profile.tile_h)
+    assert o.canonical(vc.lattice(profile, start), vc.lattice(profile, goal))[1] == got
+    view.close()
+
+
+def test_batch_of_random_queries(apgx, oracle, gpu):
+    print("[case] v1 512-query batch on negative_coordinates", flush=True)
+    profile, _, _ = vc.corpus()["negative_coordinates"]
+    nodes, b = board_for(profile)
+    rng = np.random.default_rng(3)
+    n = len(b.nodes)
+    pairs = [tuple(rng.choice(n, size=2, replace=False)) for _ in range(512)]
+    qs = [v1.make_query(i, int(s), int(g), b) for i, (s, g) in enumerate(pairs)]
+    view = v1.View(b)
+    results, paths, states, tel = view.execute(qs)
+    o = oracle.V1(nodes, profile.costs, profile.heading, profile.tile_w, profile.tile_h)
+    for i in range(0, 512, 37):
+        s, g = pairs[i]
+        sx = (b.nodes[s].lattice_x, b.nodes[s].lattice_y)
+        gx = (b.nodes[g].lattice_x, b.nodes[g].lattice_y)
+        cost, ost = o.canonical(sx, gx)
+        p = paths[i]
+        got = states[p.state_offset:p.state_offset + p.state_count]
+        assert got == ost and path_cost(b, got, profile.costs) == cost
+    assert tel.kernel_milliseconds > 0 and tel.chunk_rounds == 8
+    view.close()
+
+
+def test_cancel_and_budget(apgx, gpu):
+    profile, start, goal = vc.corpus()["high_turn_maze"]
+    _, b = board_for(profile)
+    s, g = b.node_index(*vc.lattice(profile, start)), b.node_index(*vc.lattice(profile, goal))
+    view = v1.View(b)
+    r, p, _, _ = view.execute([v1.make_query(0, s, g, b)], cancel=True)
+    assert r[0].completion == v1.CANCELLED and p[0].state_count == 0
+    r, p, _, tel = view.execute([v1.make_query(0, s, g, b, maximum_rounds=2)], maximum_rounds=2)
+    assert r[0].completion == v1.BUDGET_EXHAUSTED and r[0].rounds == 2
+    assert tel.finalization_launch_count == 1
+    view.close()
