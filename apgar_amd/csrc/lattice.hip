@@ -51,6 +51,7 @@ constexpr uint32_t kSourceWord = 7u;           // distance 0, no predecessor
 constexpr uint32_t kFromBelow = 0, kFromSouth = 1, kFromWest = 2, kFromEast = 3,
                    kFromNorth = 4, kFromAbove = 5;
 constexpr uint32_t kSearching = 0xFFu;
+constexpr uint32_t kFinalWalk = 0xFEu;  // last pin settled; path written after the round loop
 constexpr int kThreads = 256;
 constexpr int kMaxTileCells = 16384;
 constexpr int kKeep = kMaxTileCells / kThreads;  // labels kept in registers per thread
@@ -104,6 +105,15 @@ struct SearchArgs {
   unsigned long long cell_capacity;
   uint32_t round;      // round being executed (round kernel) / next round (phase kernel)
   uint32_t max_sweeps;
+  // Distance-ordered scheduling (delta-stepping over tiles).  tile_key holds,
+  // per round parity, the smallest distance that can newly enter a queued
+  // tile; net_minkey the smallest key queued for a net.  A tile is processed
+  // only while its key is within `delta` of its net's smallest pending key.
+  uint32_t* tile_key;    // 2 * tile_count
+  uint32_t* net_minkey;  // 2 * net_count
+  uint32_t tile_count;
+  uint32_t delta;
+  uint32_t min_edge;     // smallest edge weight: bounds path lengths by distance / min_edge + 1
 };
 
 template <typename CostT>
@@ -111,12 +121,47 @@ __device__ __forceinline__ uint32_t LoadCost(const void* cost, uint64_t index) {
   return static_cast<const CostT*>(cost)[index];
 }
 
+// Queues `tile` for `round` with entry key `key` (a lower bound, minus the
+// entry cost, of any label the pending change can create in the tile).
 __device__ __forceinline__ void Push(const SearchArgs& a, uint32_t tile, uint32_t net,
-                                     uint32_t round) {
+                                     uint32_t round, uint32_t key) {
+  atomicMin(&a.tile_key[(round & 1) * a.tile_count + tile], key);
+  atomicMin(&a.net_minkey[(round & 1) * a.net_count + net], key);
   if (atomicMax(&a.tile_stamp[tile], round) < round) {
     const uint32_t slot = atomicAdd(&a.counters[2 * round], 1u);
     a.wl[round & 1][slot] = tile;
     atomicMax(&a.state[net].last_push, round);
+  }
+}
+
+// Admission of a popped item: items of finished nets are dropped, items whose
+// key is beyond the net's bucket (smallest pending key + delta) move to the
+// next round unprocessed.  The round's key slot is cleared for reuse two
+// rounds later.
+__device__ __forceinline__ bool AdmitItem(const SearchArgs& a, uint32_t g) {
+  const uint32_t r = a.round;
+  uint32_t* slot = &a.tile_key[(r & 1) * a.tile_count + g];
+  const uint32_t key = *slot;
+  *slot = kInfD;
+  const uint32_t n = a.tile_net[g];
+  if (a.state[n].status != kSearching) return false;
+  const uint32_t base = a.net_minkey[(r & 1) * a.net_count + n];
+  const uint32_t limit = base >= kInfD - a.delta ? kInfD : base + a.delta;
+  if (key > limit) {
+    Push(a, g, n, r + 1, key);
+    return false;
+  }
+  return true;
+}
+
+// Pops the next admitted item of this round (thread 0 only); kNever when the
+// round's list is exhausted.
+__device__ __forceinline__ uint32_t PopAdmitted(const SearchArgs& a, uint32_t count) {
+  for (;;) {
+    const uint32_t i = atomicAdd(&a.counters[2 * a.round + 1], 1u);
+    if (i >= count) return kNever;
+    const uint32_t g = a.wl[a.round & 1][i];
+    if (AdmitItem(a, g)) return g;
   }
 }
 
@@ -282,7 +327,6 @@ __device__ void ProcessTile(const SearchArgs& a, uint32_t g, uint32_t* s_lab, ui
     }
   }
 
-  uint32_t flags = 0;
 #pragma unroll
   for (int k = 0; k < kKeep; ++k) {
     const int i = tid + k * kThreads;
@@ -292,23 +336,24 @@ __device__ void ProcessTile(const SearchArgs& a, uint32_t g, uint32_t* s_lab, ui
         const uint32_t v = s_lab[(l * (TY + 2) + y + 1) * PX + x + 1];
         if (v != orig[k] || !initd) lab[(l * h + by + y) * w + bx + x] = v;
         if ((v >> 3) != (orig[k] >> 3)) {
-          flags |= (x == 0 ? 1u : 0u) | (x == nx - 1 ? 2u : 0u) | (y == 0 ? 4u : 0u) |
-                   (y == ny - 1 ? 8u : 0u);
+          const uint32_t d = v >> 3;
+          if (x == 0) atomicMin(&s_flags[0], d);
+          if (x == nx - 1) atomicMin(&s_flags[1], d);
+          if (y == 0) atomicMin(&s_flags[2], d);
+          if (y == ny - 1) atomicMin(&s_flags[3], d);
         }
       }
     }
   }
-  if (flags) atomicOr(s_flags, flags);
   __syncthreads();
   if (tid == 0) {
-    flags = *s_flags;
     if (!initd) a.tile_init[g] = r;
     const uint32_t next = r + 1;
-    if ((flags & 1u) && tx > 0) Push(a, g - 1, n, next);
-    if ((flags & 2u) && tx + 1 < ns.tiles_x) Push(a, g + 1, n, next);
-    if ((flags & 4u) && ty > 0) Push(a, g - ns.tiles_x, n, next);
-    if ((flags & 8u) && ty + 1 < ns.tiles_y) Push(a, g + ns.tiles_x, n, next);
-    if (!converged) Push(a, g, n, next);
+    if (s_flags[0] < kInfD && tx > 0) Push(a, g - 1, n, next, s_flags[0]);
+    if (s_flags[1] < kInfD && tx + 1 < ns.tiles_x) Push(a, g + 1, n, next, s_flags[1]);
+    if (s_flags[2] < kInfD && ty > 0) Push(a, g - ns.tiles_x, n, next, s_flags[2]);
+    if (s_flags[3] < kInfD && ty + 1 < ns.tiles_y) Push(a, g + ns.tiles_x, n, next, s_flags[3]);
+    if (!converged) Push(a, g, n, next, 0);
     if (atomicMax(&a.state[n].last_counted, r) < r) atomicAdd(&a.state[n].rounds, 1u);
     atomicAdd(&a.stats[0], 1ull);
     atomicAdd(&a.stats[1], static_cast<unsigned long long>(nx) * ny * L);
@@ -324,19 +369,341 @@ __global__ __launch_bounds__(kThreads) void RoundKernel(SearchArgs a, uint32_t c
   uint32_t* s_lab = smem;
   uint32_t* s_cost = smem + static_cast<int>(a.L) * (TY + 2) * PX;
   __shared__ uint32_t s_item;
-  __shared__ uint32_t s_flags;
+  __shared__ uint32_t s_flags[4];
   const uint32_t count = a.counters[2 * a.round];
   for (;;) {
     if (threadIdx.x == 0) {
-      const uint32_t i = atomicAdd(&a.counters[2 * a.round + 1], 1u);
-      s_item = i < count ? a.wl[a.round & 1][i] : kNever;
-      s_flags = 0;
+      s_item = PopAdmitted(a, count);
+      for (int k = 0; k < 4; ++k) s_flags[k] = kInfD;
     }
     __syncthreads();
     const uint32_t g = s_item;
     if (g == kNever) break;
-    ProcessTile<TX, TY>(a, g, s_lab, s_cost, &s_flags, cost_bytes);
+    ProcessTile<TX, TY>(a, g, s_lab, s_cost, s_flags, cost_bytes);
     __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Register-resident tile kernel (64 x 64 x L tiles, one wave per layer).
+//
+// Wave l holds layer l of the tile column-major in VGPRs: lane x keeps the 64
+// packed labels of column x and the 64 pre-shifted entry costs (cost << 3,
+// all-ones for a blocked or out-of-window cell).  A relaxation step is a
+// saturating add, an and_or that swaps in the direction code, a min and a
+// compare, with no memory access in the dependency chain:
+//   north/south  serial over the lane's own 64 registers;
+//   east/west    after a wave-private LDS transpose (pitch 65: conflict free),
+//                serial over the transposed row, then transposed back;
+//   vias         every wave publishes its layer in LDS, then relaxes each cell
+//                from the layers below and above (one hop per sweep; sweeps
+//                repeat until the tile is stable, so chains of vias converge).
+// ---------------------------------------------------------------------------
+constexpr int kRT = 64;          // tile side
+constexpr int kRP = kRT + 1;     // LDS pitch of the transpose buffers
+constexpr uint32_t kBlocked8 = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint32_t Relax8(uint32_t carry, uint32_t c8, uint32_t code) {
+  return (__builtin_elementwise_add_sat(carry, c8) & ~7u) | code;
+}
+
+__device__ __forceinline__ void WaveSync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Packed cost pair helpers: each 32-bit register holds two u16 entries
+// e = (cost << 3) + 1 for a free cell and 0 for a blocked one, so e - 1 (mod
+// 2^32) is the pre-shifted cost or all ones.
+__device__ __forceinline__ uint32_t CostLo(uint32_t pair) { return (pair & 0xFFFFu) - 1u; }
+__device__ __forceinline__ uint32_t CostHi(uint32_t pair) { return (pair >> 16) - 1u; }
+template <int I>
+__device__ __forceinline__ uint32_t CostAt(const uint32_t* pairs) {
+  return (I & 1) ? CostHi(pairs[I >> 1]) : CostLo(pairs[I >> 1]);
+}
+
+template <int L>
+__global__ __launch_bounds__(64 * L) __attribute__((amdgpu_waves_per_eu(2)))
+void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
+  extern __shared__ uint32_t smem[];
+  uint32_t* s_t = smem;                       // L transpose/publish buffers, kRT * kRP each
+  __shared__ uint32_t s_item[2];  // current / prefetched item (parity of the iteration)
+  __shared__ uint32_t s_side[5];   // smallest changed border distance per side (W, E, S, N), tile min
+  __shared__ uint32_t s_changed[L];
+  const int l = threadIdx.x >> 6;
+  const uint32_t r = a.round;
+  const uint32_t count = a.counters[2 * r];
+  // The last wave pops the next admitted item during the current item's
+  // epilogue, so the pop's dependent atomics overlap the pushes.
+  const bool popper = threadIdx.x == 64 * (L - 1);
+  if (popper) s_item[0] = PopAdmitted(a, count);
+  for (uint32_t iter = 0;; ++iter) {
+    if (threadIdx.x == 0) {
+      for (int k = 0; k < 5; ++k) s_side[k] = kInfD;
+    }
+    __syncthreads();
+    const uint32_t g = s_item[iter & 1];
+    if (g == kNever) break;
+    // The lane index is opaque per tile: every LDS and global offset derived
+    // from it is recomputed per tile rather than hoisted out of this loop
+    // (64 hoisted offsets would stay live across the whole loop).
+    int lane = static_cast<int>(threadIdx.x & 63);
+    asm volatile("" : "+v"(lane));
+    uint32_t* T = s_t + l * kRT * kRP;
+    const uint32_t n = a.tile_net[g];
+    const NetStatic ns = a.nets[n];
+    const uint32_t t = g - ns.tile_offset;
+    const uint32_t tx = t % ns.tiles_x, ty = t / ns.tiles_x;
+    const uint32_t bx = tx * kRT, by = ty * kRT;
+    const int nx = static_cast<int>(min(static_cast<uint32_t>(kRT), ns.w - bx));
+    const int ny = static_cast<int>(min(static_cast<uint32_t>(kRT), ns.h - by));
+    const uint32_t w = ns.w;
+    // Uniform bases; per-cell offsets fit in 32 bits (cells < 2^32).
+    uint32_t* lbase = a.labels + ns.label_offset + (static_cast<uint64_t>(l) * ns.h + by) * w + bx;
+    const uint64_t cbase_index = (static_cast<uint64_t>(l) * a.H + ns.y0 + by) * a.W + ns.x0 + bx;
+    const uint16_t* cbase16 = static_cast<const uint16_t*>(a.cost) + cbase_index;
+    const uint32_t* cbase32 = static_cast<const uint32_t*>(a.cost) + cbase_index;
+    const bool initd = a.tile_init[g] != kNever;
+    auto ready = [&](uint32_t ntx, uint32_t nty) {
+      const uint32_t v = a.tile_init[ns.tile_offset + nty * ns.tiles_x + ntx];
+      return v != kNever && v < r;
+    };
+    const bool has_w = tx > 0 && ready(tx - 1, ty);
+    const bool has_e = tx + 1 < ns.tiles_x && ready(tx + 1, ty);
+    const bool has_s = ty > 0 && ready(tx, ty - 1);
+    const bool has_n = ty + 1 < ns.tiles_y && ready(tx, ty + 1);
+
+    const bool xin = lane < nx;
+    uint32_t lab[kRT], cst[kRT / 2], cstr[kRT / 2];
+    if (nx == kRT && ny == kRT) {
+      // Full tile: unrolled coalesced row loads straight into registers.
+#pragma unroll
+      for (int y = 0; y < kRT; ++y) {
+        const uint32_t c = cost_bytes == 2 ? cbase16[y * a.W + lane] : cbase32[y * a.W + lane];
+        const uint32_t e = c ? (c << 3) + 1u : 0u;
+        if (y & 1) cst[y >> 1] |= e << 16; else cst[y >> 1] = e;
+      }
+      if (initd) {
+#pragma unroll
+        for (int y = 0; y < kRT; ++y) lab[y] = lbase[y * w + lane];
+      } else {
+#pragma unroll
+        for (int y = 0; y < kRT; ++y) lab[y] = kInf;
+      }
+#pragma unroll
+      for (int i = 0; i < kRT / 2; ++i) {
+        T[(2 * i) * kRP + lane] = cst[i] & 0xFFFFu;
+        T[(2 * i + 1) * kRP + lane] = cst[i] >> 16;
+      }
+      WaveSync();
+#pragma unroll
+      for (int i = 0; i < kRT / 2; ++i) cstr[i] = T[lane * kRP + 2 * i] | (T[lane * kRP + 2 * i + 1] << 16);
+      WaveSync();
+#pragma unroll
+      for (int y = 0; y < kRT; ++y) T[y * kRP + lane] = lab[y];
+      WaveSync();
+    } else {
+      // Window-edge tile: predicated runtime loops stage rows through the
+      // wave's LDS buffer (out-of-window cells masked to blocked / unreached),
+      // then unrolled conflict-free reads fill registers.  No per-row
+      // predicate lives in the unrolled register code.
+      const uint32_t xc = min(static_cast<uint32_t>(lane), static_cast<uint32_t>(nx - 1));
+#pragma unroll 8
+      for (int y = 0; y < kRT; ++y) {
+        const uint32_t row = static_cast<uint32_t>(min(y, ny - 1));
+        const uint32_t coff = row * a.W + xc;
+        uint32_t c = cost_bytes == 2 ? cbase16[coff] : cbase32[coff];
+        c = (xin && y < ny) ? c : 0u;
+        T[y * kRP + lane] = c ? (c << 3) + 1u : 0u;
+      }
+      WaveSync();
+#pragma unroll
+      for (int i = 0; i < kRT / 2; ++i) {
+        cst[i] = T[(2 * i) * kRP + lane] | (T[(2 * i + 1) * kRP + lane] << 16);
+        cstr[i] = T[lane * kRP + 2 * i] | (T[lane * kRP + 2 * i + 1] << 16);
+      }
+      WaveSync();
+#pragma unroll 8
+      for (int y = 0; y < kRT; ++y) {
+        const uint32_t row = static_cast<uint32_t>(min(y, ny - 1));
+        uint32_t v = initd ? lbase[row * w + xc] : kInf;
+        T[y * kRP + lane] = (xin && y < ny) ? v : kInf;
+      }
+      WaveSync();
+#pragma unroll
+      for (int y = 0; y < kRT; ++y) lab[y] = T[y * kRP + lane];
+    }
+    // Borders before the sweeps (distances compared after).
+    const uint32_t orig_s = T[lane];
+    const uint32_t orig_n = T[(ny - 1) * kRP + lane];
+    const uint32_t orig_w = T[lane * kRP];
+    const uint32_t orig_e = T[lane * kRP + nx - 1];
+    const uint32_t halo_s = (has_s && xin) ? lbase[static_cast<int64_t>(lane) - static_cast<int64_t>(w)] : kInf;
+    const uint32_t halo_n = (has_n && xin) ? lbase[lane + kRT * w] : kInf;
+    const uint32_t halo_w = (has_w && lane < ny) ? lbase[static_cast<int64_t>(lane) * w - 1] : kInf;
+    const uint32_t halo_e = (has_e && lane < ny) ? lbase[lane * w + kRT] : kInf;
+    WaveSync();
+    const uint32_t via8 = a.via << 3;
+
+    uint32_t sweeps = 0;
+    bool converged = false;
+#pragma unroll 1
+    while (sweeps < a.max_sweeps) {
+      // Keep the packed costs packed: without this the compiler unpacks all
+      // 128 entries once outside the loop and runs out of registers.
+#pragma unroll
+      for (int i = 0; i < kRT / 2; ++i) asm volatile("" : "+v"(cst[i]), "+v"(cstr[i]));
+      uint32_t chv = 0;  // OR of (old ^ new) over every relaxation of this sweep
+      uint32_t carry = halo_s;
+#pragma unroll
+      for (int y = 0; y < kRT; ++y) {
+        const uint32_t c8 = (y & 1) ? CostHi(cst[y >> 1]) : CostLo(cst[y >> 1]);
+        const uint32_t nv = min(lab[y], Relax8(carry, c8, kFromSouth));
+        chv |= nv ^ lab[y];
+        lab[y] = nv;
+        carry = nv;
+      }
+      carry = halo_n;
+#pragma unroll
+      for (int y = kRT - 1; y >= 0; --y) {
+        const uint32_t c8 = (y & 1) ? CostHi(cst[y >> 1]) : CostLo(cst[y >> 1]);
+        const uint32_t nv = min(lab[y], Relax8(carry, c8, kFromNorth));
+        chv |= nv ^ lab[y];
+        lab[y] = nv;
+        carry = nv;
+      }
+#pragma unroll
+      for (int y = 0; y < kRT; ++y) T[y * kRP + lane] = lab[y];
+      WaveSync();
+#pragma unroll
+      for (int x = 0; x < kRT; ++x) lab[x] = T[lane * kRP + x];
+      carry = halo_w;
+#pragma unroll
+      for (int x = 0; x < kRT; ++x) {
+        const uint32_t c8 = (x & 1) ? CostHi(cstr[x >> 1]) : CostLo(cstr[x >> 1]);
+        const uint32_t nv = min(lab[x], Relax8(carry, c8, kFromWest));
+        chv |= nv ^ lab[x];
+        lab[x] = nv;
+        carry = nv;
+      }
+      carry = halo_e;
+#pragma unroll
+      for (int x = kRT - 1; x >= 0; --x) {
+        const uint32_t c8 = (x & 1) ? CostHi(cstr[x >> 1]) : CostLo(cstr[x >> 1]);
+        const uint32_t nv = min(lab[x], Relax8(carry, c8, kFromEast));
+        chv |= nv ^ lab[x];
+        lab[x] = nv;
+        carry = nv;
+      }
+      WaveSync();
+#pragma unroll
+      for (int x = 0; x < kRT; ++x) T[lane * kRP + x] = lab[x];
+      WaveSync();
+#pragma unroll
+      for (int y = 0; y < kRT; ++y) lab[y] = T[y * kRP + lane];
+      bool any;
+      {
+        if (a.via != 0) {  // the host passes via = 0 for single-layer lattices
+          __syncthreads();  // every layer published in its buffer
+          const uint32_t* below = s_t + (l > 0 ? l - 1 : 0) * kRT * kRP;
+          const uint32_t* above = s_t + (l + 1 < L ? l + 1 : l) * kRT * kRP;
+          const bool has_below = l > 0, has_above = l + 1 < L;
+#pragma unroll
+          for (int y = 0; y < kRT; ++y) {
+            const uint32_t c8 = (y & 1) ? CostHi(cst[y >> 1]) : CostLo(cst[y >> 1]);
+            const uint32_t vc = c8 == kBlocked8 ? kBlocked8 : via8;
+            const uint32_t fb = has_below ? Relax8(below[y * kRP + lane], vc, kFromBelow) : kInf;
+            const uint32_t fa = has_above ? Relax8(above[y * kRP + lane], vc, kFromAbove) : kInf;
+            const uint32_t nv = min(lab[y], min(fb, fa));
+            chv |= nv ^ lab[y];
+            lab[y] = nv;
+          }
+        }
+        const bool wave_any = __any(chv != 0);
+        if (lane == 0) s_changed[l] = wave_any ? 1u : 0u;
+        __syncthreads();
+        any = false;
+#pragma unroll
+        for (int k = 0; k < L; ++k) any |= s_changed[k] != 0;
+      }
+      ++sweeps;
+      if (!any) {
+        converged = true;
+        break;
+      }
+      __syncthreads();  // flags and buffers are reused next sweep
+    }
+
+    // Write back through the tile buffer; detect border distance changes.
+    if constexpr (L > 1) __syncthreads();  // other layers may still read this buffer
+#pragma unroll
+    for (int y = 0; y < kRT; ++y) T[y * kRP + lane] = lab[y];
+    WaveSync();
+    uint32_t col_min = kInf;
+#pragma unroll
+    for (int y = 0; y < kRT; ++y) col_min = min(col_min, lab[y]);
+    if (!converged) atomicMin(&s_side[4], col_min >> 3);
+    {
+      const uint32_t ds = T[lane] >> 3, dn = T[(ny - 1) * kRP + lane] >> 3;
+      const uint32_t dw = T[lane * kRP] >> 3, de = T[lane * kRP + nx - 1] >> 3;
+      if (xin && ds != (orig_s >> 3)) atomicMin(&s_side[2], ds);
+      if (xin && dn != (orig_n >> 3)) atomicMin(&s_side[3], dn);
+      if (lane < ny && dw != (orig_w >> 3)) atomicMin(&s_side[0], dw);
+      if (lane < ny && de != (orig_e >> 3)) atomicMin(&s_side[1], de);
+    }
+    if (xin) {
+#pragma unroll 8
+      for (int y = 0; y < ny; ++y) lbase[y * w + lane] = T[y * kRP + lane];
+    }
+    __syncthreads();
+    // Epilogue: lanes 0..4 of the first wave issue the four neighbour pushes
+    // and the self push together; lane 5 counts the net's round, lane 6 the
+    // statistics; the popper fetches the next item meanwhile.
+    if (threadIdx.x < 7) {
+      const uint32_t next = r + 1;
+      const int k = threadIdx.x;
+      if (k == 0 && !initd) a.tile_init[g] = r;
+      if (k < 5) {
+        const uint32_t key = s_side[k];
+        bool exists;
+        uint32_t target;
+        switch (k) {
+          case 0: exists = tx > 0; target = g - 1; break;
+          case 1: exists = tx + 1 < ns.tiles_x; target = g + 1; break;
+          case 2: exists = ty > 0; target = g - ns.tiles_x; break;
+          case 3: exists = ty + 1 < ns.tiles_y; target = g + ns.tiles_x; break;
+          default: exists = !converged; target = g; break;
+        }
+        if (key < kInfD && exists) Push(a, target, n, next, key);
+      } else if (k == 5) {
+        if (atomicMax(&a.state[n].last_counted, r) < r) atomicAdd(&a.state[n].rounds, 1u);
+      } else {
+        atomicAdd(&a.stats[0], 1ull);
+        atomicAdd(&a.stats[1], static_cast<unsigned long long>(nx) * ny * L);
+        atomicAdd(&a.stats[2], static_cast<unsigned long long>(sweeps));
+        atomicAdd(&a.stats[3], static_cast<unsigned long long>(sweeps) * nx * ny * L);
+      }
+    }
+    if (popper) s_item[(iter + 1) & 1] = PopAdmitted(a, count);
+  }
+}
+
+size_t RegTileLdsBytes(uint32_t layers) {
+  return static_cast<size_t>(layers) * (kRT * kRP + 2 * kRT) * 4;
+}
+
+const void* RegKernelFor(uint32_t layers) {
+  switch (layers) {
+    case 1: return reinterpret_cast<const void*>(&RegRoundKernel<1>);
+    case 2: return reinterpret_cast<const void*>(&RegRoundKernel<2>);
+    case 3: return reinterpret_cast<const void*>(&RegRoundKernel<3>);
+    case 4: return reinterpret_cast<const void*>(&RegRoundKernel<4>);
+    case 5: return reinterpret_cast<const void*>(&RegRoundKernel<5>);
+    case 6: return reinterpret_cast<const void*>(&RegRoundKernel<6>);
+    case 7: return reinterpret_cast<const void*>(&RegRoundKernel<7>);
+    case 8: return reinterpret_cast<const void*>(&RegRoundKernel<8>);
+    default: return nullptr;
   }
 }
 
@@ -378,7 +745,7 @@ __global__ __launch_bounds__(kThreads) void SeedKernel(SearchArgs a) {
   if (threadIdx.x == 0) {
     a.tile_init[g] = 0;
     a.state[n].connected = 1u;
-    Push(a, g, n, 1u);
+    Push(a, g, n, 1u, 0u);
   }
 }
 
@@ -394,6 +761,10 @@ __device__ __forceinline__ uint32_t Neighbour(uint32_t cell, uint32_t code, uint
   }
 }
 
+template <int TX, int TY>
+__device__ void WalkConnection(const SearchArgs& a, const NetStatic& ns, NetState& st,
+                               DevConnection& c, uint32_t start, bool reseed, uint32_t n);
+
 // Runs after every round.  Nets without queued work hold exact distances from
 // their current tree: connect the nearest unconnected pin (ties: lowest pin
 // index) along the predecessor chain, make that path part of the tree and
@@ -402,8 +773,12 @@ template <int TX, int TY>
 __global__ void PhaseKernel(SearchArgs a) {
   const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= a.net_count) return;
+  // The key buffer read by the round just executed collects pushes of the
+  // round after the next one.
+  a.net_minkey[((a.round - 1) & 1) * a.net_count + n] = kInfD;
   NetState& st = a.state[n];
-  if (st.status != kSearching || st.last_push >= a.round) return;
+  if (st.status != kSearching) return;
+  const uint32_t pending = a.net_minkey[(a.round & 1) * a.net_count + n];
   const NetStatic ns = a.nets[n];
   uint32_t* lab = a.labels + ns.label_offset;
   const uint32_t plane = a.W * a.H;
@@ -418,65 +793,98 @@ __global__ void PhaseKernel(SearchArgs a) {
       best_pin = p;
     }
   }
+  // Pending work can only create labels above its keys, so the nearest pin is
+  // settled (its distance and every label on its predecessor chain exact) once
+  // its distance is at most the smallest pending key.
   if (best_pin == kNever) {
-    st.status = APGX_NET_DISCONNECTED;
+    if (pending >= kInfD) st.status = APGX_NET_DISCONNECTED;
     return;
   }
-  const uint32_t start = a.pins[ns.pin_offset + best_pin];
-  // Pass 1: length of the chain back to a tree cell (distance 0).  Distances
-  // strictly decrease along the chain; the step bound and window check only
-  // guard against a corrupted word.
-  const uint64_t max_len = static_cast<uint64_t>(ns.w) * ns.h * a.L;
-  uint32_t len = 1;
-  for (uint32_t v = start;;) {
-    const uint32_t word = lab[LocalIndex(ns, a.W, a.H, v)];
-    if ((word >> 3) == 0) break;
-    const uint32_t code = word & 7u;
-    const uint32_t x = v % a.W, y = (v / a.W) % a.H, l = v / plane;
-    const bool ok = code <= kFromAbove && len < max_len &&
-                    (code != kFromBelow || l > 0) && (code != kFromAbove || l + 1 < a.L) &&
-                    (code != kFromWest || x > ns.x0) && (code != kFromEast || x + 1 < ns.x0 + ns.w) &&
-                    (code != kFromSouth || y > ns.y0) && (code != kFromNorth || y + 1 < ns.y0 + ns.h);
-    if (!ok) {
-      st.status = APGX_NET_INVALID;
-      return;
-    }
-    v = Neighbour(v, code, a.W, plane);
-    ++len;
-  }
-  const unsigned long long offset = atomicAdd(a.cell_cursor, static_cast<unsigned long long>(len));
-  if (offset + len > a.cell_capacity) {
-    st.status = APGX_NET_INVALID;
-    return;
-  }
-  // Pass 2: record the path and turn it into sources for the next step.
+  if (best > pending) return;
   const bool last = st.n_connected + 2 == ns.pin_count;
-  uint32_t pushed = kNever;
-  uint32_t v = start;
-  for (uint32_t i = 0; i < len; ++i) {
-    const uint64_t li = LocalIndex(ns, a.W, a.H, v);
-    const uint32_t word = lab[li];
-    a.cells[offset + i] = v;
-    if (!last && (word >> 3) != 0) {
-      lab[li] = kSourceWord;
-      const uint32_t g = TileOf<TX, TY>(ns, a.W, a.H, v);
-      if (g != pushed) {
-        Push(a, g, n, a.round);
-        pushed = g;
-      }
-    }
-    v = Neighbour(v, word & 7u, a.W, plane);
-  }
   DevConnection& c = a.conns[ns.conn_offset + st.n_connected];
   c.pin_index = best_pin;
   c.distance = best;
-  c.cell_offset = offset;
-  c.cell_count = len;
+  c.cell_offset = 0;
+  c.cell_count = 0;
   c.reserved = 0;
   st.cost += best;
   st.connected |= 1u << best_pin;
   st.n_connected += 1;
-  if (last) st.status = APGX_NET_REACHED;
+  if (last) {
+    // Nothing depends on the last path: it is written after the round loop so
+    // this (per-round) kernel stays short.
+    st.status = kFinalWalk;
+    return;
+  }
+  WalkConnection<TX, TY>(a, ns, st, c, a.pins[ns.pin_offset + best_pin], true, n);
+}
+
+// Writes the predecessor chain of a settled pin back to the tree (distance 0)
+// in one pass into a reservation of min(distance / min_edge + 1, window cells)
+// cells, which bounds the chain since every step costs at least min_edge; when
+// `reseed`, the path cells become sources of the next connection step and
+// their tiles are queued.  The step bound and window checks guard against a
+// corrupted word only.
+template <int TX, int TY>
+__device__ void WalkConnection(const SearchArgs& a, const NetStatic& ns, NetState& st,
+                               DevConnection& c, uint32_t start, bool reseed, uint32_t n) {
+  uint32_t* lab = a.labels + ns.label_offset;
+  const uint32_t plane = a.W * a.H;
+  const uint64_t bound = min(static_cast<uint64_t>(c.distance) / max(a.min_edge, 1u) + 1,
+                             static_cast<uint64_t>(ns.w) * ns.h * a.L);
+  const unsigned long long offset = atomicAdd(a.cell_cursor, static_cast<unsigned long long>(bound));
+  if (offset + bound > a.cell_capacity) {
+    st.status = APGX_NET_INVALID;
+    atomicOr(&a.stats[7], 1ull);  // reported to the caller as resource exhaustion
+    return;
+  }
+  uint32_t pushed = kNever;
+  uint32_t len = 0;
+  for (uint32_t v = start;;) {
+    const uint64_t li = LocalIndex(ns, a.W, a.H, v);
+    const uint32_t word = lab[li];
+    if (len >= bound) {
+      st.status = APGX_NET_INVALID;
+      return;
+    }
+    a.cells[offset + len++] = v;
+    if ((word >> 3) == 0) break;
+    const uint32_t code = word & 7u;
+    const uint32_t x = v % a.W, y = (v / a.W) % a.H, l = v / plane;
+    const bool ok = code <= kFromAbove && (code != kFromBelow || l > 0) &&
+                    (code != kFromAbove || l + 1 < a.L) && (code != kFromWest || x > ns.x0) &&
+                    (code != kFromEast || x + 1 < ns.x0 + ns.w) && (code != kFromSouth || y > ns.y0) &&
+                    (code != kFromNorth || y + 1 < ns.y0 + ns.h);
+    if (!ok) {
+      st.status = APGX_NET_INVALID;
+      return;
+    }
+    if (reseed) {
+      lab[li] = kSourceWord;
+      const uint32_t g = TileOf<TX, TY>(ns, a.W, a.H, v);
+      if (g != pushed) {
+        Push(a, g, n, a.round, 0u);
+        pushed = g;
+      }
+    }
+    v = Neighbour(v, code, a.W, plane);
+  }
+  c.cell_offset = offset;
+  c.cell_count = len;
+}
+
+// After the round loop: paths of nets whose last pin settled.  One thread per net.
+template <int TX, int TY>
+__global__ void FinalWalkKernel(SearchArgs a) {
+  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= a.net_count) return;
+  NetState& st = a.state[n];
+  if (st.status != kFinalWalk) return;
+  const NetStatic ns = a.nets[n];
+  DevConnection& c = a.conns[ns.conn_offset + st.n_connected - 1];
+  st.status = APGX_NET_REACHED;
+  WalkConnection<TX, TY>(a, ns, st, c, a.pins[ns.pin_offset + c.pin_index], false, n);
 }
 
 struct TileShape {
@@ -491,13 +899,15 @@ struct Kernels {
   const void* round;
   const void* seed;
   const void* phase;
+  const void* final_walk;
 };
 
 template <int TX, int TY>
 Kernels KernelsFor() {
   return Kernels{reinterpret_cast<const void*>(&RoundKernel<TX, TY>),
                  reinterpret_cast<const void*>(&SeedKernel<TX, TY>),
-                 reinterpret_cast<const void*>(&PhaseKernel<TX, TY>)};
+                 reinterpret_cast<const void*>(&PhaseKernel<TX, TY>),
+                 reinterpret_cast<const void*>(&FinalWalkKernel<TX, TY>)};
 }
 
 bool KernelsForShape(TileShape s, Kernels* k) {
@@ -521,6 +931,7 @@ struct apgx_lattice {
   int device = 0;
   uint32_t min_edge = 0;      // smallest edge weight (free cell cost or via)
   uint64_t free_cells = 0;
+  double mean_cost = 1.0;     // mean free-cell cost
   std::mutex cache_mutex;
   void* cached_labels = nullptr;  // reusable label pool (returned by searches)
   size_t cached_label_bytes = 0;
@@ -576,6 +987,7 @@ int FinishLattice(const apgx_lattice_desc* desc, const void* host_costs, apgx_la
   const uint64_t cells = static_cast<uint64_t>(desc->width) * desc->height * desc->layers;
   uint32_t min_cost = 0xFFFFFFFFu, max_cost = 0;
   uint64_t free_cells = 0;
+  double cost_sum = 0.0;
   for (uint64_t c = 0; c < cells; ++c) {
     const uint32_t v = desc->cost_bytes == 2 ? static_cast<const uint16_t*>(host_costs)[c]
                                              : static_cast<const uint32_t*>(host_costs)[c];
@@ -583,11 +995,13 @@ int FinishLattice(const apgx_lattice_desc* desc, const void* host_costs, apgx_la
     if (v > APGX_MAX_CELL_COST)
       return Fail(APGX_E_INTERNAL_INVARIANT, "cell cost exceeds APGX_MAX_CELL_COST");
     ++free_cells;
+    cost_sum += v;
     min_cost = std::min(min_cost, v);
     max_cost = std::max(max_cost, v);
   }
   lat->desc = *desc;
   lat->free_cells = free_cells;
+  lat->mean_cost = free_cells ? cost_sum / static_cast<double>(free_cells) : 1.0;
   const uint8_t* bytes = static_cast<const uint8_t*>(host_costs);
   lat->host_costs.assign(bytes, bytes + cells * desc->cost_bytes);
   lat->min_edge = min_cost;
@@ -652,6 +1066,8 @@ namespace apgx {
 namespace {
 
 TileShape DefaultShape(const apgx_lattice_desc& d, size_t lds_limit) {
+  // Register-resident 64 x 64 tiles (one wave per layer) up to eight layers.
+  if (d.layers <= 8 && RegTileLdsBytes(d.layers) <= lds_limit) return TileShape{64, 64};
   const TileShape candidates[] = {{64, 64}, {64, 32}, {32, 32}, {32, 16}};
   // Prefer the largest tile that fits the register-resident cell budget and
   // leaves room for two workgroups per CU.
@@ -696,9 +1112,16 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
   Kernels k{};
   if (!KernelsForShape(shape, &k))
     return Fail(APGX_E_UNSUPPORTED, "apgx_route_nets: unsupported tile shape");
-  if (static_cast<uint64_t>(shape.tx) * shape.ty * d.layers > kMaxTileCells)
+  const bool reg_tile = shape.tx == 64 && shape.ty == 64 && d.layers <= 8;
+  uint32_t block = kThreads;
+  size_t lds = TileLdsBytes(shape, d.layers);
+  if (reg_tile) {
+    k.round = RegKernelFor(d.layers);
+    block = 64 * d.layers;
+    lds = RegTileLdsBytes(d.layers);
+  } else if (static_cast<uint64_t>(shape.tx) * shape.ty * d.layers > kMaxTileCells) {
     return Fail(APGX_E_UNSUPPORTED, "apgx_route_nets: tile holds more cells than a workgroup keeps");
-  const size_t lds = TileLdsBytes(shape, d.layers);
+  }
   if (lds > lds_limit)
     return Fail(APGX_E_UNSUPPORTED, "apgx_route_nets: tile does not fit in LDS");
 
@@ -751,7 +1174,7 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
     tile_net.insert(tile_net.end(), nt, i);
     tiles += nt;
     label_words += static_cast<uint64_t>(n.w) * n.h * d.layers;
-    window_cells_total += static_cast<uint64_t>(n.w) * n.h * d.layers;
+    window_cells_total += static_cast<uint64_t>(n.w) * n.h * d.layers * (q.pin_count - 1);
     conns += q.pin_count - 1;
   }
   if (tiles >= 0xFFFFFFF0ull) {
@@ -779,8 +1202,11 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
   Part p_conns{sizeof(DevConnection) * std::max<uint64_t>(conns, 1), 0};
   Part p_cursor{8, 0};
   Part p_cells{4 * cell_capacity, 0};
+  Part p_tile_key{8 * std::max<uint64_t>(tiles, 1), 0};
+  Part p_net_key{8 * std::max<uint32_t>(net_count, 1), 0};
   Part* parts[] = {&p_nets, &p_state, &p_tile_net, &p_stamp, &p_init, &p_wl0, &p_wl1,
-                   &p_counters, &p_stats, &p_pins, &p_conns, &p_cursor, &p_cells};
+                   &p_counters, &p_stats, &p_pins, &p_conns, &p_cursor, &p_cells,
+                   &p_tile_key, &p_net_key};
   size_t total = 0;
   for (Part* p : parts) { p->offset = total; total += align(p->bytes); }
   const size_t label_bytes = static_cast<size_t>(std::max<uint64_t>(label_words, 1)) * 4;
@@ -824,7 +1250,20 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
   a.cells = reinterpret_cast<uint32_t*>(at(p_cells));
   a.cell_cursor = reinterpret_cast<unsigned long long*>(at(p_cursor));
   a.cell_capacity = cell_capacity;
-  a.max_sweeps = 64;
+  a.max_sweeps = opt.max_sweeps ? opt.max_sweeps : 64;
+  a.tile_key = reinterpret_cast<uint32_t*>(at(p_tile_key));
+  a.net_minkey = reinterpret_cast<uint32_t*>(at(p_net_key));
+  a.tile_count = static_cast<uint32_t>(tiles);
+  a.min_edge = std::max(lat->min_edge, 1u);
+  // Bucket width: about one tile width of distance by default; all-ones
+  // disables ordering (every queued tile is processed every round).
+  if (opt.frontier_delta == 0xFFFFFFFFu) {
+    a.delta = kInfD;
+  } else if (opt.frontier_delta != 0) {
+    a.delta = std::min(opt.frontier_delta, kInfD);
+  } else {
+    a.delta = static_cast<uint32_t>(std::max(1.0, 64.0 * lat->mean_cost));
+  }
   s->d_state = a.state;
   s->d_conns = a.conns;
   s->d_cells = a.cells;
@@ -851,6 +1290,8 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
   check(hipMemsetAsync(a.counters, 0, 4 * counter_words, stream), "counter clear");
   check(hipMemsetAsync(a.stats, 0, 64, stream), "stats clear");
   check(hipMemsetAsync(a.cell_cursor, 0, 8, stream), "cursor clear");
+  check(hipMemsetAsync(a.tile_key, 0xFF, 8 * std::max<uint64_t>(tiles, 1), stream), "key clear");
+  check(hipMemsetAsync(a.net_minkey, 0xFF, 8 * std::max<uint32_t>(net_count, 1), stream), "key clear");
   if (status == APGX_OK && lds > 65536)
     check(hipFuncSetAttribute(k.round, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)), "hipFuncSetAttribute");
 
@@ -867,7 +1308,7 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
 
   int occupancy = 1;
   if (status == APGX_OK)
-    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occupancy, k.round, kThreads, lds), "occupancy query");
+    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occupancy, k.round, block, lds), "occupancy query");
   occupancy = std::max(occupancy, 1);
   const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(
       static_cast<uint64_t>(prop.multiProcessorCount) * occupancy, std::max<uint64_t>(tiles, 1))));
@@ -896,7 +1337,7 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
           round_events.push_back(e1);
           check(hipEventRecord(e0, stream), "hipEventRecord");
         }
-        status = Launch(k.round, dim3(grid), dim3(kThreads), lds, stream, rargs, "round launch");
+        status = Launch(k.round, dim3(grid), dim3(block), lds, stream, rargs, "round launch");
         if (status != APGX_OK) break;
         if (opt.profile_rounds) check(hipEventRecord(round_events.back(), stream), "hipEventRecord");
         SearchArgs pa = a;
@@ -910,6 +1351,10 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
       ++syncs;
       if (next_count == 0) done = true;
       else if (r > max_rounds) done = true;  // budget: remaining searching nets stop
+    }
+    if (status == APGX_OK) {
+      void* fargs[] = {&a};
+      status = Launch(k.final_walk, dim3((net_count + 63) / 64), dim3(64), 0, stream, fargs, "final walk launch");
     }
     check(hipEventRecord(ev1, stream), "hipEventRecord");
     check(hipStreamSynchronize(stream), "hipStreamSynchronize");
@@ -943,12 +1388,14 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
   if (ev0) (void)hipEventDestroy(ev0);
   if (ev1) (void)hipEventDestroy(ev1);
   if (stream) (void)hipStreamDestroy(stream);
+  if (status == APGX_OK && stats[7] != 0)
+    status = Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_route_nets: connection cell buffer exhausted");
   if (status != APGX_OK) {
     apgx_search_release(s);
     return status;
   }
   for (auto& x : s->state)
-    if (x.status == kSearching) x.status = APGX_NET_BUDGET_EXHAUSTED;
+    if (x.status == kSearching || x.status == kFinalWalk) x.status = APGX_NET_BUDGET_EXHAUSTED;
   apgx_route_stats& rs = s->stats;
   rs.device_ms = ms;
   rs.tile_visits = stats[0];

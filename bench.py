@@ -42,6 +42,8 @@ def parse_args():
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--tile", type=str, default="0x0")
+    p.add_argument("--delta", type=int, default=0, help="tile scheduling bucket width (0: default)")
+    p.add_argument("--max-sweeps", type=int, default=0)
     return p.parse_args()
 
 
@@ -90,7 +92,8 @@ def main():
     print(f"[bench] lattice {cost.shape} uploaded; {args.queries} queries", file=sys.stderr, flush=True)
 
     def step(profile=False):
-        routes, stats = lat.route_pairs(pairs, tile=tile, profile_rounds=1 if profile else 0)
+        routes, stats = lat.route_pairs(pairs, tile=tile, profile_rounds=1 if profile else 0,
+                                        frontier_delta=args.delta, max_sweeps=args.max_sweeps)
         return routes, stats
 
     for i in range(args.warmup):
@@ -145,6 +148,7 @@ def main():
         "reached": reached,
         "rounds": st.rounds,
         "tile_visits": st.tile_visits,
+        "inner_sweeps_per_visit": st.inner_sweeps / max(st.tile_visits, 1),
         "relaxations_per_s": st.relaxations / (ms / 1e3),
         "roofline": {
             "bound": "hbm",

@@ -167,7 +167,9 @@ typedef struct apgx_route_options {
   uint64_t max_device_bytes;  /* 0: unlimited (workspace budget) */
   uint32_t use_graph;         /* 1: launch round chunks as a captured hipGraph */
   uint32_t profile_rounds;    /* 1: time every round kernel with hipEvents (round_kernel_ms) */
-  uint32_t reserved[2];
+  uint32_t frontier_delta;    /* distance bucket width of tile scheduling; 0: 64 x mean cell
+                                 cost; 0xFFFFFFFF: unordered (every queued tile every round) */
+  uint32_t max_sweeps;        /* LDS/register sweeps per tile visit before re-queueing; 0: 64 */
 } apgx_route_options;
 
 typedef struct apgx_route_stats {

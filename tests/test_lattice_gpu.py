@@ -54,8 +54,9 @@ def test_random_small_lattices(apgx, oracle, gpu, seed):
     pairs = rng.choice(free, size=(12, 2), replace=True)
     pairs = pairs[pairs[:, 0] != pairs[:, 1]]
     tile = [(0, 0), (32, 16), (32, 32), (64, 32)][seed % 4]
+    delta = [0, 0xFFFFFFFF, 1, 50][(seed // 2) % 4]
     lat = apgx.Lattice(cost, via_cost=via)
-    routes, _ = lat.route_pairs(pairs, tile=tile)
+    routes, _ = lat.route_pairs(pairs, tile=tile, frontier_delta=delta)
     o = oracle.Dense(cost, via)
     for (s, t), r in zip(pairs, routes):
         dist = o.dijkstra([int(s)])
@@ -86,7 +87,8 @@ def test_windows_and_multi_pin_nets(apgx, oracle, gpu, seed):
         windows.append((x0, y0, w, h))
     nets, pins = apgx.nets_from_lists(pin_lists, windows)
     lat = apgx.Lattice(cost, via_cost=via)
-    routes, _ = lat.route(nets, pins, tile=[(0, 0), (32, 16), (32, 32), (64, 32)][seed])
+    routes, _ = lat.route(nets, pins, tile=[(0, 0), (32, 16), (32, 32), (64, 32)][seed],
+                          frontier_delta=[0, 0xFFFFFFFF, 7, 0][seed])
     for pl, win, r in zip(pin_lists, windows, routes):
         rc, conns = oracle.Dense(cost, via, win).route_net(pl)
         if rc == 1:
