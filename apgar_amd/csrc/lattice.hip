@@ -114,6 +114,7 @@ struct SearchArgs {
   uint32_t tile_count;
   uint32_t delta;
   uint32_t min_edge;     // smallest edge weight: bounds path lengths by distance / min_edge + 1
+  uint32_t goal_cost;    // smallest in-layer step cost for the goal bound; 0 turns it off
 };
 
 template <typename CostT>
@@ -132,6 +133,40 @@ __device__ __forceinline__ void Push(const SearchArgs& a, uint32_t tile, uint32_
     a.wl[round & 1][slot] = tile;
     atomicMax(&a.state[net].last_push, round);
   }
+}
+
+// Lower bound on the distance from any cell of the window rectangle
+// [x0, x1] x [y0, y1] (lattice coordinates, every layer) to the nearest
+// unconnected pin of the net: each in-layer step costs at least goal_cost and
+// vias cost nothing in the bound.  The per-cell bound goal_cost * Manhattan
+// distance is consistent (it drops by at most one step's cost along an edge),
+// so adding it to tile keys orders tiles by A* priority without changing any
+// label: a change queued with key g can only create labels whose own
+// priority exceeds g + bound.
+__device__ __forceinline__ uint32_t GoalBound(const SearchArgs& a, const NetStatic& ns, uint32_t connected,
+                                              uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1) {
+  if (a.goal_cost == 0) return 0;
+  uint32_t best = 0xFFFFFFFFu;
+  for (uint32_t p = 0; p < ns.pin_count; ++p) {
+    if (connected & (1u << p)) continue;
+    const uint32_t cell = a.pins[ns.pin_offset + p];
+    const uint32_t x = cell % a.W, y = (cell / a.W) % a.H;
+    const uint32_t dx = x < x0 ? x0 - x : (x > x1 ? x - x1 : 0u);
+    const uint32_t dy = y < y0 ? y0 - y : (y > y1 ? y - y1 : 0u);
+    best = min(best, dx + dy);
+  }
+  return best == 0xFFFFFFFFu ? 0u : best * a.goal_cost;
+}
+
+// Tile key from the smallest distance a change starts from and the goal bound
+// of the tile it enters; stays below kInfD (kInfD means nothing queued).
+__device__ __forceinline__ uint32_t TileKey(const SearchArgs& a, const NetStatic& ns, uint32_t connected,
+                                            uint32_t tx, uint32_t ty, uint32_t tw, uint32_t th,
+                                            uint32_t dist) {
+  const uint32_t x0 = ns.x0 + tx * tw, y0 = ns.y0 + ty * th;
+  const uint32_t x1 = min(x0 + tw, ns.x0 + ns.w) - 1, y1 = min(y0 + th, ns.y0 + ns.h) - 1;
+  const uint64_t k = static_cast<uint64_t>(dist) + GoalBound(a, ns, connected, x0, x1, y0, y1);
+  return static_cast<uint32_t>(min(k, static_cast<uint64_t>(kInfD - 1)));
 }
 
 // Admission of a popped item: items of finished nets are dropped, items whose
@@ -349,10 +384,14 @@ __device__ void ProcessTile(const SearchArgs& a, uint32_t g, uint32_t* s_lab, ui
   if (tid == 0) {
     if (!initd) a.tile_init[g] = r;
     const uint32_t next = r + 1;
-    if (s_flags[0] < kInfD && tx > 0) Push(a, g - 1, n, next, s_flags[0]);
-    if (s_flags[1] < kInfD && tx + 1 < ns.tiles_x) Push(a, g + 1, n, next, s_flags[1]);
-    if (s_flags[2] < kInfD && ty > 0) Push(a, g - ns.tiles_x, n, next, s_flags[2]);
-    if (s_flags[3] < kInfD && ty + 1 < ns.tiles_y) Push(a, g + ns.tiles_x, n, next, s_flags[3]);
+    const uint32_t conn = a.state[n].connected;
+    if (s_flags[0] < kInfD && tx > 0) Push(a, g - 1, n, next, TileKey(a, ns, conn, tx - 1, ty, TX, TY, s_flags[0]));
+    if (s_flags[1] < kInfD && tx + 1 < ns.tiles_x)
+      Push(a, g + 1, n, next, TileKey(a, ns, conn, tx + 1, ty, TX, TY, s_flags[1]));
+    if (s_flags[2] < kInfD && ty > 0)
+      Push(a, g - ns.tiles_x, n, next, TileKey(a, ns, conn, tx, ty - 1, TX, TY, s_flags[2]));
+    if (s_flags[3] < kInfD && ty + 1 < ns.tiles_y)
+      Push(a, g + ns.tiles_x, n, next, TileKey(a, ns, conn, tx, ty + 1, TX, TY, s_flags[3]));
     if (!converged) Push(a, g, n, next, 0);
     if (atomicMax(&a.state[n].last_counted, r) < r) atomicAdd(&a.state[n].rounds, 1u);
     atomicAdd(&a.stats[0], 1ull);
@@ -667,15 +706,16 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
       if (k < 5) {
         const uint32_t key = s_side[k];
         bool exists;
-        uint32_t target;
+        uint32_t target, ttx = tx, tty = ty;
         switch (k) {
-          case 0: exists = tx > 0; target = g - 1; break;
-          case 1: exists = tx + 1 < ns.tiles_x; target = g + 1; break;
-          case 2: exists = ty > 0; target = g - ns.tiles_x; break;
-          case 3: exists = ty + 1 < ns.tiles_y; target = g + ns.tiles_x; break;
+          case 0: exists = tx > 0; target = g - 1; ttx = tx - 1; break;
+          case 1: exists = tx + 1 < ns.tiles_x; target = g + 1; ttx = tx + 1; break;
+          case 2: exists = ty > 0; target = g - ns.tiles_x; tty = ty - 1; break;
+          case 3: exists = ty + 1 < ns.tiles_y; target = g + ns.tiles_x; tty = ty + 1; break;
           default: exists = !converged; target = g; break;
         }
-        if (key < kInfD && exists) Push(a, target, n, next, key);
+        if (key < kInfD && exists)
+          Push(a, target, n, next, TileKey(a, ns, a.state[n].connected, ttx, tty, kRT, kRT, key));
       } else if (k == 5) {
         if (atomicMax(&a.state[n].last_counted, r) < r) atomicAdd(&a.state[n].rounds, 1u);
       } else {
@@ -930,6 +970,7 @@ struct apgx_lattice {
   void* d_cost = nullptr;
   int device = 0;
   uint32_t min_edge = 0;      // smallest edge weight (free cell cost or via)
+  uint32_t min_cell_cost = 0; // smallest free cell cost (0 when every cell is blocked)
   uint64_t free_cells = 0;
   double mean_cost = 1.0;     // mean free-cell cost
   std::mutex cache_mutex;
@@ -1005,6 +1046,7 @@ int FinishLattice(const apgx_lattice_desc* desc, const void* host_costs, apgx_la
   const uint8_t* bytes = static_cast<const uint8_t*>(host_costs);
   lat->host_costs.assign(bytes, bytes + cells * desc->cost_bytes);
   lat->min_edge = min_cost;
+  lat->min_cell_cost = free_cells ? min_cost : 0u;
   if (desc->via_cost != 0 && desc->layers > 1) lat->min_edge = std::min(min_cost, desc->via_cost);
   return APGX_OK;
 }
@@ -1255,6 +1297,7 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
   a.net_minkey = reinterpret_cast<uint32_t*>(at(p_net_key));
   a.tile_count = static_cast<uint32_t>(tiles);
   a.min_edge = std::max(lat->min_edge, 1u);
+  a.goal_cost = opt.no_goal_bound ? 0u : lat->min_cell_cost;
   // Bucket width: about one tile width of distance by default; all-ones
   // disables ordering (every queued tile is processed every round).
   if (opt.frontier_delta == 0xFFFFFFFFu) {

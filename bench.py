@@ -44,6 +44,7 @@ def parse_args():
     p.add_argument("--tile", type=str, default="0x0")
     p.add_argument("--delta", type=int, default=0, help="tile scheduling bucket width (0: default)")
     p.add_argument("--max-sweeps", type=int, default=0)
+    p.add_argument("--no-goal-bound", action="store_true", help="order tiles by distance only")
     return p.parse_args()
 
 
@@ -93,7 +94,8 @@ def main():
 
     def step(profile=False):
         routes, stats = lat.route_pairs(pairs, tile=tile, profile_rounds=1 if profile else 0,
-                                        frontier_delta=args.delta, max_sweeps=args.max_sweeps)
+                                        frontier_delta=args.delta, max_sweeps=args.max_sweeps,
+                                        goal_bound=not args.no_goal_bound)
         return routes, stats
 
     for i in range(args.warmup):

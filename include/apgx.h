@@ -170,6 +170,8 @@ typedef struct apgx_route_options {
   uint32_t frontier_delta;    /* distance bucket width of tile scheduling; 0: 64 x mean cell
                                  cost; 0xFFFFFFFF: unordered (every queued tile every round) */
   uint32_t max_sweeps;        /* LDS/register sweeps per tile visit before re-queueing; 0: 64 */
+  uint32_t no_goal_bound;     /* 1: order tiles by distance only; 0: distance plus the
+                                 consistent lower bound to the nearest unconnected pin */
 } apgx_route_options;
 
 typedef struct apgx_route_stats {
