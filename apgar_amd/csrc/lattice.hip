@@ -427,20 +427,27 @@ __global__ __launch_bounds__(kThreads) void RoundKernel(SearchArgs a, uint32_t c
 // Register-resident tile kernel (64 x 64 x L tiles, one wave per layer).
 //
 // Wave l holds layer l of the tile column-major in VGPRs: lane x keeps the 64
-// packed labels of column x and the 64 pre-shifted entry costs (cost << 3,
-// all-ones for a blocked or out-of-window cell).  A relaxation step is a
-// saturating add, an and_or that swaps in the direction code, a min and a
-// compare, with no memory access in the dependency chain:
+// packed labels of column x and the 64 pre-shifted entry costs as signed u16
+// pairs (cost << 3, or 0xFFFF for a blocked or out-of-window cell, so cell
+// costs are limited to kRegMaxCost).  A relaxation step is one saturating add
+// of the sign-extended half (all ones for a blocked cell saturates), an and_or
+// that swaps in the direction code and a min, with no memory access in the
+// dependency chain:
 //   north/south  serial over the lane's own 64 registers;
 //   east/west    after a wave-private LDS transpose (pitch 65: conflict free),
 //                serial over the transposed row, then transposed back;
 //   vias         every wave publishes its layer in LDS, then relaxes each cell
 //                from the layers below and above (one hop per sweep; sweeps
 //                repeat until the tile is stable, so chains of vias converge).
+// Stability test: labels only decrease, so a lane's column is unchanged by a
+// sweep exactly when the sums of its 64 low and of its 64 high label halves
+// (each below 2^22, no wrap) are both unchanged.
 // ---------------------------------------------------------------------------
 constexpr int kRT = 64;          // tile side
 constexpr int kRP = kRT + 1;     // LDS pitch of the transpose buffers
 constexpr uint32_t kBlocked8 = 0xFFFFFFFFu;
+constexpr uint32_t kBlockedHalf = 0xFFFFu;  // packed cost half of a blocked cell
+constexpr uint32_t kRegMaxCost = 4095;      // (cost << 3) must stay a positive int16
 
 __device__ __forceinline__ uint32_t Relax8(uint32_t carry, uint32_t c8, uint32_t code) {
   return (__builtin_elementwise_add_sat(carry, c8) & ~7u) | code;
@@ -452,11 +459,16 @@ __device__ __forceinline__ void WaveSync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Packed cost pair helpers: each 32-bit register holds two u16 entries
-// e = (cost << 3) + 1 for a free cell and 0 for a blocked one, so e - 1 (mod
-// 2^32) is the pre-shifted cost or all ones.
-__device__ __forceinline__ uint32_t CostLo(uint32_t pair) { return (pair & 0xFFFFu) - 1u; }
-__device__ __forceinline__ uint32_t CostHi(uint32_t pair) { return (pair >> 16) - 1u; }
+// Packed cost pair helpers: each 32-bit register holds two 16-bit entries,
+// cost << 3 for a free cell and 0xFFFF for a blocked one; sign extension gives
+// the pre-shifted cost or all ones (both fold into the SDWA operand of the add).
+__device__ __forceinline__ uint32_t CostLo(uint32_t pair) {
+  return static_cast<uint32_t>(static_cast<int32_t>(static_cast<int16_t>(pair & 0xFFFFu)));
+}
+__device__ __forceinline__ uint32_t CostHi(uint32_t pair) {
+  return static_cast<uint32_t>(static_cast<int32_t>(pair) >> 16);
+}
+__device__ __forceinline__ uint32_t PackCost(uint32_t c) { return c ? c << 3 : kBlockedHalf; }
 template <int I>
 __device__ __forceinline__ uint32_t CostAt(const uint32_t* pairs) {
   return (I & 1) ? CostHi(pairs[I >> 1]) : CostLo(pairs[I >> 1]);
@@ -470,7 +482,7 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
   __shared__ uint32_t s_item[2];  // current / prefetched item (parity of the iteration)
   __shared__ uint32_t s_side[5];   // smallest changed border distance per side (W, E, S, N), tile min
   __shared__ uint32_t s_changed[L];
-  const int l = threadIdx.x >> 6;
+  const int l = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));  // wave-uniform layer
   const uint32_t r = a.round;
   const uint32_t count = a.counters[2 * r];
   // The last wave pops the next admitted item during the current item's
@@ -520,7 +532,7 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
 #pragma unroll
       for (int y = 0; y < kRT; ++y) {
         const uint32_t c = cost_bytes == 2 ? cbase16[y * a.W + lane] : cbase32[y * a.W + lane];
-        const uint32_t e = c ? (c << 3) + 1u : 0u;
+        const uint32_t e = PackCost(c);
         if (y & 1) cst[y >> 1] |= e << 16; else cst[y >> 1] = e;
       }
       if (initd) {
@@ -554,7 +566,7 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
         const uint32_t coff = row * a.W + xc;
         uint32_t c = cost_bytes == 2 ? cbase16[coff] : cbase32[coff];
         c = (xin && y < ny) ? c : 0u;
-        T[y * kRP + lane] = c ? (c << 3) + 1u : 0u;
+        T[y * kRP + lane] = PackCost(c);
       }
       WaveSync();
 #pragma unroll
@@ -585,6 +597,18 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
     WaveSync();
     const uint32_t via8 = a.via << 3;
 
+    uint32_t sum_lo = 0, sum_hi = 0;  // label half sums of the lane's column
+#pragma unroll
+    for (int y = 0; y < kRT; ++y) {
+      sum_lo += lab[y] & 0xFFFFu;
+      sum_hi += lab[y] >> 16;
+    }
+    // Vias from a missing layer get an all-ones cost (the add saturates).
+    const uint32_t via_b = l > 0 ? via8 : kBlocked8;
+    const uint32_t via_a = l + 1 < L ? via8 : kBlocked8;
+    const uint32_t* below = s_t + (l > 0 ? l - 1 : l) * kRT * kRP;
+    const uint32_t* above = s_t + (l + 1 < L ? l + 1 : l) * kRT * kRP;
+
     uint32_t sweeps = 0;
     bool converged = false;
 #pragma unroll 1
@@ -593,24 +617,19 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
       // 128 entries once outside the loop and runs out of registers.
 #pragma unroll
       for (int i = 0; i < kRT / 2; ++i) asm volatile("" : "+v"(cst[i]), "+v"(cstr[i]));
-      uint32_t chv = 0;  // OR of (old ^ new) over every relaxation of this sweep
       uint32_t carry = halo_s;
 #pragma unroll
       for (int y = 0; y < kRT; ++y) {
         const uint32_t c8 = (y & 1) ? CostHi(cst[y >> 1]) : CostLo(cst[y >> 1]);
-        const uint32_t nv = min(lab[y], Relax8(carry, c8, kFromSouth));
-        chv |= nv ^ lab[y];
-        lab[y] = nv;
-        carry = nv;
+        lab[y] = min(lab[y], Relax8(carry, c8, kFromSouth));
+        carry = lab[y];
       }
       carry = halo_n;
 #pragma unroll
       for (int y = kRT - 1; y >= 0; --y) {
         const uint32_t c8 = (y & 1) ? CostHi(cst[y >> 1]) : CostLo(cst[y >> 1]);
-        const uint32_t nv = min(lab[y], Relax8(carry, c8, kFromNorth));
-        chv |= nv ^ lab[y];
-        lab[y] = nv;
-        carry = nv;
+        lab[y] = min(lab[y], Relax8(carry, c8, kFromNorth));
+        carry = lab[y];
       }
 #pragma unroll
       for (int y = 0; y < kRT; ++y) T[y * kRP + lane] = lab[y];
@@ -621,19 +640,15 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
 #pragma unroll
       for (int x = 0; x < kRT; ++x) {
         const uint32_t c8 = (x & 1) ? CostHi(cstr[x >> 1]) : CostLo(cstr[x >> 1]);
-        const uint32_t nv = min(lab[x], Relax8(carry, c8, kFromWest));
-        chv |= nv ^ lab[x];
-        lab[x] = nv;
-        carry = nv;
+        lab[x] = min(lab[x], Relax8(carry, c8, kFromWest));
+        carry = lab[x];
       }
       carry = halo_e;
 #pragma unroll
       for (int x = kRT - 1; x >= 0; --x) {
         const uint32_t c8 = (x & 1) ? CostHi(cstr[x >> 1]) : CostLo(cstr[x >> 1]);
-        const uint32_t nv = min(lab[x], Relax8(carry, c8, kFromEast));
-        chv |= nv ^ lab[x];
-        lab[x] = nv;
-        carry = nv;
+        lab[x] = min(lab[x], Relax8(carry, c8, kFromEast));
+        carry = lab[x];
       }
       WaveSync();
 #pragma unroll
@@ -641,25 +656,32 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
       WaveSync();
 #pragma unroll
       for (int y = 0; y < kRT; ++y) lab[y] = T[y * kRP + lane];
+      if (a.via != 0) {  // the host passes via = 0 for single-layer lattices
+        __syncthreads();  // every layer published in its buffer
+#pragma unroll
+        for (int y = 0; y < kRT; ++y) {
+          // All ones for a blocked cell (sign of its packed cost half): the
+          // via adds then saturate, so the cell keeps its label.  Arithmetic
+          // rather than a branch keeps both LDS reads unconditional.
+          const uint32_t blk = (y & 1) ? static_cast<uint32_t>(static_cast<int32_t>(cst[y >> 1]) >> 31)
+                                       : static_cast<uint32_t>(static_cast<int32_t>(cst[y >> 1] << 16) >> 31);
+          const uint32_t fb = __builtin_elementwise_add_sat(below[y * kRP + lane], via_b | blk) & ~7u;  // kFromBelow = 0
+          const uint32_t fa = Relax8(above[y * kRP + lane], via_a | blk, kFromAbove);
+          lab[y] = min(lab[y], min(fb, fa));
+        }
+      }
+      uint32_t new_lo = 0, new_hi = 0;
+#pragma unroll
+      for (int y = 0; y < kRT; ++y) {
+        new_lo += lab[y] & 0xFFFFu;
+        new_hi += lab[y] >> 16;
+      }
+      const bool lane_changed = new_lo != sum_lo || new_hi != sum_hi;
+      sum_lo = new_lo;
+      sum_hi = new_hi;
       bool any;
       {
-        if (a.via != 0) {  // the host passes via = 0 for single-layer lattices
-          __syncthreads();  // every layer published in its buffer
-          const uint32_t* below = s_t + (l > 0 ? l - 1 : 0) * kRT * kRP;
-          const uint32_t* above = s_t + (l + 1 < L ? l + 1 : l) * kRT * kRP;
-          const bool has_below = l > 0, has_above = l + 1 < L;
-#pragma unroll
-          for (int y = 0; y < kRT; ++y) {
-            const uint32_t c8 = (y & 1) ? CostHi(cst[y >> 1]) : CostLo(cst[y >> 1]);
-            const uint32_t vc = c8 == kBlocked8 ? kBlocked8 : via8;
-            const uint32_t fb = has_below ? Relax8(below[y * kRP + lane], vc, kFromBelow) : kInf;
-            const uint32_t fa = has_above ? Relax8(above[y * kRP + lane], vc, kFromAbove) : kInf;
-            const uint32_t nv = min(lab[y], min(fb, fa));
-            chv |= nv ^ lab[y];
-            lab[y] = nv;
-          }
-        }
-        const bool wave_any = __any(chv != 0);
+        const bool wave_any = __any(lane_changed);
         if (lane == 0) s_changed[l] = wave_any ? 1u : 0u;
         __syncthreads();
         any = false;
@@ -971,6 +993,7 @@ struct apgx_lattice {
   int device = 0;
   uint32_t min_edge = 0;      // smallest edge weight (free cell cost or via)
   uint32_t min_cell_cost = 0; // smallest free cell cost (0 when every cell is blocked)
+  uint32_t max_cell_cost = 0; // largest free cell cost
   uint64_t free_cells = 0;
   double mean_cost = 1.0;     // mean free-cell cost
   std::mutex cache_mutex;
@@ -1047,6 +1070,7 @@ int FinishLattice(const apgx_lattice_desc* desc, const void* host_costs, apgx_la
   lat->host_costs.assign(bytes, bytes + cells * desc->cost_bytes);
   lat->min_edge = min_cost;
   lat->min_cell_cost = free_cells ? min_cost : 0u;
+  lat->max_cell_cost = max_cost;
   if (desc->via_cost != 0 && desc->layers > 1) lat->min_edge = std::min(min_cost, desc->via_cost);
   return APGX_OK;
 }
@@ -1107,9 +1131,15 @@ extern "C" const void* apgx_lattice_device_costs(const apgx_lattice* lat) {
 namespace apgx {
 namespace {
 
-TileShape DefaultShape(const apgx_lattice_desc& d, size_t lds_limit) {
-  // Register-resident 64 x 64 tiles (one wave per layer) up to eight layers.
-  if (d.layers <= 8 && RegTileLdsBytes(d.layers) <= lds_limit) return TileShape{64, 64};
+// Register-resident 64 x 64 tiles: one wave per layer, up to eight layers,
+// cell costs up to kRegMaxCost.
+bool RegTileUsable(const apgx_lattice& lat, size_t lds_limit) {
+  return lat.desc.layers <= 8 && lat.max_cell_cost <= kRegMaxCost && RegTileLdsBytes(lat.desc.layers) <= lds_limit;
+}
+
+TileShape DefaultShape(const apgx_lattice& lat, size_t lds_limit) {
+  const apgx_lattice_desc& d = lat.desc;
+  if (RegTileUsable(lat, lds_limit)) return TileShape{64, 64};
   const TileShape candidates[] = {{64, 64}, {64, 32}, {32, 32}, {32, 16}};
   // Prefer the largest tile that fits the register-resident cell budget and
   // leaves room for two workgroups per CU.
@@ -1149,12 +1179,12 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
   hipDeviceProp_t prop;
   APGX_HIP(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
   const size_t lds_limit = prop.sharedMemPerBlock;
-  TileShape shape = DefaultShape(d, lds_limit);
+  TileShape shape = DefaultShape(*lat, lds_limit);
   if (opt.tile_w || opt.tile_h) shape = TileShape{static_cast<int>(opt.tile_w), static_cast<int>(opt.tile_h)};
   Kernels k{};
   if (!KernelsForShape(shape, &k))
     return Fail(APGX_E_UNSUPPORTED, "apgx_route_nets: unsupported tile shape");
-  const bool reg_tile = shape.tx == 64 && shape.ty == 64 && d.layers <= 8;
+  const bool reg_tile = shape.tx == 64 && shape.ty == 64 && RegTileUsable(*lat, lds_limit);
   uint32_t block = kThreads;
   size_t lds = TileLdsBytes(shape, d.layers);
   if (reg_tile) {
