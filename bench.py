@@ -45,6 +45,8 @@ def parse_args():
     p.add_argument("--delta", type=int, default=0, help="tile scheduling bucket width (0: default)")
     p.add_argument("--max-sweeps", type=int, default=0)
     p.add_argument("--no-goal-bound", action="store_true", help="order tiles by distance only")
+    p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                   help="only run a device-to-device copy of MIB MiB (HBM counter scale reference)")
     return p.parse_args()
 
 
@@ -78,8 +80,26 @@ def cpu_baseline(cost, pairs, sample, threads):
                       f"one A* per thread, {threads} threads, {dt:.1f} s"}
 
 
+def copy_calibration(mib):
+    """A plain copy kernel of known size: reads and writes `mib` MiB once, so
+    rocprofv3's FETCH_SIZE / WRITE_SIZE for it give the counters' scale."""
+    import torch
+
+    n = mib << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n, "launches": 3}))
+
+
 def main():
     args = parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); queries are "
               "independent and would shard by query list, which this build does not launch.",
@@ -159,7 +179,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "RoundKernel",
+            "kernel": (f"RegRoundKernel<{cost.shape[0]}>" if (st.tile_w, st.tile_h) == (64, 64)
+                       else f"RoundKernel<{st.tile_w},{st.tile_h}>"),
             "launches": launches,
             "avg_launch_us": avg_launch_s * 1e6,
             "algorithmic_bytes_per_launch": bytes_per_launch,

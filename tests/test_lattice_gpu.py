@@ -103,6 +103,30 @@ def test_windows_and_multi_pin_nets(apgx, oracle, gpu, seed):
     lat.close()
 
 
+@pytest.mark.parametrize("layers", [1, 3])
+def test_costs_above_register_tile_limit(apgx, oracle, gpu, layers):
+    """Cell costs above 4095 do not fit the register tile's packed halves; the
+    search must fall back to the LDS tile kernel and stay exact."""
+    _say(f"[case] large costs layers={layers}")
+    rng = np.random.default_rng(77 + layers)
+    cost = random_lattice(rng, layers, 70, 90, blocked=0.1, cmax=8)
+    cost = (cost.astype(np.int64) * 9000).astype(np.uint32)  # up to 72000
+    via = 0 if layers == 1 else 40000
+    free = np.flatnonzero(cost.reshape(-1))
+    pairs = rng.choice(free, size=(10, 2), replace=False)
+    lat = apgx.Lattice(cost, via_cost=via)
+    routes, stats = lat.route_pairs(pairs)
+    o = oracle.Dense(cost, via)
+    for (s, t), r in zip(pairs, routes):
+        dist = o.dijkstra([int(s)])
+        if dist[t] == UNR:
+            assert r.status == apgx.NET_DISCONNECTED
+            continue
+        assert r.status == apgx.NET_REACHED and r.cost == dist[t]
+        assert np.array_equal(r.connections[0][2], o.canonical_path(dist, int(t)))
+    lat.close()
+
+
 def test_edge_cases(apgx, gpu):
     _say("[case] edge cases")
     cost = np.ones((1, 16, 16), dtype=np.uint32)
