@@ -80,6 +80,16 @@ class RouteStats(C.Structure):
                 ("graph_launches", C.c_uint32), ("round_kernel_ms", C.c_double)]
 
 
+class TuneCandidate(C.Structure):
+    _fields_ = [("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("frontier_delta", C.c_uint32),
+                ("max_sweeps", C.c_uint32), ("device_ms", C.c_double)]
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("sample_nets", C.c_uint32), ("candidate_count", C.c_uint32), ("best", C.c_uint32),
+                ("reserved", C.c_uint32), ("candidates", TuneCandidate * 24)]
+
+
 class LatticeGenParams(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("layers", C.c_uint32),
                 ("cost_min", C.c_uint32), ("cost_max", C.c_uint32), ("blocked_ppm", C.c_uint32),
@@ -104,6 +114,8 @@ _SIGNATURES = {
                                        _P(C.c_uint32), C.c_uint64]),
     "apgx_search_stats": (C.c_int, [C.c_void_p, _P(RouteStats)]),
     "apgx_search_release": (None, [C.c_void_p]),
+    "apgx_autotune": (C.c_int, [C.c_void_p, _P(NetQuery), C.c_uint32, _P(C.c_uint32), C.c_uint64,
+                                C.c_uint32, _P(RouteOptions), _P(RouteOptions), _P(TuneReport)]),
     "apgx_generate_lattice": (C.c_int, [_P(LatticeGenParams), _P(C.c_uint32)]),
     "apgx_generate_pairs": (C.c_int, [_P(C.c_uint32), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                       C.c_uint64, _P(C.c_uint32)]),
@@ -255,6 +267,31 @@ class Lattice:
                            cells[k.cell_offset:k.cell_offset + k.cell_count].copy()))
             routes.append(NetRoute(r.net_id, r.status, r.cost, r.rounds, cs))
         return routes, stats
+
+    def autotune(self, nets, pins: np.ndarray, sample_nets=256, **base):
+        """Picks tile shape and schedule on a strided sample of the nets.
+
+        Returns (options, report): options is a dict of route() keyword
+        arguments, report a list of (tile_w, tile_h, frontier_delta,
+        max_sweeps, device_ms) per candidate with the chosen index."""
+        pins = np.ascontiguousarray(pins, dtype=np.uint32)
+        b = RouteOptions()
+        for k, v in base.items():
+            if k == "tile":
+                b.tile_w, b.tile_h = v
+            elif k == "goal_bound":
+                b.no_goal_bound = 0 if v else 1
+            else:
+                setattr(b, k, v)
+        out, rep = RouteOptions(), TuneReport()
+        check(lib().apgx_autotune(self.handle, nets, len(nets), u32p(pins), pins.size, sample_nets,
+                                  C.byref(b), C.byref(out), C.byref(rep)))
+        opts = dict(base)
+        opts.update(tile=(out.tile_w, out.tile_h), frontier_delta=out.frontier_delta,
+                    max_sweeps=out.max_sweeps)
+        cands = [(c.tile_w, c.tile_h, c.frontier_delta, c.max_sweeps, c.device_ms)
+                 for c in rep.candidates[:rep.candidate_count]]
+        return opts, {"sample_nets": rep.sample_nets, "best": rep.best, "candidates": cands}
 
     def route_pairs(self, pairs: np.ndarray, window=None, **kw):
         """Single-pair queries (pairs: (n, 2) source/target cells)."""

@@ -139,6 +139,16 @@ extern "C" int apgx_generate_nets(const uint32_t* cost, uint32_t width, uint32_t
   }
   Rng rng(seed);
   const uint64_t plane = static_cast<uint64_t>(width) * height;
+  // Summed-area table of free cells over all layers: window free counts in O(1).
+  const uint64_t sw = static_cast<uint64_t>(width) + 1;
+  std::vector<uint64_t> sat(sw * (static_cast<uint64_t>(height) + 1), 0);
+  for (uint32_t y = 0; y < height; ++y) {
+    uint64_t row = 0;
+    for (uint32_t x = 0; x < width; ++x) {
+      for (uint32_t l = 0; l < layers; ++l) row += cost[l * plane + static_cast<uint64_t>(y) * width + x] != 0;
+      sat[(y + 1) * sw + x + 1] = sat[y * sw + x + 1] + row;
+    }
+  }
   uint64_t used = 0;
   const double ratio = std::log2(static_cast<double>(max_side) / min_side);
   for (uint32_t n = 0; n < count; ++n) {
@@ -148,11 +158,8 @@ extern "C" int apgx_generate_nets(const uint32_t* cost, uint32_t width, uint32_t
     const uint32_t x0 = rng.Range(0, width - side);
     const uint32_t y0 = rng.Range(0, height - side);
     const uint32_t want = rng.Range(min_pins, max_pins);
-    uint64_t window_free = 0;
-    for (uint32_t l = 0; l < layers; ++l)
-      for (uint32_t y = y0; y < y0 + side; ++y)
-        for (uint32_t x = x0; x < x0 + side; ++x)
-          window_free += cost[l * plane + static_cast<uint64_t>(y) * width + x] != 0;
+    const uint64_t x1 = x0 + side, y1 = y0 + side;
+    const uint64_t window_free = sat[y1 * sw + x1] - sat[y0 * sw + x1] - sat[y1 * sw + x0] + sat[y0 * sw + x0];
     const uint32_t pins = static_cast<uint32_t>(std::min<uint64_t>(want, window_free));
     nets_out[n].net_id = n + 1;
     nets_out[n].pin_offset = static_cast<uint32_t>(used);

@@ -103,7 +103,10 @@ struct SearchArgs {
   uint32_t* cells;
   unsigned long long* cell_cursor;
   unsigned long long cell_capacity;
-  uint32_t round;      // round being executed (round kernel) / next round (phase kernel)
+  // Device-resident round number: kernels read it at entry and the phase
+  // kernel's last block advances it, so one captured graph of a chunk of
+  // rounds replays unchanged for every chunk.
+  uint32_t* round_ptr;  // [0] round, [1] phase-kernel block completion count
   uint32_t max_sweeps;
   // Distance-ordered scheduling (delta-stepping over tiles).  tile_key holds,
   // per round parity, the smallest distance that can newly enter a queued
@@ -173,8 +176,7 @@ __device__ __forceinline__ uint32_t TileKey(const SearchArgs& a, const NetStatic
 // key is beyond the net's bucket (smallest pending key + delta) move to the
 // next round unprocessed.  The round's key slot is cleared for reuse two
 // rounds later.
-__device__ __forceinline__ bool AdmitItem(const SearchArgs& a, uint32_t g) {
-  const uint32_t r = a.round;
+__device__ __forceinline__ bool AdmitItem(const SearchArgs& a, uint32_t r, uint32_t g) {
   uint32_t* slot = &a.tile_key[(r & 1) * a.tile_count + g];
   const uint32_t key = *slot;
   *slot = kInfD;
@@ -191,12 +193,12 @@ __device__ __forceinline__ bool AdmitItem(const SearchArgs& a, uint32_t g) {
 
 // Pops the next admitted item of this round (thread 0 only); kNever when the
 // round's list is exhausted.
-__device__ __forceinline__ uint32_t PopAdmitted(const SearchArgs& a, uint32_t count) {
+__device__ __forceinline__ uint32_t PopAdmitted(const SearchArgs& a, uint32_t r, uint32_t count) {
   for (;;) {
-    const uint32_t i = atomicAdd(&a.counters[2 * a.round + 1], 1u);
+    const uint32_t i = atomicAdd(&a.counters[2 * r + 1], 1u);
     if (i >= count) return kNever;
-    const uint32_t g = a.wl[a.round & 1][i];
-    if (AdmitItem(a, g)) return g;
+    const uint32_t g = a.wl[r & 1][i];
+    if (AdmitItem(a, r, g)) return g;
   }
 }
 
@@ -206,7 +208,7 @@ __device__ __forceinline__ uint32_t Step(uint32_t carry_d, uint32_t cost, uint32
 
 // One tile visit: stage, sweep to local convergence, write back, activate.
 template <int TX, int TY>
-__device__ void ProcessTile(const SearchArgs& a, uint32_t g, uint32_t* s_lab, uint32_t* s_cost,
+__device__ void ProcessTile(const SearchArgs& a, uint32_t r, uint32_t g, uint32_t* s_lab, uint32_t* s_cost,
                             uint32_t* s_flags, uint32_t cost_bytes) {
   constexpr int PX = TX + 3;   // odd LDS pitch: row scans are bank-conflict free
   constexpr int CPX = TX + 1;
@@ -222,7 +224,6 @@ __device__ void ProcessTile(const SearchArgs& a, uint32_t g, uint32_t* s_lab, ui
   uint32_t* lab = a.labels + ns.label_offset;
   const uint64_t w = ns.w, h = ns.h;
   const bool initd = a.tile_init[g] != kNever;
-  const uint32_t r = a.round;
   auto ready = [&](uint32_t ntx, uint32_t nty) {
     const uint32_t v = a.tile_init[ns.tile_offset + nty * ns.tiles_x + ntx];
     return v != kNever && v < r;
@@ -409,16 +410,17 @@ __global__ __launch_bounds__(kThreads) void RoundKernel(SearchArgs a, uint32_t c
   uint32_t* s_cost = smem + static_cast<int>(a.L) * (TY + 2) * PX;
   __shared__ uint32_t s_item;
   __shared__ uint32_t s_flags[4];
-  const uint32_t count = a.counters[2 * a.round];
+  const uint32_t r = __builtin_amdgcn_readfirstlane(*a.round_ptr);
+  const uint32_t count = a.counters[2 * r];
   for (;;) {
     if (threadIdx.x == 0) {
-      s_item = PopAdmitted(a, count);
+      s_item = PopAdmitted(a, r, count);
       for (int k = 0; k < 4; ++k) s_flags[k] = kInfD;
     }
     __syncthreads();
     const uint32_t g = s_item;
     if (g == kNever) break;
-    ProcessTile<TX, TY>(a, g, s_lab, s_cost, s_flags, cost_bytes);
+    ProcessTile<TX, TY>(a, r, g, s_lab, s_cost, s_flags, cost_bytes);
     __syncthreads();
   }
 }
@@ -483,12 +485,12 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
   __shared__ uint32_t s_side[5];   // smallest changed border distance per side (W, E, S, N), tile min
   __shared__ uint32_t s_changed[L];
   const int l = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));  // wave-uniform layer
-  const uint32_t r = a.round;
+  const uint32_t r = __builtin_amdgcn_readfirstlane(*a.round_ptr);
   const uint32_t count = a.counters[2 * r];
   // The last wave pops the next admitted item during the current item's
   // epilogue, so the pop's dependent atomics overlap the pushes.
   const bool popper = threadIdx.x == 64 * (L - 1);
-  if (popper) s_item[0] = PopAdmitted(a, count);
+  if (popper) s_item[0] = PopAdmitted(a, r, count);
   for (uint32_t iter = 0;; ++iter) {
     if (threadIdx.x == 0) {
       for (int k = 0; k < 5; ++k) s_side[k] = kInfD;
@@ -747,7 +749,7 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
         atomicAdd(&a.stats[3], static_cast<unsigned long long>(sweeps) * nx * ny * L);
       }
     }
-    if (popper) s_item[(iter + 1) & 1] = PopAdmitted(a, count);
+    if (popper) s_item[(iter + 1) & 1] = PopAdmitted(a, r, count);
   }
 }
 
@@ -824,7 +826,7 @@ __device__ __forceinline__ uint32_t Neighbour(uint32_t cell, uint32_t code, uint
 }
 
 template <int TX, int TY>
-__device__ void WalkConnection(const SearchArgs& a, const NetStatic& ns, NetState& st,
+__device__ void WalkConnection(const SearchArgs& a, uint32_t r, const NetStatic& ns, NetState& st,
                                DevConnection& c, uint32_t start, bool reseed, uint32_t n);
 
 // Runs after every round.  Nets without queued work hold exact distances from
@@ -832,15 +834,33 @@ __device__ void WalkConnection(const SearchArgs& a, const NetStatic& ns, NetStat
 // index) along the predecessor chain, make that path part of the tree and
 // queue its tiles for the next round.  One thread per net.
 template <int TX, int TY>
+__device__ void PhaseNet(const SearchArgs& a, uint32_t r, uint32_t n);
+
+template <int TX, int TY>
 __global__ void PhaseKernel(SearchArgs a) {
+  const uint32_t r = __builtin_amdgcn_readfirstlane(*a.round_ptr) + 1;  // the round new work goes to
   const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= a.net_count) return;
+  if (n < a.net_count) PhaseNet<TX, TY>(a, r, n);
+  // The last block to finish advances the device round: every block read it
+  // above, before its completion count.
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&a.round_ptr[1], 1u) == gridDim.x - 1) {
+      a.round_ptr[1] = 0;
+      atomicAdd(&a.round_ptr[0], 1u);
+    }
+  }
+}
+
+template <int TX, int TY>
+__device__ void PhaseNet(const SearchArgs& a, uint32_t r, uint32_t n) {
   // The key buffer read by the round just executed collects pushes of the
   // round after the next one.
-  a.net_minkey[((a.round - 1) & 1) * a.net_count + n] = kInfD;
+  a.net_minkey[((r - 1) & 1) * a.net_count + n] = kInfD;
   NetState& st = a.state[n];
   if (st.status != kSearching) return;
-  const uint32_t pending = a.net_minkey[(a.round & 1) * a.net_count + n];
+  const uint32_t pending = a.net_minkey[(r & 1) * a.net_count + n];
   const NetStatic ns = a.nets[n];
   uint32_t* lab = a.labels + ns.label_offset;
   const uint32_t plane = a.W * a.H;
@@ -879,7 +899,7 @@ __global__ void PhaseKernel(SearchArgs a) {
     st.status = kFinalWalk;
     return;
   }
-  WalkConnection<TX, TY>(a, ns, st, c, a.pins[ns.pin_offset + best_pin], true, n);
+  WalkConnection<TX, TY>(a, r, ns, st, c, a.pins[ns.pin_offset + best_pin], true, n);
 }
 
 // Writes the predecessor chain of a settled pin back to the tree (distance 0)
@@ -889,7 +909,7 @@ __global__ void PhaseKernel(SearchArgs a) {
 // their tiles are queued.  The step bound and window checks guard against a
 // corrupted word only.
 template <int TX, int TY>
-__device__ void WalkConnection(const SearchArgs& a, const NetStatic& ns, NetState& st,
+__device__ void WalkConnection(const SearchArgs& a, uint32_t r, const NetStatic& ns, NetState& st,
                                DevConnection& c, uint32_t start, bool reseed, uint32_t n) {
   uint32_t* lab = a.labels + ns.label_offset;
   const uint32_t plane = a.W * a.H;
@@ -926,7 +946,7 @@ __device__ void WalkConnection(const SearchArgs& a, const NetStatic& ns, NetStat
       lab[li] = kSourceWord;
       const uint32_t g = TileOf<TX, TY>(ns, a.W, a.H, v);
       if (g != pushed) {
-        Push(a, g, n, a.round, 0u);
+        Push(a, g, n, r, 0u);
         pushed = g;
       }
     }
@@ -946,7 +966,7 @@ __global__ void FinalWalkKernel(SearchArgs a) {
   const NetStatic ns = a.nets[n];
   DevConnection& c = a.conns[ns.conn_offset + st.n_connected - 1];
   st.status = APGX_NET_REACHED;
-  WalkConnection<TX, TY>(a, ns, st, c, a.pins[ns.pin_offset + c.pin_index], false, n);
+  WalkConnection<TX, TY>(a, 0u, ns, st, c, a.pins[ns.pin_offset + c.pin_index], false, n);
 }
 
 struct TileShape {
@@ -1276,9 +1296,10 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
   Part p_cells{4 * cell_capacity, 0};
   Part p_tile_key{8 * std::max<uint64_t>(tiles, 1), 0};
   Part p_net_key{8 * std::max<uint32_t>(net_count, 1), 0};
+  Part p_round{8, 0};
   Part* parts[] = {&p_nets, &p_state, &p_tile_net, &p_stamp, &p_init, &p_wl0, &p_wl1,
                    &p_counters, &p_stats, &p_pins, &p_conns, &p_cursor, &p_cells,
-                   &p_tile_key, &p_net_key};
+                   &p_tile_key, &p_net_key, &p_round};
   size_t total = 0;
   for (Part* p : parts) { p->offset = total; total += align(p->bytes); }
   const size_t label_bytes = static_cast<size_t>(std::max<uint64_t>(label_words, 1)) * 4;
@@ -1325,6 +1346,7 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
   a.max_sweeps = opt.max_sweeps ? opt.max_sweeps : 64;
   a.tile_key = reinterpret_cast<uint32_t*>(at(p_tile_key));
   a.net_minkey = reinterpret_cast<uint32_t*>(at(p_net_key));
+  a.round_ptr = reinterpret_cast<uint32_t*>(at(p_round));
   a.tile_count = static_cast<uint32_t>(tiles);
   a.min_edge = std::max(lat->min_edge, 1u);
   a.goal_cost = opt.no_goal_bound ? 0u : lat->min_cell_cost;
@@ -1365,6 +1387,8 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
   check(hipMemsetAsync(a.cell_cursor, 0, 8, stream), "cursor clear");
   check(hipMemsetAsync(a.tile_key, 0xFF, 8 * std::max<uint64_t>(tiles, 1), stream), "key clear");
   check(hipMemsetAsync(a.net_minkey, 0xFF, 8 * std::max<uint32_t>(net_count, 1), stream), "key clear");
+  check(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(a.round_ptr), 1u, 1, stream), "round init");
+  check(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(a.round_ptr + 1), 0u, 1, stream), "round init");
   if (status == APGX_OK && lds > 65536)
     check(hipFuncSetAttribute(k.round, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)), "hipFuncSetAttribute");
 
@@ -1386,38 +1410,59 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
   const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(
       static_cast<uint64_t>(prop.multiProcessorCount) * occupancy, std::max<uint64_t>(tiles, 1))));
   const uint32_t chunk = opt.rounds_per_sync ? opt.rounds_per_sync : 8;
-  uint32_t r = 1, syncs = 0;
+  uint32_t r = 1, syncs = 0, graph_launches = 0;
   std::vector<hipEvent_t> round_events;
   if (status == APGX_OK && net_count) {
     check(hipEventRecord(ev0, stream), "hipEventRecord");
-    a.round = 0;
     {
       void* args[] = {&a};
       if (status == APGX_OK) status = Launch(k.seed, dim3(net_count), dim3(kThreads), 0, stream, args, "seed launch");
     }
     bool done = false;
-    while (status == APGX_OK && !done) {
-      // Rounds 1..max_rounds only: the counter array holds max_rounds + 2 rounds.
-      const uint32_t this_chunk = std::min(chunk, max_rounds + 1 - r);
-      for (uint32_t c = 0; c < this_chunk && status == APGX_OK; ++c, ++r) {
-        a.round = r;
+    hipGraphExec_t graph_exec = nullptr;
+    // One chunk of rounds: round kernel then phase kernel, per round.  The
+    // kernels take the round from device memory, so the same launches replay.
+    auto launch_chunk = [&](uint32_t rounds, bool timed) {
+      for (uint32_t c = 0; c < rounds && status == APGX_OK; ++c) {
         uint32_t cb = d.cost_bytes;
         void* rargs[] = {&a, &cb};
-        if (opt.profile_rounds) {
+        if (timed) {
           hipEvent_t e0 = nullptr, e1 = nullptr;
-          if (!check(hipEventCreate(&e0), "hipEventCreate") || !check(hipEventCreate(&e1), "hipEventCreate")) break;
+          if (!check(hipEventCreate(&e0), "hipEventCreate") || !check(hipEventCreate(&e1), "hipEventCreate")) return;
           round_events.push_back(e0);
           round_events.push_back(e1);
           check(hipEventRecord(e0, stream), "hipEventRecord");
         }
         status = Launch(k.round, dim3(grid), dim3(block), lds, stream, rargs, "round launch");
-        if (status != APGX_OK) break;
-        if (opt.profile_rounds) check(hipEventRecord(round_events.back(), stream), "hipEventRecord");
-        SearchArgs pa = a;
-        pa.round = r + 1;
-        void* pargs[] = {&pa};
+        if (status != APGX_OK) return;
+        if (timed) check(hipEventRecord(round_events.back(), stream), "hipEventRecord");
+        void* pargs[] = {&a};
         status = Launch(k.phase, dim3((net_count + 63) / 64), dim3(64), 0, stream, pargs, "phase launch");
       }
+    };
+    while (status == APGX_OK && !done) {
+      // Rounds 1..max_rounds only: the counter array holds max_rounds + 2 rounds.
+      const uint32_t this_chunk = std::min(chunk, max_rounds + 1 - r);
+      if (opt.use_graph && !opt.profile_rounds && this_chunk == chunk) {
+        if (graph_exec == nullptr) {
+          hipGraph_t graph = nullptr;
+          if (!check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture")) break;
+          launch_chunk(chunk, false);
+          const bool captured = check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
+          if (status != APGX_OK || !captured) {
+            if (graph) (void)hipGraphDestroy(graph);
+            break;
+          }
+          const bool ok = check(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+          (void)hipGraphDestroy(graph);
+          if (!ok) break;
+        }
+        if (!check(hipGraphLaunch(graph_exec, stream), "hipGraphLaunch")) break;
+        ++graph_launches;
+      } else {
+        launch_chunk(this_chunk, opt.profile_rounds != 0);
+      }
+      r += this_chunk;
       uint32_t next_count = 0;
       if (!check(hipMemcpyAsync(&next_count, a.counters + 2ull * r, 4, hipMemcpyDeviceToHost, stream), "status readback")) break;
       if (!check(hipStreamSynchronize(stream), "hipStreamSynchronize")) break;
@@ -1425,6 +1470,7 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
       if (next_count == 0) done = true;
       else if (r > max_rounds) done = true;  // budget: remaining searching nets stop
     }
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (status == APGX_OK) {
       void* fargs[] = {&a};
       status = Launch(k.final_walk, dim3((net_count + 63) / 64), dim3(64), 0, stream, fargs, "final walk launch");
@@ -1481,6 +1527,7 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
   rs.tile_w = static_cast<uint32_t>(shape.tx);
   rs.tile_h = static_cast<uint32_t>(shape.ty);
   rs.host_syncs = syncs;
+  rs.graph_launches = graph_launches;
   rs.round_kernel_ms = round_ms;
   rs.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall_start).count();
   *out = s;
@@ -1574,4 +1621,71 @@ extern "C" void apgx_search_release(apgx_search* s) {
     }
   }
   delete s;
+}
+
+extern "C" int apgx_autotune(apgx_lattice* lat, const apgx_net_query* nets, uint32_t net_count,
+                             const uint32_t* pins, uint64_t pin_count, uint32_t sample_nets,
+                             const apgx_route_options* base, apgx_route_options* tuned,
+                             apgx_tune_report* report) {
+  using namespace apgx;
+  if (lat == nullptr || tuned == nullptr || (net_count && (nets == nullptr || pins == nullptr)))
+    return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_autotune: null argument");
+  apgx_route_options opt{};
+  if (base) opt = *base;
+  opt.profile_rounds = 0;
+  *tuned = opt;
+  const uint32_t want = sample_nets ? sample_nets : 256u;
+  const uint32_t sample = std::min(net_count, want);
+  std::vector<apgx_net_query> picked(sample);
+  for (uint32_t i = 0; i < sample; ++i)
+    picked[i] = nets[static_cast<uint64_t>(i) * net_count / std::max(sample, 1u)];
+  hipDeviceProp_t prop{};
+  APGX_HIP(hipGetDeviceProperties(&prop, lat->device), "hipGetDeviceProperties");
+  const bool reg = RegTileUsable(*lat, prop.sharedMemPerBlock);
+  const uint32_t mean = static_cast<uint32_t>(std::max(1.0, lat->mean_cost));
+  std::vector<apgx_tune_candidate> cands;
+  const uint32_t deltas[] = {16 * mean, 64 * mean, 256 * mean, 0xFFFFFFFFu};
+  const TileShape shapes[] = {{64, 64}, {64, 32}, {32, 32}};
+  for (const TileShape& t : shapes) {
+    if (t.tx == 64 && t.ty == 64 && !reg) continue;
+    if (!(t.tx == 64 && t.ty == 64) &&
+        (static_cast<uint64_t>(t.tx) * t.ty * lat->desc.layers > kMaxTileCells ||
+         TileLdsBytes(t, lat->desc.layers) > prop.sharedMemPerBlock))
+      continue;
+    for (uint32_t dl : deltas) {
+      cands.push_back(apgx_tune_candidate{static_cast<uint32_t>(t.tx), static_cast<uint32_t>(t.ty), dl, 64, 0.0});
+      if (t.tx == 64 && t.ty == 64 && reg)
+        cands.push_back(apgx_tune_candidate{64, 64, dl, 16, 0.0});
+    }
+  }
+  if (cands.size() > APGX_TUNE_MAX_CANDIDATES) cands.resize(APGX_TUNE_MAX_CANDIDATES);
+  uint32_t best = 0;
+  for (uint32_t i = 0; i < cands.size(); ++i) {
+    apgx_route_options o = opt;
+    o.tile_w = cands[i].tile_w;
+    o.tile_h = cands[i].tile_h;
+    o.frontier_delta = cands[i].frontier_delta;
+    o.max_sweeps = cands[i].max_sweeps;
+    apgx_search* s = nullptr;
+    const int rc = apgx_route_nets(lat, picked.data(), sample, pins, pin_count, &o, &s);
+    if (rc != APGX_OK) return rc;
+    cands[i].device_ms = s->stats.device_ms;
+    apgx_search_release(s);
+    if (cands[i].device_ms < cands[best].device_ms) best = i;
+  }
+  if (!cands.empty()) {
+    tuned->tile_w = cands[best].tile_w;
+    tuned->tile_h = cands[best].tile_h;
+    tuned->frontier_delta = cands[best].frontier_delta;
+    tuned->max_sweeps = cands[best].max_sweeps;
+  }
+  if (report) {
+    std::memset(report, 0, sizeof(*report));
+    report->sample_nets = sample;
+    report->candidate_count = static_cast<uint32_t>(cands.size());
+    report->best = best;
+    for (uint32_t i = 0; i < cands.size(); ++i) report->candidates[i] = cands[i];
+  }
+  ClearError();
+  return APGX_OK;
 }

@@ -1,12 +1,18 @@
 #!/usr/bin/env python3
 """Benchmark of the batched lattice search on one MI355X.
 
-One step routes the four_layer_1024 workload (4 x 1024 x 1024 lattice, uint16
-cell cost U[1,8], 15% rectangular keepouts, vias cost 20, 1024 single-pair
-queries) through the C-ABI.  The cost plane is resident in HBM before timing;
-`value` is queries per second over the device envelope of each step (hipEvent
-from the seed kernel to the last phase kernel).  The host-inclusive rate
-(query upload, result readback) is reported beside it as wall_queries_per_s.
+Default workload, four_layer_1024: a 4 x 1024 x 1024 lattice with uint16 cell
+costs U[1,8], 15% rectangular keepouts and vias costing 20, searched by 1024
+single-pair queries through the C-ABI.  The cost plane is resident in HBM
+before timing.  `value` is queries per second over the device envelope of each
+step (hipEvent from the seed kernel to the final path walk).  The rate that
+includes the host side (query upload, result readback) is reported beside it
+as wall_queries_per_s.
+
+--workload eight_layer_2048: an 8 x 2048 x 2048 lattice with the same model,
+4096 nets of 2-8 pins and log-uniform windows of side 32-2048.  Tile shape and
+schedule come from the autotuner on a 256-net sample (reported, not timed), and
+round chunks replay as captured hipGraphs.  The value is nets/s.
 
 Prints one JSON line.  --gpus above 1 is rejected: this build runs on one GPU.
 """
@@ -26,10 +32,9 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 import apgar_amd as apgx  # noqa: E402
-from tests.lattice_cases import FOUR_LAYER_1024, make_lattice  # noqa: E402
+from tests.lattice_cases import EIGHT_LAYER_2048, FOUR_LAYER_1024, make_lattice  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_FILE = ROOT / "profiles" / "round_kernel_four_layer_1024_traffic.txt"
 
 
 def parse_args():
@@ -37,34 +42,40 @@ def parse_args():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--queries", type=int, default=FOUR_LAYER_1024["queries"])
-    p.add_argument("--cpu-sample", type=int, default=32, help="queries timed on the CPU oracle")
+    p.add_argument("--workload", choices=["four_layer_1024", "eight_layer_2048"],
+                   default="four_layer_1024")
+    p.add_argument("--queries", type=int, default=0, help="queries / nets (0: the workload's count)")
+    p.add_argument("--cpu-sample", type=int, default=0, help="queries timed on the CPU oracle (0: auto)")
     p.add_argument("--cpu-threads", type=int, default=0)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--tile", type=str, default="0x0")
     p.add_argument("--delta", type=int, default=0, help="tile scheduling bucket width (0: default)")
     p.add_argument("--max-sweeps", type=int, default=0)
     p.add_argument("--no-goal-bound", action="store_true", help="order tiles by distance only")
+    p.add_argument("--graph", type=int, default=-1, help="1/0: replay round chunks as hipGraphs "
+                   "(default: on for eight_layer_2048)")
+    p.add_argument("--no-tune", action="store_true", help="eight_layer_2048: skip the autotuner")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run a device-to-device copy of MIB MiB (HBM counter scale reference)")
     return p.parse_args()
 
 
-def read_traffic():
+def read_traffic(workload):
     """HBM bytes per round-kernel launch from the committed rocprofv3 PMC summary."""
-    if not TRAFFIC_FILE.exists():
+    f = ROOT / "profiles" / f"round_kernel_{workload}_traffic.txt"
+    if not f.exists():
         return None
-    for line in TRAFFIC_FILE.read_text().splitlines():
+    for line in f.read_text().splitlines():
         if line.startswith("traffic_bytes_per_launch:"):
             return float(line.split(":", 1)[1])
     return None
 
 
-def cpu_baseline(cost, pairs, sample, threads):
-    """CPU A* oracle (restated reference algorithm) on a bounded sample."""
+def cpu_pairs_baseline(cost, via, pairs, sample, threads):
+    """CPU A* oracle (restated reference algorithm) on a bounded sample of pairs."""
     from oracle import py as oracle_py
 
-    o = oracle_py.Dense(cost, FOUR_LAYER_1024["via"])
+    o = oracle_py.Dense(cost, via)
     idx = list(range(min(sample, len(pairs))))
 
     def one(i):
@@ -78,6 +89,27 @@ def cpu_baseline(cost, pairs, sample, threads):
     return {"value": len(idx) / dt, "unit": "queries/s", "cores": threads, "kind": "port",
             "sample": f"first {len(idx)} of the {len(pairs)} four_layer_1024 queries, "
                       f"one A* per thread, {threads} threads, {dt:.1f} s"}
+
+
+def cpu_nets_baseline(cost, via, nets, pins, sample, threads):
+    """CPU oracle multi-pin routing (repeated Dijkstra from the tree) on an evenly
+    strided sample of nets, one net per thread."""
+    from oracle import py as oracle_py
+
+    idx = [i * len(nets) // sample for i in range(sample)]
+
+    def one(i):
+        n = nets[i]
+        win = (n.window.x0, n.window.y0, n.window.width, n.window.height)
+        return oracle_py.Dense(cost, via, win).route_net(pins[n.pin_offset:n.pin_offset + n.pin_count])
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as pool:
+        list(pool.map(one, idx))
+    dt = time.perf_counter() - t0
+    return {"value": len(idx) / dt, "unit": "nets/s", "cores": threads, "kind": "port",
+            "sample": f"{len(idx)} of the {len(nets)} eight_layer_2048 nets (every "
+                      f"{len(nets) // sample}th), one net per thread, {threads} threads, {dt:.1f} s"}
 
 
 def copy_calibration(mib):
@@ -95,6 +127,10 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": n, "launches": 3}))
 
 
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
 def main():
     args = parse_args()
     if args.copy_calibration:
@@ -105,23 +141,42 @@ def main():
               "independent and would shard by query list, which this build does not launch.",
               file=sys.stderr)
         sys.exit(2)
-    spec = FOUR_LAYER_1024
-    tile = tuple(int(v) for v in args.tile.split("x"))
+    big = args.workload == "eight_layer_2048"
+    spec = EIGHT_LAYER_2048 if big else FOUR_LAYER_1024
     cost = make_lattice(apgx, spec)
-    pairs = apgx.generate_pairs(cost, args.queries, 202)
     lat = apgx.Lattice(cost, via_cost=spec["via"], cost_bytes=2)
-    print(f"[bench] lattice {cost.shape} uploaded; {args.queries} queries", file=sys.stderr, flush=True)
+    opts = dict(tile=tuple(int(v) for v in args.tile.split("x")), frontier_delta=args.delta,
+                max_sweeps=args.max_sweeps, goal_bound=not args.no_goal_bound,
+                use_graph=(1 if big else 0) if args.graph < 0 else args.graph)
+    tune = None
+    if big:
+        count = args.queries or spec["nets"]
+        nets, pins = apgx.generate_nets(cost, count, spec["min_pins"], spec["max_pins"],
+                                        spec["min_side"], spec["max_side"], spec["seed"])
+        if not args.no_tune and args.tile == "0x0" and args.delta == 0 and args.max_sweeps == 0:
+            t0 = time.perf_counter()
+            opts, report = lat.autotune(nets, pins, sample_nets=256, **opts)
+            tune = {"chosen": {"tile": "%dx%d" % opts["tile"], "frontier_delta": opts["frontier_delta"],
+                               "max_sweeps": opts["max_sweeps"]},
+                    "sample_nets": report["sample_nets"], "candidates": len(report["candidates"]),
+                    "tune_s": time.perf_counter() - t0}
+            log(f"[bench] autotune: {tune}")
+        unit, what = "nets/s", "nets"
 
-    def step(profile=False):
-        routes, stats = lat.route_pairs(pairs, tile=tile, profile_rounds=1 if profile else 0,
-                                        frontier_delta=args.delta, max_sweeps=args.max_sweeps,
-                                        goal_bound=not args.no_goal_bound)
-        return routes, stats
+        def step(profile=False):
+            return lat.route(nets, pins, profile_rounds=1 if profile else 0, **opts)
+    else:
+        count = args.queries or spec["queries"]
+        pairs = apgx.generate_pairs(cost, count, 202)
+        unit, what = "queries/s", "queries"
+
+        def step(profile=False):
+            return lat.route_pairs(pairs, profile_rounds=1 if profile else 0, **opts)
+    log(f"[bench] {args.workload}: lattice {cost.shape} uploaded; {count} {what}")
 
     for i in range(args.warmup):
         _, st = step()
-        print(f"[bench] warmup {i}: device {st.device_ms:.2f} ms rounds {st.rounds}",
-              file=sys.stderr, flush=True)
+        log(f"[bench] warmup {i}: device {st.device_ms:.2f} ms rounds {st.rounds}")
     apgx.synchronize()
     t0 = time.perf_counter()
     dev_ms, all_stats = [], []
@@ -131,24 +186,25 @@ def main():
         dev_ms.append(st.device_ms)
         all_stats.append(st)
         reached = sum(r.status == apgx.NET_REACHED for r in routes)
-        print(f"[bench] step {i}: device {st.device_ms:.2f} ms wall {st.wall_ms:.2f} ms "
-              f"rounds {st.rounds} tiles {st.tile_visits}", file=sys.stderr, flush=True)
+        log(f"[bench] step {i}: device {st.device_ms:.2f} ms wall {st.wall_ms:.2f} ms "
+            f"rounds {st.rounds} tiles {st.tile_visits}")
     apgx.synchronize()
     wall = (time.perf_counter() - t0) / args.steps
     ms = float(np.mean(dev_ms))
 
-    # Roofline of the dominant kernel (the round kernel), timed per launch.
+    # Roofline of the dominant kernel (the round kernel), each launch timed with
+    # hipEvents on the search stream (profiled step, direct launches).
     _, prof = step(profile=True)
     launches = max(prof.rounds, 1)
     bytes_per_launch = prof.algorithmic_bytes / launches
     avg_launch_s = prof.round_kernel_ms / 1e3 / launches
     achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
-    traffic = read_traffic()
     st = all_stats[-1]
+    L = cost.shape[0]
     line = {
         "metric": "queries/s & % HBM roofline",
-        "value": args.queries / (ms / 1e3),
-        "unit": "queries/s",
+        "value": count / (ms / 1e3),
+        "unit": unit,
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -159,14 +215,16 @@ def main():
         "dtype": "u32",
         "data": "synthetic",
         "config": {
-            "workload": "four_layer_1024",
-            "lattice": "4x1024x1024, uint16 cost U[1,8], 15% keepouts (sides 4-64), vias 20",
-            "queries": args.queries,
+            "workload": args.workload,
+            "lattice": ("8x2048x2048" if big else "4x1024x1024")
+                       + ", uint16 cost U[1,8], 15% keepouts (sides 4-64), vias 20",
+            what: count,
             "tile": f"{st.tile_w}x{st.tile_h}",
+            "graph": bool(opts["use_graph"]),
             "parallelism": "single GPU",
         },
         "wall_ms_per_step": wall * 1e3,
-        "wall_queries_per_s": args.queries / wall,
+        f"wall_{what}_per_s": count / wall,
         "reached": reached,
         "rounds": st.rounds,
         "tile_visits": st.tile_visits,
@@ -178,17 +236,24 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "kernel": (f"RegRoundKernel<{cost.shape[0]}>" if (st.tile_w, st.tile_h) == (64, 64)
+            "traffic": read_traffic(args.workload),
+            "kernel": (f"RegRoundKernel<{L}>" if (st.tile_w, st.tile_h) == (64, 64)
                        else f"RoundKernel<{st.tile_w},{st.tile_h}>"),
             "launches": launches,
             "avg_launch_us": avg_launch_s * 1e6,
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
     }
+    if tune:
+        line["autotune"] = tune
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(cost, pairs, args.cpu_sample, threads)
+        if big:
+            line["cpu_baseline"] = cpu_nets_baseline(cost, spec["via"], nets, pins,
+                                                     args.cpu_sample or 16, threads)
+        else:
+            line["cpu_baseline"] = cpu_pairs_baseline(cost, spec["via"], pairs,
+                                                      args.cpu_sample or 32, threads)
     print(json.dumps(line))
     lat.close()
 

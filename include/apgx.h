@@ -210,6 +210,36 @@ int apgx_search_readback(apgx_search* search, apgx_net_result* results,
 int apgx_search_stats(const apgx_search* search, apgx_route_stats* out);
 void apgx_search_release(apgx_search* search);
 
+/* Autotuner.  Times a fixed set of candidate tile shapes and tile schedules
+ * (frontier_delta, max_sweeps) on an evenly strided sample of the nets and
+ * writes the fastest candidate into *tuned; every other field is copied from
+ * *base (NULL: defaults).  The set: tiles 64x64 (register-resident, when the
+ * lattice allows), 64x32 and 32x32 (LDS); delta 16, 64 and 256 x mean cell
+ * cost and unordered; max_sweeps 64 and 16 on 64x64 tiles.  Results do not
+ * depend on the choice (labels are the unique fixed point); only speed does.
+ * No reference counterpart: the reference has one fixed launch shape. */
+#define APGX_TUNE_MAX_CANDIDATES 24u
+typedef struct apgx_tune_candidate {
+  uint32_t tile_w;
+  uint32_t tile_h;
+  uint32_t frontier_delta;
+  uint32_t max_sweeps;
+  double device_ms;            /* search time of the sample */
+} apgx_tune_candidate;
+
+typedef struct apgx_tune_report {
+  uint32_t sample_nets;
+  uint32_t candidate_count;
+  uint32_t best;               /* index of the chosen candidate */
+  uint32_t reserved;
+  apgx_tune_candidate candidates[APGX_TUNE_MAX_CANDIDATES];
+} apgx_tune_report;
+
+int apgx_autotune(apgx_lattice* lattice, const apgx_net_query* nets, uint32_t net_count,
+                  const uint32_t* pins, uint64_t pin_count, uint32_t sample_nets,
+                  const apgx_route_options* base, apgx_route_options* tuned,
+                  apgx_tune_report* report);
+
 /* ------------------------------------------------------------------------ */
 /* V1: reference DeviceCompiledBoardV1 / DeviceCandidateBatch records        */
 /* ------------------------------------------------------------------------ */

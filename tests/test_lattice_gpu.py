@@ -7,7 +7,7 @@ path cost, canonical-predecessor condition) for every query.
 import numpy as np
 import pytest
 
-from tests.lattice_cases import FOUR_LAYER_1024, SINGLE_LAYER_256, make_lattice
+from tests.lattice_cases import EIGHT_LAYER_2048, FOUR_LAYER_1024, SINGLE_LAYER_256, make_lattice
 from tests.test_oracle_dense import check_canonical, random_lattice
 
 pytestmark = pytest.mark.gpu
@@ -87,8 +87,10 @@ def test_windows_and_multi_pin_nets(apgx, oracle, gpu, seed):
         windows.append((x0, y0, w, h))
     nets, pins = apgx.nets_from_lists(pin_lists, windows)
     lat = apgx.Lattice(cost, via_cost=via)
-    routes, _ = lat.route(nets, pins, tile=[(0, 0), (32, 16), (32, 32), (64, 32)][seed],
-                          frontier_delta=[0, 0xFFFFFFFF, 7, 0][seed], goal_bound=seed != 3)
+    routes, stats = lat.route(nets, pins, tile=[(0, 0), (32, 16), (32, 32), (64, 32)][seed],
+                              frontier_delta=[0, 0xFFFFFFFF, 7, 0][seed], goal_bound=seed != 3,
+                              use_graph=seed % 2, rounds_per_sync=[8, 2, 8, 3][seed])
+    assert stats.graph_launches > 0 or seed % 2 == 0
     for pl, win, r in zip(pin_lists, windows, routes):
         rc, conns = oracle.Dense(cost, via, win).route_net(pl)
         if rc == 1:
@@ -194,4 +196,88 @@ def test_four_layer_1024_full_size(apgx, oracle, gpu):
             continue
         assert r.cost == dist[t]
         assert np.array_equal(r.connections[0][2], o.canonical_path(dist, t))
+    lat.close()
+
+
+def check_net_structure(flat_cost, W, plane, via, pins, window, route):
+    """Every connection of a reached net: starts at its pin, ends on the tree
+    built so far, steps to lattice neighbours inside the window through free
+    cells, and its cost equals its distance."""
+    x0, y0, w, h = window
+    tree = {int(pins[0])}
+    total = 0
+    for pin, d, cells in route.connections:
+        cells = cells.astype(np.int64)
+        assert cells[0] == pins[pin] and int(cells[-1]) in tree
+        steps = np.abs(np.diff(cells))
+        assert np.isin(steps, [1, W, plane]).all()
+        xs, ys = cells % W, (cells // W) % (plane // W)
+        assert ((xs >= x0) & (xs < x0 + w) & (ys >= y0) & (ys < y0 + h)).all()
+        assert (flat_cost[cells] > 0).all()
+        cw = np.where(steps == plane, via, flat_cost[cells[:-1]].astype(np.int64))
+        assert int(cw.sum()) == d
+        total += d
+        tree.update(int(c) for c in cells)
+    assert total == route.cost
+
+
+def test_eight_layer_2048_full_size(apgx, oracle, gpu):
+    spec = EIGHT_LAYER_2048
+    _say("[case] eight_layer_2048: 4096 nets of 2-8 pins, log-uniform windows, graph launches")
+    cost = make_lattice(apgx, spec)
+    nets, pins = apgx.generate_nets(cost, spec["nets"], spec["min_pins"], spec["max_pins"],
+                                    spec["min_side"], spec["max_side"], spec["seed"])
+    lat = apgx.Lattice(cost, via_cost=spec["via"], cost_bytes=2)
+    routes, stats = lat.route(nets, pins, use_graph=1)
+    _say(f"[info] device_ms={stats.device_ms:.1f} rounds={stats.rounds} tiles={stats.tile_visits} "
+         f"graph_launches={stats.graph_launches}")
+    assert stats.graph_launches > 0
+    L, H, W = cost.shape
+    plane = H * W
+    flat = cost.reshape(-1)
+    reached = 0
+    for i, r in enumerate(routes):
+        n = nets[i]
+        assert r.status in (apgx.NET_REACHED, apgx.NET_DISCONNECTED)
+        if r.status == apgx.NET_REACHED:
+            reached += 1
+            assert len(r.connections) == n.pin_count - 1
+            p = pins[n.pin_offset:n.pin_offset + n.pin_count]
+            win = (n.window.x0, n.window.y0, n.window.width, n.window.height)
+            check_net_structure(flat, W, plane, spec["via"], p, win, r)
+    assert reached > 0.9 * len(routes)
+    # Oracle on nets spread over the window-size distribution (up to the median).
+    order = sorted(range(len(routes)), key=lambda i: nets[i].window.width)
+    for q in range(6):
+        i = order[q * len(order) // 12]
+        n = nets[i]
+        win = (n.window.x0, n.window.y0, n.window.width, n.window.height)
+        p = pins[n.pin_offset:n.pin_offset + n.pin_count]
+        rc, conns = oracle.Dense(cost, spec["via"], win).route_net(p)
+        r = routes[i]
+        if rc == 1:
+            assert r.status == apgx.NET_DISCONNECTED
+            continue
+        assert r.status == apgx.NET_REACHED and len(r.connections) == len(conns)
+        for (gp, gd, gc), (op, od, oc) in zip(r.connections, conns):
+            assert (gp, gd) == (op, od) and np.array_equal(gc, oc)
+    lat.close()
+
+
+def test_autotune_picks_a_measured_candidate(apgx, gpu):
+    _say("[case] autotune on 96 nets of a 2x300x300 lattice")
+    rng = np.random.default_rng(9)
+    cost = random_lattice(rng, 2, 300, 300, blocked=0.1, cmax=8)
+    nets, pins = apgx.generate_nets(cost, 96, 2, 6, 32, 300, 11)
+    lat = apgx.Lattice(cost, via_cost=10)
+    opts, report = lat.autotune(nets, pins, sample_nets=32)
+    assert report["sample_nets"] == 32 and len(report["candidates"]) >= 8
+    best = report["candidates"][report["best"]]
+    assert all(best[4] <= c[4] for c in report["candidates"])
+    assert opts["tile"] == best[:2]
+    a, _ = lat.route(nets, pins, **opts)
+    b, _ = lat.route(nets, pins)
+    for x, y in zip(a, b):
+        assert (x.status, x.cost) == (y.status, y.cost)
+        assert all(np.array_equal(p[2], q[2]) for p, q in zip(x.connections, y.connections))
     lat.close()
