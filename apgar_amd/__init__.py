@@ -90,6 +90,27 @@ class TuneReport(C.Structure):
                 ("reserved", C.c_uint32), ("candidates", TuneCandidate * 24)]
 
 
+class NegotiationPolicy(C.Structure):
+    _fields_ = [("maximum_passes", C.c_uint32), ("batch_nets", C.c_uint32),
+                ("initial_present_cost", C.c_uint64), ("present_cost_increment", C.c_uint64),
+                ("historical_cost_increment", C.c_uint64),
+                ("maximum_congestion_cost_value", C.c_uint64)]
+
+
+class NegotiationSummary(C.Structure):
+    _fields_ = [("termination", C.c_uint32), ("completed_passes", C.c_uint32),
+                ("routed_nets", C.c_uint32), ("batches", C.c_uint32),
+                ("total_overuse_units", C.c_uint64), ("tile_visits", C.c_uint64),
+                ("search_device_ms", C.c_double), ("wall_ms", C.c_double)]
+
+
+class BoardGenParams(C.Structure):
+    _fields_ = [(k, C.c_uint32) for k in (
+        "width", "height", "layers", "cost_min", "cost_max", "keepout_ppm", "keepout_min_side",
+        "keepout_max_side", "nets", "min_pins", "max_pins", "bga_fields", "bga_pads", "bga_pitch",
+        "pad_pin_ppm", "scatter_radius", "window_margin", "reserved")] + [("seed", C.c_uint64)]
+
+
 class LatticeGenParams(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("layers", C.c_uint32),
                 ("cost_min", C.c_uint32), ("cost_max", C.c_uint32), ("blocked_ppm", C.c_uint32),
@@ -114,6 +135,18 @@ _SIGNATURES = {
                                        _P(C.c_uint32), C.c_uint64]),
     "apgx_search_stats": (C.c_int, [C.c_void_p, _P(RouteStats)]),
     "apgx_search_release": (None, [C.c_void_p]),
+    "apgx_generate_board": (C.c_int, [_P(BoardGenParams), _P(C.c_uint32), _P(NetQuery), _P(C.c_uint32),
+                                      _P(C.c_uint64)]),
+    "apgx_negotiation_policy_defaults": (None, [_P(NegotiationPolicy)]),
+    "apgx_negotiate": (C.c_int, [C.c_void_p, _P(C.c_uint8), _P(NetQuery), C.c_uint32, _P(C.c_uint32),
+                                 C.c_uint64, _P(NegotiationPolicy), _P(C.c_uint32), _P(RouteOptions),
+                                 _P(C.c_void_p)]),
+    "apgx_negotiation_get_summary": (C.c_int, [C.c_void_p, _P(NegotiationSummary), _P(C.c_uint64),
+                                               C.c_uint32]),
+    "apgx_negotiation_planes": (C.c_int, [C.c_void_p, _P(C.c_uint32), _P(C.c_uint32)]),
+    "apgx_negotiation_routes": (C.c_int, [C.c_void_p, _P(C.c_uint32), _P(C.c_uint32), C.c_uint64,
+                                          _P(C.c_uint64)]),
+    "apgx_negotiation_release": (None, [C.c_void_p]),
     "apgx_autotune": (C.c_int, [C.c_void_p, _P(NetQuery), C.c_uint32, _P(C.c_uint32), C.c_uint64,
                                 C.c_uint32, _P(RouteOptions), _P(RouteOptions), _P(TuneReport)]),
     "apgx_generate_lattice": (C.c_int, [_P(LatticeGenParams), _P(C.c_uint32)]),
@@ -195,6 +228,17 @@ def generate_nets(cost: np.ndarray, count: int, min_pins: int, max_pins: int, mi
     check(lib().apgx_generate_nets(u32p(flat), W, H, L, count, min_pins, max_pins, min_side,
                                    max_side, seed, nets, u32p(pins), C.byref(total)))
     return nets, pins[: total.value].copy()
+
+
+def generate_board(**params):
+    """Allocator board (apgx_generate_board): returns (cost (L, H, W) uint32, nets, pins)."""
+    p = BoardGenParams(**params)
+    cost = np.empty(p.width * p.height * p.layers, dtype=np.uint32)
+    nets = (NetQuery * p.nets)()
+    pins = np.empty(p.nets * p.max_pins, dtype=np.uint32)
+    total = C.c_uint64(0)
+    check(lib().apgx_generate_board(C.byref(p), u32p(cost), nets, u32p(pins), C.byref(total)))
+    return cost.reshape(p.layers, p.height, p.width), nets, pins[:total.value].copy()
 
 
 # --------------------------------------------------------------------- search
@@ -292,6 +336,65 @@ class Lattice:
         cands = [(c.tile_w, c.tile_h, c.frontier_delta, c.max_sweeps, c.device_ms)
                  for c in rep.candidates[:rep.candidate_count]]
         return opts, {"sample_nets": rep.sample_nets, "best": rep.best, "candidates": cands}
+
+    def negotiate(self, nets, pins: np.ndarray, *, capacity=None, initial_history=None,
+                  search_options=None, **policy):
+        """Negotiated-congestion allocator (apgx_negotiate).  policy keywords:
+        maximum_passes, batch_nets, initial_present_cost, present_cost_increment,
+        historical_cost_increment, maximum_congestion_cost_value.  Returns a dict
+        like the oracle's: termination, passes, routed, overuse, overuse_per_pass,
+        history / usage planes (L, H, W) and per-net tree cells (None: unrouted)."""
+        pins = np.ascontiguousarray(pins, dtype=np.uint32)
+        pol = NegotiationPolicy()
+        lib().apgx_negotiation_policy_defaults(C.byref(pol))
+        for k, v in policy.items():
+            setattr(pol, k, v)
+        L, H, W = self.shape
+        n = L * H * W
+        cap = None if capacity is None else np.ascontiguousarray(capacity, dtype=np.uint8).reshape(-1)
+        init = (None if initial_history is None
+                else np.ascontiguousarray(initial_history, dtype=np.uint32).reshape(-1))
+        opts = RouteOptions()
+        for k, v in (search_options or {}).items():
+            if k == "tile":
+                opts.tile_w, opts.tile_h = v
+            else:
+                setattr(opts, k, v)
+        handle = C.c_void_p()
+        check(lib().apgx_negotiate(self.handle,
+                                   cap.ctypes.data_as(_P(C.c_uint8)) if cap is not None else None,
+                                   nets, len(nets), u32p(pins), pins.size, C.byref(pol),
+                                   u32p(init) if init is not None else None, C.byref(opts),
+                                   C.byref(handle)))
+        try:
+            summ = NegotiationSummary()
+            per_pass = np.zeros(max(pol.maximum_passes, 1), dtype=np.uint64)
+            check(lib().apgx_negotiation_get_summary(handle, C.byref(summ),
+                                                     per_pass.ctypes.data_as(_P(C.c_uint64)),
+                                                     per_pass.size))
+            hist = np.empty(n, dtype=np.uint32)
+            usage = np.empty(n, dtype=np.uint32)
+            check(lib().apgx_negotiation_planes(handle, u32p(hist), u32p(usage)))
+            total = C.c_uint64(0)
+            rlen = np.zeros(max(len(nets), 1), dtype=np.uint32)
+            check(lib().apgx_negotiation_routes(handle, u32p(rlen), None, 0, C.byref(total)))
+            cells = np.empty(max(total.value, 1), dtype=np.uint32)
+            check(lib().apgx_negotiation_routes(handle, u32p(rlen), u32p(cells), cells.size,
+                                                C.byref(total)))
+        finally:
+            lib().apgx_negotiation_release(handle)
+        routes, off = [], 0
+        for i in range(len(nets)):
+            k = int(rlen[i])
+            routes.append(cells[off:off + k].copy() if k else None)
+            off += k
+        return {"termination": summ.termination, "passes": summ.completed_passes,
+                "routed": summ.routed_nets, "overuse": int(summ.total_overuse_units),
+                "overuse_per_pass": [int(v) for v in per_pass[:summ.completed_passes]],
+                "batches": summ.batches, "tile_visits": summ.tile_visits,
+                "search_device_ms": summ.search_device_ms, "wall_ms": summ.wall_ms,
+                "history": hist.reshape(L, H, W), "occupancy": usage.reshape(L, H, W),
+                "routes": routes}
 
     def route_pairs(self, pairs: np.ndarray, window=None, **kw):
         """Single-pair queries (pairs: (n, 2) source/target cells)."""

@@ -183,3 +183,112 @@ extern "C" int apgx_generate_nets(const uint32_t* cost, uint32_t width, uint32_t
   apgx::ClearError();
   return APGX_OK;
 }
+
+extern "C" int apgx_generate_board(const apgx_board_gen_params* p, uint32_t* cost,
+                                   apgx_net_query* nets_out, uint32_t* pins_out, uint64_t* pin_total) {
+  if (p == nullptr || cost == nullptr || nets_out == nullptr || pins_out == nullptr ||
+      pin_total == nullptr || p->bga_fields == 0 || p->bga_pads == 0 || p->bga_pitch == 0 ||
+      p->min_pins < 2 || p->min_pins > p->max_pins || p->max_pins > APGX_MAX_NET_PINS ||
+      static_cast<uint64_t>(p->bga_pads) * p->bga_pitch > std::min(p->width, p->height) ||
+      p->pad_pin_ppm > 1000000) {
+    return apgx::Fail(APGX_E_INTERNAL_INVARIANT, "apgx_generate_board: invalid parameters");
+  }
+  apgx_lattice_gen_params lp{p->width, p->height, p->layers, p->cost_min, p->cost_max, 0,
+                             p->keepout_ppm, p->keepout_min_side, p->keepout_max_side, 0, p->seed};
+  if (int rc = apgx_generate_lattice(&lp, cost); rc != APGX_OK) return rc;
+  Rng rng(p->seed ^ 0xB6A0B6A0B6A0B6A0ull);
+  const uint64_t plane = static_cast<uint64_t>(p->width) * p->height;
+  const uint32_t side = p->bga_pads * p->bga_pitch;
+  // Field placement: uniform positions, rejecting overlaps (bounded attempts).
+  std::vector<uint32_t> fx, fy;
+  for (uint32_t f = 0, tries = 0; f < p->bga_fields; ++tries) {
+    if (tries > 100000) return apgx::Fail(APGX_E_INTERNAL_INVARIANT, "apgx_generate_board: fields do not fit");
+    const uint32_t x = rng.Range(0, p->width - side), y = rng.Range(0, p->height - side);
+    bool overlap = false;
+    for (uint32_t g = 0; g < f; ++g)
+      overlap |= x < fx[g] + side && fx[g] < x + side && y < fy[g] + side && fy[g] < y + side;
+    if (overlap) continue;
+    fx.push_back(x);
+    fy.push_back(y);
+    ++f;
+  }
+  const uint32_t pads = p->bga_pads * p->bga_pads;
+  auto pad_cell = [&](uint32_t f, uint32_t i) {
+    const uint32_t x = fx[f] + (i % p->bga_pads) * p->bga_pitch + p->bga_pitch / 2;
+    const uint32_t y = fy[f] + (i / p->bga_pads) * p->bga_pitch + p->bga_pitch / 2;
+    return static_cast<uint64_t>(y) * p->width + x;  // layer 0
+  };
+  for (uint32_t f = 0; f < p->bga_fields; ++f)
+    for (uint32_t i = 0; i < pads; ++i) {
+      uint32_t& c = cost[pad_cell(f, i)];
+      if (c == 0) c = rng.Range(p->cost_min, p->cost_max);
+    }
+  // Pad cells are reserved for pad pins: scattered pins never land on one.
+  std::vector<uint8_t> used(plane * p->layers, 0);
+  for (uint32_t f = 0; f < p->bga_fields; ++f)
+    for (uint32_t i = 0; i < pads; ++i) used[pad_cell(f, i)] = 1;
+  std::vector<uint32_t> free_pads(static_cast<size_t>(p->bga_fields) * pads);
+  std::vector<uint32_t> free_count(p->bga_fields, pads);
+  for (uint32_t f = 0; f < p->bga_fields; ++f)
+    for (uint32_t i = 0; i < pads; ++i) free_pads[static_cast<size_t>(f) * pads + i] = i;
+  // Takes a random unused pad of field f (or of the next field with one left).
+  auto take_pad = [&](uint32_t f, uint64_t* out) {
+    for (uint32_t k = 0; k < p->bga_fields; ++k) {
+      const uint32_t g = (f + k) % p->bga_fields;
+      if (free_count[g] == 0) continue;
+      uint32_t* list = &free_pads[static_cast<size_t>(g) * pads];
+      const uint32_t j = rng.Range(0, free_count[g] - 1);
+      const uint32_t pad = list[j];
+      list[j] = list[--free_count[g]];
+      *out = pad_cell(g, pad);
+      return true;
+    }
+    return false;
+  };
+  uint64_t w = 0;
+  for (uint32_t n = 0; n < p->nets; ++n) {
+    const uint32_t k = rng.Range(p->min_pins, p->max_pins);
+    const uint32_t f = rng.Range(0, p->bga_fields - 1);
+    const uint64_t base = w;
+    uint64_t c = 0;
+    if (!take_pad(f, &c)) return apgx::Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_generate_board: pads exhausted");
+    used[c] = 1;
+    pins_out[w++] = static_cast<uint32_t>(c);
+    const uint32_t cx = fx[f] + side / 2, cy = fy[f] + side / 2;
+    const uint32_t r = p->scatter_radius ? p->scatter_radius : std::max(p->width, p->height);
+    const uint32_t x0 = cx > r ? cx - r : 0, x1 = std::min(p->width - 1, cx + r);
+    const uint32_t y0 = cy > r ? cy - r : 0, y1 = std::min(p->height - 1, cy + r);
+    for (uint32_t i = 1; i < k; ++i) {
+      if (rng.Ppm(p->pad_pin_ppm)) {
+        const uint32_t g = rng.Range(0, 1) ? f : rng.Range(0, p->bga_fields - 1);
+        if (!take_pad(g, &c)) return apgx::Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_generate_board: pads exhausted");
+      } else {
+        uint32_t tries = 0;
+        do {
+          if (++tries > 1000000) return apgx::Fail(APGX_E_INTERNAL_INVARIANT, "apgx_generate_board: no free cell");
+          const uint32_t l = rng.Range(0, p->layers - 1);
+          c = l * plane + static_cast<uint64_t>(rng.Range(y0, y1)) * p->width + rng.Range(x0, x1);
+        } while (cost[c] == 0 || used[c]);
+      }
+      used[c] = 1;
+      pins_out[w++] = static_cast<uint32_t>(c);
+    }
+    uint32_t bx0 = p->width, by0 = p->height, bx1 = 0, by1 = 0;
+    for (uint64_t j = base; j < w; ++j) {
+      const uint32_t x = static_cast<uint32_t>(pins_out[j] % p->width);
+      const uint32_t y = static_cast<uint32_t>((pins_out[j] / p->width) % p->height);
+      bx0 = std::min(bx0, x); bx1 = std::max(bx1, x);
+      by0 = std::min(by0, y); by1 = std::max(by1, y);
+    }
+    const uint32_t m = p->window_margin;
+    const uint32_t wx0 = bx0 > m ? bx0 - m : 0, wy0 = by0 > m ? by0 - m : 0;
+    const uint32_t wx1 = std::min(p->width - 1, bx1 + m), wy1 = std::min(p->height - 1, by1 + m);
+    nets_out[n].net_id = n + 1;
+    nets_out[n].pin_offset = static_cast<uint32_t>(base);
+    nets_out[n].pin_count = k;
+    nets_out[n].window = apgx_window{wx0, wy0, wx1 - wx0 + 1, wy1 - wy0 + 1};
+  }
+  *pin_total = w;
+  apgx::ClearError();
+  return APGX_OK;
+}

@@ -40,6 +40,7 @@
 
 #include "apgx.h"
 #include "common.h"
+#include "lattice_internal.h"
 
 namespace apgx {
 namespace {
@@ -1007,25 +1008,6 @@ bool KernelsForShape(TileShape s, Kernels* k) {
 // Host side
 // ---------------------------------------------------------------------------
 
-struct apgx_lattice {
-  apgx_lattice_desc desc{};
-  void* d_cost = nullptr;
-  int device = 0;
-  uint32_t min_edge = 0;      // smallest edge weight (free cell cost or via)
-  uint32_t min_cell_cost = 0; // smallest free cell cost (0 when every cell is blocked)
-  uint32_t max_cell_cost = 0; // largest free cell cost
-  uint64_t free_cells = 0;
-  double mean_cost = 1.0;     // mean free-cell cost
-  std::mutex cache_mutex;
-  void* cached_labels = nullptr;  // reusable label pool (returned by searches)
-  size_t cached_label_bytes = 0;
-  std::vector<uint8_t> host_costs;  // host copy for request preflight
-  uint32_t HostCost(uint64_t cell) const {
-    return desc.cost_bytes == 2 ? reinterpret_cast<const uint16_t*>(host_costs.data())[cell]
-                                : reinterpret_cast<const uint32_t*>(host_costs.data())[cell];
-  }
-};
-
 struct apgx_search {
   apgx_lattice* lattice = nullptr;
   uint32_t net_count = 0;
@@ -1035,6 +1017,7 @@ struct apgx_search {
   void* d_labels = nullptr;
   size_t label_bytes = 0;
   void* d_block = nullptr;   // every other device buffer, one allocation
+  size_t block_bytes = 0;
   apgx::NetState* d_state = nullptr;
   apgx::DevConnection* d_conns = nullptr;
   uint32_t* d_cells = nullptr;
@@ -1100,6 +1083,7 @@ int FinishLattice(const apgx_lattice_desc* desc, const void* host_costs, apgx_la
 
 extern "C" int apgx_lattice_create(const apgx_lattice_desc* desc, const void* host_costs,
                                    apgx_lattice** out) {
+  // (apgx_lattice_release is declared in apgx.h and tolerates partial objects.)
   using namespace apgx;
   if (out == nullptr || host_costs == nullptr)
     return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_lattice_create: null argument");
@@ -1113,13 +1097,20 @@ extern "C" int apgx_lattice_create(const apgx_lattice_desc* desc, const void* ho
   }
   const size_t bytes = static_cast<size_t>(desc->width) * desc->height * desc->layers * desc->cost_bytes;
   hipError_t e = hipGetDevice(&lat->device);
+  int lds = 0, cus = 0;
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, lat->device);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, lat->device);
   if (e == hipSuccess) e = hipMalloc(&lat->d_cost, bytes);
   if (e == hipSuccess) e = hipMemcpy(lat->d_cost, host_costs, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&lat->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&lat->events[0]);
+  if (e == hipSuccess) e = hipEventCreate(&lat->events[1]);
   if (e != hipSuccess) {
-    if (lat->d_cost) (void)hipFree(lat->d_cost);
-    delete lat;
+    apgx_lattice_release(lat);
     return HipFail(e, "lattice upload");
   }
+  lat->lds_limit = static_cast<size_t>(lds);
+  lat->cu_count = static_cast<uint32_t>(cus);
   *out = lat;
   ClearError();
   return APGX_OK;
@@ -1141,6 +1132,10 @@ extern "C" void apgx_lattice_release(apgx_lattice* lat) {
   if (lat == nullptr) return;
   if (lat->d_cost) (void)hipFree(lat->d_cost);
   if (lat->cached_labels) (void)hipFree(lat->cached_labels);
+  if (lat->cached_block) (void)hipFree(lat->cached_block);
+  for (hipEvent_t ev : lat->events)
+    if (ev) (void)hipEventDestroy(ev);
+  if (lat->stream) (void)hipStreamDestroy(lat->stream);
   delete lat;
 }
 
@@ -1153,13 +1148,12 @@ namespace {
 
 // Register-resident 64 x 64 tiles: one wave per layer, up to eight layers,
 // cell costs up to kRegMaxCost.
-bool RegTileUsable(const apgx_lattice& lat, size_t lds_limit) {
-  return lat.desc.layers <= 8 && lat.max_cell_cost <= kRegMaxCost && RegTileLdsBytes(lat.desc.layers) <= lds_limit;
+bool RegTileUsable(uint32_t layers, uint32_t max_cost, size_t lds_limit) {
+  return layers <= 8 && max_cost <= kRegMaxCost && RegTileLdsBytes(layers) <= lds_limit;
 }
 
-TileShape DefaultShape(const apgx_lattice& lat, size_t lds_limit) {
-  const apgx_lattice_desc& d = lat.desc;
-  if (RegTileUsable(lat, lds_limit)) return TileShape{64, 64};
+TileShape DefaultShape(const apgx_lattice_desc& d, uint32_t max_cost, size_t lds_limit) {
+  if (RegTileUsable(d.layers, max_cost, lds_limit)) return TileShape{64, 64};
   const TileShape candidates[] = {{64, 64}, {64, 32}, {32, 32}, {32, 16}};
   // Prefer the largest tile that fits the register-resident cell budget and
   // leaves room for two workgroups per CU.
@@ -1169,6 +1163,29 @@ TileShape DefaultShape(const apgx_lattice& lat, size_t lds_limit) {
     return s;
   }
   return TileShape{32, 16};
+}
+
+// Occupancy of a kernel at a launch shape, queried once per shape.
+int OccupancyOf(const void* kernel, uint32_t block, size_t lds, int* out) {
+  struct Key {
+    const void* k;
+    uint32_t b;
+    size_t l;
+    int occ;
+  };
+  static std::mutex mu;
+  static std::vector<Key> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  for (const Key& c : cache)
+    if (c.k == kernel && c.b == block && c.l == lds) {
+      *out = c.occ;
+      return APGX_OK;
+    }
+  int occ = 1;
+  APGX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, block, lds), "occupancy query");
+  cache.push_back(Key{kernel, block, lds, occ});
+  *out = occ;
+  return APGX_OK;
 }
 
 int Launch(const void* kernel, dim3 grid, dim3 block, size_t lds, hipStream_t stream,
@@ -1183,6 +1200,12 @@ int Launch(const void* kernel, dim3 grid, dim3 block, size_t lds, hipStream_t st
 extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, uint32_t net_count,
                                const uint32_t* pins, uint64_t pin_count,
                                const apgx_route_options* options, apgx_search** out) {
+  return apgx::RouteNets(lat, nullptr, nets, net_count, pins, pin_count, options, out);
+}
+
+int apgx::RouteNets(apgx_lattice* lat, const CostOverride* ov, const apgx_net_query* nets,
+                    uint32_t net_count, const uint32_t* pins, uint64_t pin_count,
+                    const apgx_route_options* options, apgx_search** out) {
   using namespace apgx;
   const auto wall_start = std::chrono::steady_clock::now();
   if (lat == nullptr || out == nullptr || (net_count != 0 && (nets == nullptr || pins == nullptr)))
@@ -1196,15 +1219,15 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
   if (options) opt = *options;
   const uint32_t max_rounds = opt.max_rounds ? opt.max_rounds : 100000u;
   const apgx_lattice_desc& d = lat->desc;
-  hipDeviceProp_t prop;
-  APGX_HIP(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
-  const size_t lds_limit = prop.sharedMemPerBlock;
-  TileShape shape = DefaultShape(*lat, lds_limit);
+  const size_t lds_limit = lat->lds_limit;
+  const uint32_t max_cost = ov ? ov->max_cost : lat->max_cell_cost;
+  const uint32_t cost_bytes = ov ? 4u : lat->desc.cost_bytes;
+  TileShape shape = DefaultShape(lat->desc, max_cost, lds_limit);
   if (opt.tile_w || opt.tile_h) shape = TileShape{static_cast<int>(opt.tile_w), static_cast<int>(opt.tile_h)};
   Kernels k{};
   if (!KernelsForShape(shape, &k))
     return Fail(APGX_E_UNSUPPORTED, "apgx_route_nets: unsupported tile shape");
-  const bool reg_tile = shape.tx == 64 && shape.ty == 64 && RegTileUsable(*lat, lds_limit);
+  const bool reg_tile = shape.tx == 64 && shape.ty == 64 && RegTileUsable(d.layers, max_cost, lds_limit);
   uint32_t block = kThreads;
   size_t lds = TileLdsBytes(shape, d.layers);
   if (reg_tile) {
@@ -1307,16 +1330,26 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
     delete s;
     return Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_route_nets: workspace exceeds max_device_bytes");
   }
-  hipError_t e = hipMalloc(&s->d_block, total);
-  if (e != hipSuccess) { delete s; return HipFail(e, "search workspace allocation"); }
   {
     std::lock_guard<std::mutex> lock(lat->cache_mutex);
+    if (lat->cached_block && lat->cached_block_bytes >= total) {
+      s->d_block = lat->cached_block;
+      s->block_bytes = lat->cached_block_bytes;
+      lat->cached_block = nullptr;
+      lat->cached_block_bytes = 0;
+    }
     if (lat->cached_labels && lat->cached_label_bytes >= label_bytes) {
       s->d_labels = lat->cached_labels;
       s->label_bytes = lat->cached_label_bytes;
       lat->cached_labels = nullptr;
       lat->cached_label_bytes = 0;
     }
+  }
+  hipError_t e = hipSuccess;
+  if (s->d_block == nullptr) {
+    e = hipMalloc(&s->d_block, total);
+    if (e != hipSuccess) { delete s; return HipFail(e, "search workspace allocation"); }
+    s->block_bytes = total;
   }
   if (s->d_labels == nullptr) {
     e = hipMalloc(&s->d_labels, label_bytes);
@@ -1325,7 +1358,7 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
   }
   auto at = [&](const Part& p) { return static_cast<char*>(s->d_block) + p.offset; };
   SearchArgs a{};
-  a.cost = lat->d_cost;
+  a.cost = ov ? static_cast<const void*>(ov->plane) : lat->d_cost;
   a.W = d.width; a.H = d.height; a.L = d.layers; a.via = d.layers > 1 ? d.via_cost : 0;
   a.nets = reinterpret_cast<NetStatic*>(at(p_nets));
   a.state = reinterpret_cast<NetState*>(at(p_state));
@@ -1364,16 +1397,14 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
   s->d_cells = a.cells;
   s->d_cursor = a.cell_cursor;
 
-  hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::unique_lock<std::mutex> lease(lat->exec_mutex);
+  hipStream_t stream = lat->stream;
+  hipEvent_t ev0 = lat->events[0], ev1 = lat->events[1];
   int status = APGX_OK;
   auto check = [&](hipError_t err, const char* op) {
     if (status == APGX_OK && err != hipSuccess) status = HipFail(err, op);
     return status == APGX_OK;
   };
-  check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
-  check(hipEventCreate(&ev0), "hipEventCreate");
-  check(hipEventCreate(&ev1), "hipEventCreate");
   check(hipMemcpyAsync(const_cast<NetStatic*>(a.nets), ns.data(), sizeof(NetStatic) * net_count, hipMemcpyHostToDevice, stream), "net upload");
   check(hipMemcpyAsync(a.state, st.data(), sizeof(NetState) * net_count, hipMemcpyHostToDevice, stream), "net state upload");
   if (!tile_net.empty())
@@ -1404,11 +1435,10 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
   }
 
   int occupancy = 1;
-  if (status == APGX_OK)
-    check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occupancy, k.round, block, lds), "occupancy query");
+  if (status == APGX_OK) status = OccupancyOf(k.round, block, lds, &occupancy);
   occupancy = std::max(occupancy, 1);
   const uint32_t grid = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(
-      static_cast<uint64_t>(prop.multiProcessorCount) * occupancy, std::max<uint64_t>(tiles, 1))));
+      static_cast<uint64_t>(lat->cu_count) * occupancy, std::max<uint64_t>(tiles, 1))));
   const uint32_t chunk = opt.rounds_per_sync ? opt.rounds_per_sync : 8;
   uint32_t r = 1, syncs = 0, graph_launches = 0;
   std::vector<hipEvent_t> round_events;
@@ -1424,7 +1454,7 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
     // kernels take the round from device memory, so the same launches replay.
     auto launch_chunk = [&](uint32_t rounds, bool timed) {
       for (uint32_t c = 0; c < rounds && status == APGX_OK; ++c) {
-        uint32_t cb = d.cost_bytes;
+        uint32_t cb = cost_bytes;
         void* rargs[] = {&a, &cb};
         if (timed) {
           hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1504,9 +1534,7 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
       round_ms += x;
   }
   for (hipEvent_t ev : round_events) (void)hipEventDestroy(ev);
-  if (ev0) (void)hipEventDestroy(ev0);
-  if (ev1) (void)hipEventDestroy(ev1);
-  if (stream) (void)hipStreamDestroy(stream);
+  lease.unlock();
   if (status == APGX_OK && stats[7] != 0)
     status = Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_route_nets: connection cell buffer exhausted");
   if (status != APGX_OK) {
@@ -1521,7 +1549,7 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
   rs.cell_visits = stats[1];
   rs.inner_sweeps = stats[2];
   rs.relaxations = stats[3] * (d.layers > 1 && d.via_cost ? 6ull : 4ull);
-  rs.algorithmic_bytes = stats[1] * (static_cast<uint64_t>(d.cost_bytes) + 8ull);
+  rs.algorithmic_bytes = stats[1] * (static_cast<uint64_t>(cost_bytes) + 8ull);
   rs.workspace_bytes = total + s->label_bytes;
   rs.rounds = r - 1;
   rs.tile_w = static_cast<uint32_t>(shape.tx);
@@ -1605,7 +1633,17 @@ extern "C" int apgx_search_stats(const apgx_search* s, apgx_route_stats* out) {
 
 extern "C" void apgx_search_release(apgx_search* s) {
   if (s == nullptr) return;
-  if (s->d_block) (void)hipFree(s->d_block);
+  if (s->d_block) {
+    apgx_lattice* lat = s->lattice;
+    std::lock_guard<std::mutex> lock(lat->cache_mutex);
+    if (lat->cached_block == nullptr || lat->cached_block_bytes < s->block_bytes) {
+      if (lat->cached_block) (void)hipFree(lat->cached_block);
+      lat->cached_block = s->d_block;
+      lat->cached_block_bytes = s->block_bytes;
+    } else {
+      (void)hipFree(s->d_block);
+    }
+  }
   if (s->d_labels) {
     apgx_lattice* lat = s->lattice;
     std::lock_guard<std::mutex> lock(lat->cache_mutex);
@@ -1639,9 +1677,8 @@ extern "C" int apgx_autotune(apgx_lattice* lat, const apgx_net_query* nets, uint
   std::vector<apgx_net_query> picked(sample);
   for (uint32_t i = 0; i < sample; ++i)
     picked[i] = nets[static_cast<uint64_t>(i) * net_count / std::max(sample, 1u)];
-  hipDeviceProp_t prop{};
-  APGX_HIP(hipGetDeviceProperties(&prop, lat->device), "hipGetDeviceProperties");
-  const bool reg = RegTileUsable(*lat, prop.sharedMemPerBlock);
+  const size_t lds_limit = lat->lds_limit;
+  const bool reg = RegTileUsable(lat->desc.layers, lat->max_cell_cost, lds_limit);
   const uint32_t mean = static_cast<uint32_t>(std::max(1.0, lat->mean_cost));
   std::vector<apgx_tune_candidate> cands;
   const uint32_t deltas[] = {16 * mean, 64 * mean, 256 * mean, 0xFFFFFFFFu};
@@ -1650,7 +1687,7 @@ extern "C" int apgx_autotune(apgx_lattice* lat, const apgx_net_query* nets, uint
     if (t.tx == 64 && t.ty == 64 && !reg) continue;
     if (!(t.tx == 64 && t.ty == 64) &&
         (static_cast<uint64_t>(t.tx) * t.ty * lat->desc.layers > kMaxTileCells ||
-         TileLdsBytes(t, lat->desc.layers) > prop.sharedMemPerBlock))
+         TileLdsBytes(t, lat->desc.layers) > lds_limit))
       continue;
     for (uint32_t dl : deltas) {
       cands.push_back(apgx_tune_candidate{static_cast<uint32_t>(t.tx), static_cast<uint32_t>(t.ty), dl, 64, 0.0});

@@ -241,6 +241,79 @@ int apgx_autotune(apgx_lattice* lattice, const apgx_net_query* nets, uint32_t ne
                   apgx_tune_report* report);
 
 /* ------------------------------------------------------------------------ */
+/* Negotiated-congestion allocator                                          */
+/* ------------------------------------------------------------------------ */
+/* Canonical Sequential Rip-Up-and-Reroute v1 on the dense lattice with one
+ * resource per cell.  Replaces RouteCpuSequentialNegotiated
+ * (src/allocator/cpu_sequential_negotiated_routing.cc:993-1329) for this
+ * lattice model; the policy scalars are CanonicalSequentialRipUpAndReroutePolicyV1
+ * (include/apgar/allocator/cpu_sequential_negotiated_routing.h:38-56) and the
+ * value bound is CpuSequentialNegotiatedRoutingLimits::maximum_congestion_cost_value
+ * (:73).  Per pass p the present factor is initial + p * present_increment;
+ * nets are ripped up and rerouted in order, batch_nets at a time against one
+ * snapshot of cell cost
+ *     base + history + present_factor * max(0, usage + 1 - capacity)
+ * (usage without the batch's own routes); a disconnected net keeps its
+ * incumbent.  At each pass boundary: feasible (every net routed, zero
+ * overuse), else stalled (every route unchanged), else pass limit, else
+ * history += historical_increment * max(0, usage - capacity).  A congestion
+ * cost above maximum_congestion_cost_value, or a cell cost above
+ * APGX_MAX_CELL_COST, stops the run with APGX_E_RESOURCE_EXHAUSTED (the
+ * reference's kBoundExhausted).  batch_nets = 1 is the reference's order
+ * exactly, and so is batch_nets = 0: nets are grouped into levels of pairwise
+ * disjoint windows (a net's level is one above the highest level of an
+ * earlier net whose window overlaps it), so a level routed at once equals its
+ * nets routed one after another, with less latency.  All arithmetic is integer
+ * and runs on the GPU. */
+#define APGX_NEGOTIATION_FEASIBLE 0u
+#define APGX_NEGOTIATION_STALLED 1u
+#define APGX_NEGOTIATION_PASS_LIMIT 2u
+
+typedef struct apgx_negotiation_policy {
+  uint32_t maximum_passes;                 /* default 8 */
+  uint32_t batch_nets;                     /* default 1; 0: window-disjoint levels (exact order) */
+  uint64_t initial_present_cost;           /* default 0 */
+  uint64_t present_cost_increment;         /* default 100 */
+  uint64_t historical_cost_increment;      /* default 10 */
+  uint64_t maximum_congestion_cost_value;  /* default 10,000,000 */
+} apgx_negotiation_policy;
+
+typedef struct apgx_negotiation_summary {
+  uint32_t termination;                    /* APGX_NEGOTIATION_* */
+  uint32_t completed_passes;
+  uint32_t routed_nets;
+  uint32_t batches;                        /* snapshots routed */
+  uint64_t total_overuse_units;            /* at the last completed pass */
+  uint64_t tile_visits;                    /* summed over every search */
+  double search_device_ms;                 /* summed search envelopes */
+  double wall_ms;                          /* host wall time of apgx_negotiate */
+} apgx_negotiation_summary;
+
+typedef struct apgx_negotiation apgx_negotiation;
+
+/* Fills the reference defaults. */
+void apgx_negotiation_policy_defaults(apgx_negotiation_policy* policy);
+/* capacity: one byte per cell (NULL: capacity 1 everywhere); initial_history:
+ * one u32 per cell seeding the history plane (NULL: zero, the reference's
+ * pass-zero state); search_options: passed to every search (NULL: defaults). */
+int apgx_negotiate(apgx_lattice* lattice, const uint8_t* capacity, const apgx_net_query* nets,
+                   uint32_t net_count, const uint32_t* pins, uint64_t pin_count,
+                   const apgx_negotiation_policy* policy, const uint32_t* initial_history,
+                   const apgx_route_options* search_options, apgx_negotiation** out);
+/* overuse_per_pass (may be NULL) receives min(capacity, completed_passes) values. */
+int apgx_negotiation_get_summary(const apgx_negotiation* negotiation, apgx_negotiation_summary* out,
+                                 uint64_t* overuse_per_pass, uint32_t capacity);
+/* Final history and usage planes (one u32 per cell each; either may be NULL). */
+int apgx_negotiation_planes(const apgx_negotiation* negotiation, uint32_t* history,
+                            uint32_t* usage);
+/* Routes: route_len[n] tree cells per net (0: never routed), cells packed in
+ * net order: pin 0, then each connection path from its pin up to (not
+ * including) the tree cell it reaches.  *cell_count gets the total. */
+int apgx_negotiation_routes(const apgx_negotiation* negotiation, uint32_t* route_len,
+                            uint32_t* cells, uint64_t cell_capacity, uint64_t* cell_count);
+void apgx_negotiation_release(apgx_negotiation* negotiation);
+
+/* ------------------------------------------------------------------------ */
 /* V1: reference DeviceCompiledBoardV1 / DeviceCandidateBatch records        */
 /* ------------------------------------------------------------------------ */
 /* Byte-identical to include/apgar/gpu/planar_router.h:170-423 (sizes checked
@@ -500,6 +573,32 @@ int apgx_generate_nets(const uint32_t* cost, uint32_t width, uint32_t height, ui
                        uint32_t count, uint32_t min_pins, uint32_t max_pins, uint32_t min_side,
                        uint32_t max_side, uint64_t seed, apgx_net_query* nets_out,
                        uint32_t* pins_out, uint64_t* pin_total);
+
+/* Allocator boards: the lattice model above (costs U[cost_min, cost_max],
+ * rectangular keepouts), `bga_fields` non-overlapping fields of bga_pads x
+ * bga_pads pads at bga_pitch (pads on layer 0, forced free), and `nets` nets
+ * of [min_pins, max_pins] distinct pins.  Pin 0 of each net is an unused pad
+ * of a random field; every further pin is, with probability pad_pin_ppm, an
+ * unused pad (same field or another), otherwise a free cell on a random layer
+ * within scatter_radius (Chebyshev) of that field's centre.  No cell is a pin
+ * of two nets.  The window of a net is its pin bounding box grown by
+ * window_margin and clipped.  cost_out holds width*height*layers u32;
+ * pins_out nets*max_pins cells. */
+typedef struct apgx_board_gen_params {
+  uint32_t width, height, layers;
+  uint32_t cost_min, cost_max;
+  uint32_t keepout_ppm, keepout_min_side, keepout_max_side;
+  uint32_t nets, min_pins, max_pins;
+  uint32_t bga_fields, bga_pads, bga_pitch;
+  uint32_t pad_pin_ppm;
+  uint32_t scatter_radius;
+  uint32_t window_margin;
+  uint32_t reserved;
+  uint64_t seed;
+} apgx_board_gen_params;
+
+int apgx_generate_board(const apgx_board_gen_params* params, uint32_t* cost_out,
+                        apgx_net_query* nets_out, uint32_t* pins_out, uint64_t* pin_total);
 
 #ifdef __cplusplus
 }

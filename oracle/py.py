@@ -116,6 +116,79 @@ class Dense:
         return rc, conns
 
 
+# ------------------------------------------------------ negotiated allocator
+class NegoPolicy(C.Structure):
+    _fields_ = [("maximum_passes", C.c_uint32), ("batch_nets", C.c_uint32),
+                ("initial_present_cost", C.c_uint64), ("present_cost_increment", C.c_uint64),
+                ("historical_cost_increment", C.c_uint64),
+                ("maximum_congestion_cost_value", C.c_uint64), ("maximum_cell_cost", C.c_uint64)]
+
+
+class NegoNet(C.Structure):
+    _fields_ = [("pin_offset", C.c_uint32), ("pin_count", C.c_uint32), ("wx0", C.c_uint32),
+                ("wy0", C.c_uint32), ("ww", C.c_uint32), ("wh", C.c_uint32)]
+
+
+class NegoResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("termination", C.c_uint32), ("completed_passes", C.c_uint32),
+                ("routed_nets", C.c_uint32), ("total_overuse_units", C.c_uint64)]
+
+
+def negotiate(cost, via, nets, pins, *, capacity=None, initial_history=None, maximum_passes=8,
+              batch_nets=1, initial_present_cost=0, present_cost_increment=100,
+              historical_cost_increment=10, maximum_congestion_cost_value=10_000_000,
+              maximum_cell_cost=(1 << 20) - 1):
+    """Runs the allocator oracle.  nets: list of (pin_offset, pin_count, window or None).
+    Returns a dict with status, termination, passes, overuse per pass, history,
+    occupancy and per-net tree cell arrays (None for unrouted nets)."""
+    h = lib()
+    if not hasattr(h, "_nego_ready"):
+        P = C.POINTER
+        h.orc_negotiate.restype = C.c_int
+        h.orc_negotiate.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_uint32), C.c_uint32,
+                                    P(C.c_uint8), P(NegoNet), C.c_uint32, P(C.c_uint32),
+                                    P(NegoPolicy), P(C.c_uint32), P(C.c_uint32), P(C.c_uint32),
+                                    P(C.c_uint64), P(C.c_uint32), P(C.c_uint32), C.c_uint64,
+                                    P(NegoResult)]
+        h._nego_ready = True
+    cost = np.ascontiguousarray(cost, dtype=np.uint32)
+    L, H, W = cost.shape
+    pins = np.ascontiguousarray(pins, dtype=np.uint32)
+    arr = (NegoNet * max(len(nets), 1))()
+    for i, (off, cnt, win) in enumerate(nets):
+        wx0, wy0, ww, wh = win if win is not None else (0, 0, 0, 0)
+        arr[i] = NegoNet(off, cnt, wx0, wy0, ww, wh)
+    pol = NegoPolicy(maximum_passes, batch_nets, initial_present_cost, present_cost_increment,
+                     historical_cost_increment, maximum_congestion_cost_value, maximum_cell_cost)
+    n = cost.size
+    cap = None if capacity is None else np.ascontiguousarray(capacity, dtype=np.uint8).reshape(-1)
+    init = None if initial_history is None else np.ascontiguousarray(initial_history, dtype=np.uint32).reshape(-1)
+    hist = np.zeros(n, dtype=np.uint32)
+    occ = np.zeros(n, dtype=np.uint32)
+    per_pass = np.zeros(max(maximum_passes, 1), dtype=np.uint64)
+    rlen = np.zeros(max(len(nets), 1), dtype=np.uint32)
+    rcap = n * 4 + 64
+    rcells = np.empty(rcap, dtype=np.uint32)
+    res = NegoResult()
+    rc = h.orc_negotiate(W, H, L, _u32(cost), via,
+                         cap.ctypes.data_as(C.POINTER(C.c_uint8)) if cap is not None else None,
+                         arr, len(nets), _u32(pins), C.byref(pol),
+                         _u32(init) if init is not None else None, _u32(hist), _u32(occ),
+                         per_pass.ctypes.data_as(C.POINTER(C.c_uint64)), _u32(rlen), _u32(rcells),
+                         rcap, C.byref(res))
+    if rc != 0:
+        raise RuntimeError("oracle negotiate failed")
+    routes, off = [], 0
+    for i in range(len(nets)):
+        k = int(rlen[i])
+        routes.append(rcells[off:off + k].copy() if k else None)
+        off += k
+    return {"status": res.status, "termination": res.termination, "passes": res.completed_passes,
+            "routed": res.routed_nets, "overuse": int(res.total_overuse_units),
+            "overuse_per_pass": [int(v) for v in per_pass[:res.completed_passes]],
+            "history": hist.reshape(L, H, W), "occupancy": occ.reshape(L, H, W), "routes": routes}
+
+
 # ------------------------------------------------------------------ V1 model
 class V1Node(C.Structure):
     _fields_ = [("x", C.c_int64), ("y", C.c_int64), ("layer", C.c_uint32), ("mask", C.c_uint8),

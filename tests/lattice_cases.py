@@ -6,6 +6,13 @@ The five lattice workloads of this project (sizes, cost models, seeds):
                      (sides U[4,64]), vias cost 20, 1024 pairs, seed 2
   eight_layer_2048   8 x 2048 x 2048, same model, 4096 nets of 2-8 pins, windows
                      log-uniform 32..2048, seed 3
+  six_layer_1536     allocator boards: 6 x 1536 x 1536, uint16 cost U[1,8], 10%
+                     keepouts, five BGA fields of 32 x 32 pads at 4-cell pitch
+                     (128 x 128-cell footprints, pads on layer 0), 2000 nets of
+                     2-6 pins: pin 0 on a pad, further pins on a pad with
+                     probability 0.3, else scattered within 384 cells of the
+                     net's field; windows = pin bounding box + 16; capacity 1
+                     per cell; seeds 0-249 (0-199 training, 200-249 held out)
 """
 from __future__ import annotations
 
@@ -18,6 +25,19 @@ EIGHT_LAYER_2048 = dict(width=2048, height=2048, layers=8, cost_min=1, cost_max=
                         keepout_ppm=150_000, keepout_min_side=4, keepout_max_side=64, seed=3,
                         via=20, nets=4096, min_pins=2, max_pins=8, min_side=32, max_side=2048,
                         cost_bytes=2)
+
+
+SIX_LAYER_1536 = dict(width=1536, height=1536, layers=6, cost_min=1, cost_max=8,
+                      keepout_ppm=100_000, keepout_min_side=4, keepout_max_side=64, nets=2000,
+                      min_pins=2, max_pins=6, bga_fields=5, bga_pads=32, bga_pitch=4,
+                      pad_pin_ppm=300_000, scatter_radius=384, window_margin=16, via=20)
+
+
+def make_board(apgx, spec, seed, **overrides):
+    """Allocator board of the six_layer_1536 family (or a scaled variant)."""
+    p = {k: v for k, v in spec.items() if k != "via"}
+    p.update(overrides)
+    return apgx.generate_board(seed=seed, **p)
 
 
 def make_lattice(apgx, spec):

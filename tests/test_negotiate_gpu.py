@@ -1,0 +1,144 @@
+"""GPU parity of the negotiated-congestion allocator against the CPU oracle.
+
+Bar: bit-exact (termination, passes, overuse per pass, history and usage
+planes, every net's tree cells) on scaled boards of the six_layer_1536 family
+and on the hand-derived bottleneck board; at full size, legality and the
+accounting identities for every net."""
+import numpy as np
+import pytest
+
+from tests.lattice_cases import SIX_LAYER_1536, make_board
+from tests.test_oracle_negotiate import bottleneck_board
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(width=96, height=96, layers=3, nets=30, bga_fields=3, bga_pads=8, bga_pitch=3,
+             scatter_radius=40, window_margin=6, keepout_max_side=16)
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def oracle_nets(nets):
+    return [(n.pin_offset, n.pin_count, (n.window.x0, n.window.y0, n.window.width, n.window.height))
+            for n in nets]
+
+
+def assert_same(gpu, orc):
+    assert gpu["termination"] == orc["termination"]
+    assert gpu["passes"] == orc["passes"]
+    assert gpu["overuse_per_pass"] == orc["overuse_per_pass"]
+    assert gpu["routed"] == orc["routed"]
+    assert np.array_equal(gpu["history"], orc["history"])
+    assert np.array_equal(gpu["occupancy"], orc["occupancy"])
+    for a, b in zip(gpu["routes"], orc["routes"]):
+        assert (a is None) == (b is None)
+        if a is not None:
+            assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("seed,batch", [(0, 1), (1, 4), (2, 16), (3, 30), (4, 0), (5, 0)])
+def test_small_boards_match_oracle(apgx, oracle, gpu, seed, batch):
+    """batch 0 (window-disjoint levels) must equal the oracle's one-net-at-a-time order."""
+    _say(f"[case] allocator parity seed={seed} batch={batch}")
+    cost, nets, pins = make_board(apgx, SIX_LAYER_1536, 1000 + seed, **SMALL)
+    via = SIX_LAYER_1536["via"]
+    lat = apgx.Lattice(cost, via_cost=via, cost_bytes=2)
+    g = lat.negotiate(nets, pins, maximum_passes=12, batch_nets=batch)
+    o = oracle.negotiate(cost, via, oracle_nets(nets), pins, maximum_passes=12, batch_nets=max(batch, 1))
+    if batch == 0:
+        assert g["batches"] < 12 * len(nets)  # levels group nets
+    assert o["status"] == 0
+    assert g["overuse_per_pass"][0] > 0  # the board is congested to begin with
+    assert_same(g, o)
+    lat.close()
+
+
+def test_seeded_history_and_capacity_match_oracle(apgx, oracle, gpu):
+    _say("[case] allocator parity with seeded history and capacity plane")
+    cost, nets, pins = make_board(apgx, SIX_LAYER_1536, 1010, **SMALL)
+    rng = np.random.default_rng(4)
+    hist0 = (rng.integers(0, 6, size=cost.shape) * 10).astype(np.uint32)
+    cap = np.where(rng.random(cost.shape) < 0.05, 2, 1).astype(np.uint8)
+    via = SIX_LAYER_1536["via"]
+    lat = apgx.Lattice(cost, via_cost=via, cost_bytes=2)
+    g = lat.negotiate(nets, pins, capacity=cap, initial_history=hist0, maximum_passes=10, batch_nets=8,
+                      present_cost_increment=50, historical_cost_increment=20)
+    o = oracle.negotiate(cost, via, oracle_nets(nets), pins, capacity=cap, initial_history=hist0,
+                         maximum_passes=10, batch_nets=8, present_cost_increment=50,
+                         historical_cost_increment=20)
+    assert_same(g, o)
+    lat.close()
+
+
+def test_bottleneck_board(apgx, oracle, gpu):
+    _say("[case] allocator bottleneck board")
+    W, H, cost, pins, onets = bottleneck_board()
+    nets, p = apgx.nets_from_lists([pins[:2], pins[2:]])
+    lat = apgx.Lattice(cost, via_cost=0)
+    for batch in (1, 2):
+        g = lat.negotiate(nets, p, maximum_passes=8 if batch == 1 else 2, batch_nets=batch)
+        o = oracle.negotiate(cost, 0, onets, pins, maximum_passes=8 if batch == 1 else 2, batch_nets=batch)
+        assert_same(g, o)
+    lat.close()
+
+
+def test_value_bound_is_reported(apgx, gpu):
+    cost = np.array([[[1, 5, 1]]], dtype=np.uint32)
+    nets, p = apgx.nets_from_lists([[0, 2]])
+    lat = apgx.Lattice(cost, via_cost=0)
+    with pytest.raises(apgx.ApgxError) as e:
+        lat.negotiate(nets, p, capacity=np.zeros(3, dtype=np.uint8), initial_present_cost=7,
+                      maximum_congestion_cost_value=6, maximum_passes=1)
+    assert e.value.code == 2
+    lat.close()
+
+
+def tree_is_legal(route, pins, flat_cost, W, H, L, window):
+    cells = set(int(c) for c in route)
+    assert len(cells) == len(route)
+    assert set(int(p) for p in pins) <= cells
+    assert (flat_cost[route] > 0).all()
+    x0, y0, w, h = window
+    xs, ys = route % W, (route // W) % H
+    assert ((xs >= x0) & (xs < x0 + w) & (ys >= y0) & (ys < y0 + h)).all()
+    plane = W * H
+    seen, stack = {int(pins[0])}, [int(pins[0])]
+    while stack:
+        c = stack.pop()
+        x, y, l = c % W, (c // W) % H, c // plane
+        for nb, ok in ((c - 1, x > 0), (c + 1, x + 1 < W), (c - W, y > 0), (c + W, y + 1 < H),
+                       (c - plane, l > 0), (c + plane, l + 1 < L)):
+            if ok and nb in cells and nb not in seen:
+                seen.add(nb)
+                stack.append(nb)
+    assert seen == cells  # one connected tree over lattice edges
+
+
+def test_six_layer_1536_full_board(apgx, gpu):
+    """Full-size board, three passes in batches of 250: legality of every route
+    and the usage / overuse / history identities (the oracle is too slow here)."""
+    spec = SIX_LAYER_1536
+    _say("[case] six_layer_1536 seed 200: 2000 nets, 3 passes of batched negotiation")
+    cost, nets, pins = make_board(apgx, spec, 200)
+    L, H, W = cost.shape
+    lat = apgx.Lattice(cost, via_cost=spec["via"], cost_bytes=2)
+    r = lat.negotiate(nets, pins, maximum_passes=3, batch_nets=250)
+    _say(f"[info] termination={r['termination']} passes={r['passes']} overuse={r['overuse_per_pass']} "
+         f"wall_ms={r['wall_ms']:.0f}")
+    flat = cost.reshape(-1)
+    usage = np.zeros(flat.size, dtype=np.int64)
+    for i, route in enumerate(r["routes"]):
+        n = nets[i]
+        if route is None:
+            continue
+        p = pins[n.pin_offset:n.pin_offset + n.pin_count]
+        tree_is_legal(route.astype(np.int64), p, flat, W, H, L,
+                      (n.window.x0, n.window.y0, n.window.width, n.window.height))
+        usage[route] += 1
+    assert np.array_equal(usage, r["occupancy"].reshape(-1))
+    assert int(np.maximum(usage - 1, 0).sum()) == r["overuse"]
+    assert int(r["history"].sum()) == 10 * sum(r["overuse_per_pass"][:-1])
+    assert r["passes"] == 3 and r["routed"] > 0.95 * len(nets)
+    lat.close()

@@ -1,0 +1,122 @@
+"""CPU checks of the negotiated-allocator oracle (oracle/negotiate.c).
+
+Pins the restated CS-RR-v1 arithmetic with the reference's zero-capacity
+known answer, a hand-derived two-net bottleneck, and invariants that hold for
+any board (occupancy and overuse recomputed from the returned routes)."""
+import numpy as np
+import pytest
+
+from tests.test_oracle_dense import random_lattice
+
+
+def cell(W, H, x, y, l=0):
+    return (l * H + y) * W + x
+
+
+def test_zero_capacity_known_answer(oracle):
+    # Reference known answer: congestion cost 12 = base 5 + initial present 7 *
+    # max(0, 0 + 1 - 0); a bound of 12 is accepted, 11 rejected
+    # (tests/allocator/cpu_sequential_negotiated_routing_test.cc:787-831).
+    cost = np.array([[[1, 5, 1]]], dtype=np.uint32)
+    cap = np.zeros(3, dtype=np.uint8)
+    nets = [(0, 2, None)]
+    pins = np.array([0, 2], dtype=np.uint32)
+    r = oracle.negotiate(cost, 0, nets, pins, capacity=cap, initial_present_cost=7,
+                         maximum_cell_cost=12, maximum_passes=1)
+    assert r["status"] == 0
+    assert r["termination"] == 2 and r["passes"] == 1 and r["overuse_per_pass"] == [3]
+    assert not r["history"].any()  # no history update after the terminal pass
+    r = oracle.negotiate(cost, 0, nets, pins, capacity=cap, initial_present_cost=7,
+                         maximum_cell_cost=11, maximum_passes=1)
+    assert r["status"] == 1  # the cell of cost 12 exceeds the bound
+    r = oracle.negotiate(cost, 0, nets, pins, capacity=cap, initial_present_cost=7,
+                         maximum_congestion_cost_value=6, maximum_passes=1)
+    assert r["status"] == 1  # congestion 7 exceeds the congestion bound
+    # Default prices: the only route never changes, so pass 1 stalls; history
+    # is updated once, after the non-terminal pass 0: 10 x overuse 1 per cell.
+    r = oracle.negotiate(cost, 0, nets, pins, capacity=cap, maximum_passes=4)
+    assert r["status"] == 0 and r["termination"] == 1 and r["passes"] == 2
+    assert r["overuse_per_pass"] == [3, 3]
+    assert list(r["history"].reshape(-1)) == [10, 10, 10]
+
+
+def bottleneck_board():
+    """7 x 5 single layer, unit costs; a wall at x = 3 with gaps at y = 1 and 3.
+    Net A runs (0,1) -> (6,1) straight through gap 1 (cost 6; detour via gap 3
+    costs 10).  Net B runs (1,0) -> (5,0) and must pass gap 1 (cost 6) or
+    detour via gap 3 (cost 10)."""
+    W, H = 7, 5
+    cost = np.ones((1, H, W), dtype=np.uint32)
+    for y in (0, 2, 4):
+        cost[0, y, 3] = 0
+    pins = np.array([cell(W, H, 0, 1), cell(W, H, 6, 1), cell(W, H, 1, 0), cell(W, H, 5, 0)],
+                    dtype=np.uint32)
+    nets = [(0, 2, None), (2, 2, None)]
+    return W, H, cost, pins, nets
+
+
+def test_two_nets_negotiate_a_bottleneck_sequentially(oracle):
+    W, H, cost, pins, nets = bottleneck_board()
+    r = oracle.negotiate(cost, 0, nets, pins, maximum_passes=8)
+    assert r["status"] == 0
+    # Pass 0 (no prices): both use gap 1 and share at least (2,1), (3,1), (4,1).
+    # Pass 1 (present factor 100): A, rerouted first against B's route, pays
+    # 300 through gap 1 and detours (cost 10); B then keeps gap 1.
+    assert r["termination"] == 0 and r["passes"] == 2
+    assert r["overuse_per_pass"][0] >= 3 and r["overuse_per_pass"][1] == 0
+    a, b = r["routes"]
+    gap1, gap3 = cell(W, H, 3, 1), cell(W, H, 3, 3)
+    assert gap3 in a and gap1 not in a and len(a) == 11
+    assert gap1 in b and len(b) == 7
+    hist = r["history"].reshape(-1)
+    shared = {cell(W, H, x, 1) for x in (2, 3, 4)}
+    assert all(hist[c] == 10 for c in shared)
+    assert set(np.flatnonzero(hist)) <= shared | {cell(W, H, 1, 1), cell(W, H, 5, 1)}
+    assert np.array_equal(r["occupancy"].reshape(-1) > 0,
+                          np.isin(np.arange(W * H), np.concatenate([a, b])))
+
+
+def test_batched_snapshot_moves_both_nets_together(oracle):
+    W, H, cost, pins, nets = bottleneck_board()
+    r = oracle.negotiate(cost, 0, nets, pins, maximum_passes=2, batch_nets=2)
+    # Both nets see the same snapshot in pass 1: no present cost (each sees
+    # the other's route ripped up), history 10 on each shared gap-1 cell, so
+    # both detour through gap 3 together.
+    a, b = r["routes"]
+    gap3 = cell(W, H, 3, 3)
+    assert gap3 in a and gap3 in b
+    assert r["termination"] == 2 and r["overuse_per_pass"][1] >= 3
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_occupancy_and_overuse_follow_from_routes(oracle, seed):
+    rng = np.random.default_rng(300 + seed)
+    L, H, W = 2, 16, 16
+    cost = random_lattice(rng, L, H, W, blocked=0.1, cmax=6)
+    free = np.flatnonzero(cost.reshape(-1))
+    nets, flat = [], []
+    for i in range(8):
+        k = int(rng.integers(2, 4))
+        p = rng.choice(free, size=k, replace=False)
+        p = [int(v) for v in p if int(v) not in flat][:k]
+        if len(p) < 2:
+            continue
+        nets.append((len(flat), len(p), None))
+        flat.extend(p)
+    pins = np.array(flat, dtype=np.uint32)
+    r = oracle.negotiate(cost, 5, nets, pins, maximum_passes=6, batch_nets=[1, 3, 8, 2][seed])
+    assert r["status"] == 0
+    occ = np.zeros(L * H * W, dtype=np.int64)
+    for t in r["routes"]:
+        if t is not None:
+            assert len(set(t.tolist())) == len(t)  # tree cells are distinct
+            occ[t] += 1
+    assert np.array_equal(occ, r["occupancy"].reshape(-1))
+    assert int(np.maximum(occ - 1, 0).sum()) == r["overuse"]
+    assert r["overuse_per_pass"][-1] == r["overuse"]
+    hist = r["history"].reshape(-1)
+    assert (hist % 10 == 0).all()
+    # Every non-terminal pass adds 10 x its overuse, summed over cells.
+    assert int(hist.sum()) == 10 * sum(r["overuse_per_pass"][:-1])
+    if r["termination"] == 0:
+        assert r["overuse"] == 0 and r["routed"] == len(nets)
