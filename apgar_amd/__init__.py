@@ -20,6 +20,7 @@ APGX_OK = 0
 ERROR_NAMES = {1: "unsupported", 2: "resource_exhausted", 3: "backend_failure", 4: "internal_invariant"}
 UNREACHABLE = 0xFFFFFFFF
 NET_REACHED, NET_DISCONNECTED, NET_BUDGET_EXHAUSTED, NET_INVALID = 0, 1, 2, 3
+NEGOTIATION_FEASIBLE, NEGOTIATION_STALLED, NEGOTIATION_PASS_LIMIT = 0, 1, 2
 LATTICE_SCHEMA_VERSION = 2
 
 
@@ -108,7 +109,7 @@ class BoardGenParams(C.Structure):
     _fields_ = [(k, C.c_uint32) for k in (
         "width", "height", "layers", "cost_min", "cost_max", "keepout_ppm", "keepout_min_side",
         "keepout_max_side", "nets", "min_pins", "max_pins", "bga_fields", "bga_pads", "bga_pitch",
-        "pad_pin_ppm", "scatter_radius", "window_margin", "reserved")] + [("seed", C.c_uint64)]
+        "pad_pin_ppm", "scatter_radius", "window_margin", "pin_clearance")] + [("seed", C.c_uint64)]
 
 
 class LatticeGenParams(C.Structure):
@@ -116,6 +117,27 @@ class LatticeGenParams(C.Structure):
                 ("cost_min", C.c_uint32), ("cost_max", C.c_uint32), ("blocked_ppm", C.c_uint32),
                 ("keepout_ppm", C.c_uint32), ("keepout_min_side", C.c_uint32),
                 ("keepout_max_side", C.c_uint32), ("reserved", C.c_uint32), ("seed", C.c_uint64)]
+
+
+class PriorConfig(C.Structure):
+    _fields_ = [("lattice_layers", C.c_uint32), ("grid_h", C.c_uint32), ("grid_w", C.c_uint32),
+                ("reserved", C.c_uint32), ("seed", C.c_uint64)]
+
+
+class PriorTrainOptions(C.Structure):
+    _fields_ = [("epochs", C.c_uint32), ("batch", C.c_uint32), ("learning_rate", C.c_float),
+                ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
+                ("augment", C.c_uint32), ("use_graph", C.c_uint32), ("seed", C.c_uint64)]
+
+
+class PriorTrainReport(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("first_epoch_loss", C.c_float), ("last_epoch_loss", C.c_float),
+                ("device_ms", C.c_double), ("wall_ms", C.c_double), ("graph", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+class PriorDecode(C.Structure):
+    _fields_ = [("tile", C.c_uint32), ("max_value", C.c_uint32), ("unit", C.c_float), ("gain", C.c_float)]
 
 
 _P = C.POINTER
@@ -149,6 +171,22 @@ _SIGNATURES = {
     "apgx_negotiation_release": (None, [C.c_void_p]),
     "apgx_autotune": (C.c_int, [C.c_void_p, _P(NetQuery), C.c_uint32, _P(C.c_uint32), C.c_uint64,
                                 C.c_uint32, _P(RouteOptions), _P(RouteOptions), _P(TuneReport)]),
+    "apgx_prior_create": (C.c_int, [_P(PriorConfig), _P(C.c_void_p)]),
+    "apgx_prior_release": (None, [C.c_void_p]),
+    "apgx_prior_param_count": (C.c_uint64, [C.c_void_p]),
+    "apgx_prior_get_params": (C.c_int, [C.c_void_p, _P(C.c_float), C.c_uint64]),
+    "apgx_prior_set_params": (C.c_int, [C.c_void_p, _P(C.c_float), C.c_uint64]),
+    "apgx_prior_forward": (C.c_int, [C.c_void_p, _P(C.c_float), C.c_uint32, _P(C.c_float)]),
+    "apgx_prior_gradients": (C.c_int, [C.c_void_p, _P(C.c_float), _P(C.c_float), C.c_uint32,
+                                       _P(C.c_float), _P(C.c_float)]),
+    "apgx_prior_train_defaults": (None, [_P(PriorTrainOptions)]),
+    "apgx_prior_train": (C.c_int, [C.c_void_p, _P(C.c_float), _P(C.c_float), C.c_uint32,
+                                   _P(PriorTrainOptions), _P(PriorTrainReport)]),
+    "apgx_prior_features": (C.c_int, [C.c_void_p, _P(C.c_uint32), C.c_uint64, C.c_uint32, _P(C.c_float)]),
+    "apgx_prior_pool_history": (C.c_int, [_P(C.c_uint32), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                          C.c_float, _P(C.c_float)]),
+    "apgx_prior_seed_history": (C.c_int, [C.c_void_p, C.c_void_p, _P(C.c_uint32), C.c_uint64,
+                                          _P(PriorDecode), _P(C.c_uint32)]),
     "apgx_generate_lattice": (C.c_int, [_P(LatticeGenParams), _P(C.c_uint32)]),
     "apgx_generate_pairs": (C.c_int, [_P(C.c_uint32), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                       C.c_uint64, _P(C.c_uint32)]),
@@ -421,3 +459,102 @@ def nets_from_lists(pin_lists, windows=None):
             nets[i].window = Window(*windows[i])
         flat.extend(int(p) for p in pl)
     return nets, np.asarray(flat, dtype=np.uint32)
+
+
+def _f32p(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_P(C.c_float))
+
+
+class Prior:
+    """Learned history-cost prior (apgx_prior_*): a conv encoder-decoder over
+    per-tile board maps, trained and run by the library's bf16 MFMA kernels."""
+
+    TILE = 64
+    UNIT = 10.0
+
+    def __init__(self, lattice_layers: int, grid_h: int, grid_w: int, seed: int = 1):
+        self.layers, self.grid = lattice_layers, (grid_h, grid_w)
+        cfg = PriorConfig(lattice_layers=lattice_layers, grid_h=grid_h, grid_w=grid_w, seed=seed)
+        h = C.c_void_p()
+        check(lib().apgx_prior_create(C.byref(cfg), C.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().apgx_prior_release(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def param_count(self) -> int:
+        return int(lib().apgx_prior_param_count(self.handle))
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.param_count, dtype=np.float32)
+        check(lib().apgx_prior_get_params(self.handle, _f32p(out), out.size))
+        return out
+
+    def set_params(self, params: np.ndarray):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        check(lib().apgx_prior_set_params(self.handle, _f32p(params), params.size))
+
+    def forward(self, inputs: np.ndarray) -> np.ndarray:
+        """inputs [n][2L][gh][gw] -> outputs [n][L][gh][gw]."""
+        x = np.ascontiguousarray(inputs, dtype=np.float32)
+        n = x.shape[0]
+        out = np.empty((n, self.layers) + self.grid, dtype=np.float32)
+        check(lib().apgx_prior_forward(self.handle, _f32p(x), n, _f32p(out)))
+        return out
+
+    def gradients(self, inputs: np.ndarray, targets: np.ndarray):
+        x = np.ascontiguousarray(inputs, dtype=np.float32)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        g = np.empty(self.param_count, dtype=np.float32)
+        loss = np.zeros(1, dtype=np.float32)
+        check(lib().apgx_prior_gradients(self.handle, _f32p(x), _f32p(t), x.shape[0], _f32p(loss), _f32p(g)))
+        return float(loss[0]), g
+
+    def train(self, inputs: np.ndarray, targets: np.ndarray, **options) -> dict:
+        x = np.ascontiguousarray(inputs, dtype=np.float32)
+        t = np.ascontiguousarray(targets, dtype=np.float32)
+        o = PriorTrainOptions()
+        lib().apgx_prior_train_defaults(C.byref(o))
+        for k, v in options.items():
+            setattr(o, k, v)
+        rep = PriorTrainReport()
+        check(lib().apgx_prior_train(self.handle, _f32p(x), _f32p(t), x.shape[0], C.byref(o), C.byref(rep)))
+        return {k: getattr(rep, k) for k, _ in PriorTrainReport._fields_ if k != "reserved"}
+
+    def seed_history(self, lattice: "Lattice", pins: np.ndarray, gain: float = 1.0,
+                     max_value: int = 10_000_000) -> np.ndarray:
+        pins = np.ascontiguousarray(pins, dtype=np.uint32)
+        L, H, W = lattice.shape
+        out = np.empty((L, H, W), dtype=np.uint32)
+        dec = PriorDecode(tile=self.TILE, max_value=max_value, unit=self.UNIT, gain=gain)
+        check(lib().apgx_prior_seed_history(self.handle, lattice.handle, u32p(pins), pins.size, C.byref(dec),
+                                            u32p(out)))
+        return out
+
+
+def prior_features(lattice: "Lattice", pins: np.ndarray, tile: int = 64) -> np.ndarray:
+    """Input planes [2L][gh][gw] of one board (pin density, blocked fraction per tile)."""
+    pins = np.ascontiguousarray(pins, dtype=np.uint32)
+    L, H, W = lattice.shape
+    out = np.empty((2 * L, -(-H // tile), -(-W // tile)), dtype=np.float32)
+    check(lib().apgx_prior_features(lattice.handle, u32p(pins), pins.size, tile, _f32p(out)))
+    return out
+
+
+def prior_targets(history: np.ndarray, tile: int = 64, unit: float = 10.0) -> np.ndarray:
+    """Training targets [L][gh][gw] = log1p(tile mean history / unit)."""
+    h = np.ascontiguousarray(history, dtype=np.uint32)
+    L, H, W = h.shape
+    out = np.empty((L, -(-H // tile), -(-W // tile)), dtype=np.float32)
+    check(lib().apgx_prior_pool_history(u32p(h), W, H, L, tile, unit, _f32p(out)))
+    return out

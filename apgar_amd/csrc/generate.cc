@@ -227,6 +227,27 @@ extern "C" int apgx_generate_board(const apgx_board_gen_params* p, uint32_t* cos
   std::vector<uint8_t> used(plane * p->layers, 0);
   for (uint32_t f = 0; f < p->bga_fields; ++f)
     for (uint32_t i = 0; i < pads; ++i) used[pad_cell(f, i)] = 1;
+  // Pin access (pin_clearance r > 0): cells within Chebyshev distance r of a
+  // pin on its layer, and the cell above each pad, are kept out of reach of
+  // scattered pins and forced free at the end, so no pin is walled in by
+  // keepouts or by another net's pin.
+  const uint32_t clr = p->pin_clearance;
+  std::vector<uint8_t> near(clr ? plane * p->layers : 0, 0);
+  auto mark_near = [&](uint64_t c) {
+    const uint64_t l = c / plane, y = (c % plane) / p->width, x = c % p->width;
+    for (int64_t dy = -static_cast<int64_t>(clr); dy <= static_cast<int64_t>(clr); ++dy)
+      for (int64_t dx = -static_cast<int64_t>(clr); dx <= static_cast<int64_t>(clr); ++dx) {
+        const int64_t yy = static_cast<int64_t>(y) + dy, xx = static_cast<int64_t>(x) + dx;
+        if (yy >= 0 && yy < p->height && xx >= 0 && xx < p->width)
+          near[l * plane + static_cast<uint64_t>(yy) * p->width + static_cast<uint64_t>(xx)] = 1;
+      }
+  };
+  if (clr)
+    for (uint32_t f = 0; f < p->bga_fields; ++f)
+      for (uint32_t i = 0; i < pads; ++i) {
+        mark_near(pad_cell(f, i));
+        if (p->layers > 1) near[plane + pad_cell(f, i)] = 1;
+      }
   std::vector<uint32_t> free_pads(static_cast<size_t>(p->bga_fields) * pads);
   std::vector<uint32_t> free_count(p->bga_fields, pads);
   for (uint32_t f = 0; f < p->bga_fields; ++f)
@@ -268,7 +289,8 @@ extern "C" int apgx_generate_board(const apgx_board_gen_params* p, uint32_t* cos
           if (++tries > 1000000) return apgx::Fail(APGX_E_INTERNAL_INVARIANT, "apgx_generate_board: no free cell");
           const uint32_t l = rng.Range(0, p->layers - 1);
           c = l * plane + static_cast<uint64_t>(rng.Range(y0, y1)) * p->width + rng.Range(x0, x1);
-        } while (cost[c] == 0 || used[c]);
+        } while (cost[c] == 0 || used[c] || (clr && near[c]));
+        if (clr) mark_near(c);
       }
       used[c] = 1;
       pins_out[w++] = static_cast<uint32_t>(c);
@@ -287,6 +309,10 @@ extern "C" int apgx_generate_board(const apgx_board_gen_params* p, uint32_t* cos
     nets_out[n].pin_offset = static_cast<uint32_t>(base);
     nets_out[n].pin_count = k;
     nets_out[n].window = apgx_window{wx0, wy0, wx1 - wx0 + 1, wy1 - wy0 + 1};
+  }
+  if (clr) {
+    for (uint64_t c = 0; c < plane * p->layers; ++c)
+      if (near[c] && cost[c] == 0) cost[c] = rng.Range(p->cost_min, p->cost_max);
   }
   *pin_total = w;
   apgx::ClearError();

@@ -119,6 +119,9 @@ struct SearchArgs {
   uint32_t delta;
   uint32_t min_edge;     // smallest edge weight: bounds path lengths by distance / min_edge + 1
   uint32_t goal_cost;    // smallest in-layer step cost for the goal bound; 0 turns it off
+  // 1: a via move into cell v weighs via + cost[v] (every entry into a cell pays
+  // that cell's cost, as the allocator's per-cell congestion needs); 0: via.
+  uint32_t via_pays_cell;
 };
 
 template <typename CostT>
@@ -341,7 +344,8 @@ __device__ void ProcessTile(const SearchArgs& a, uint32_t r, uint32_t g, uint32_
         uint32_t carry = stack[0] >> 3;
         for (int l = 1; l < L; ++l) {
           const uint32_t v = stack[l * kLab];
-          const uint32_t c = cstack[l * kCost] >= kInfD ? kInfD : a.via;
+          const uint32_t ce = cstack[l * kCost];
+          const uint32_t c = ce >= kInfD ? kInfD : a.via + (a.via_pays_cell ? ce : 0u);
           const uint32_t nv = min(v, Step(carry, c, kFromBelow));
           changed |= nv != v;
           stack[l * kLab] = nv;
@@ -349,7 +353,8 @@ __device__ void ProcessTile(const SearchArgs& a, uint32_t r, uint32_t g, uint32_
         }
         for (int l = L - 2; l >= 0; --l) {
           const uint32_t v = stack[l * kLab];
-          const uint32_t c = cstack[l * kCost] >= kInfD ? kInfD : a.via;
+          const uint32_t ce = cstack[l * kCost];
+          const uint32_t c = ce >= kInfD ? kInfD : a.via + (a.via_pays_cell ? ce : 0u);
           const uint32_t nv = min(v, Step(carry, c, kFromAbove));
           changed |= nv != v;
           stack[l * kLab] = nv;
@@ -661,16 +666,30 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
       for (int y = 0; y < kRT; ++y) lab[y] = T[y * kRP + lane];
       if (a.via != 0) {  // the host passes via = 0 for single-layer lattices
         __syncthreads();  // every layer published in its buffer
+        if (a.via_pays_cell) {
+          // Via weight via + cost of the entered cell: the packed cost half
+          // (all ones when blocked) saturates the sum for blocked cells.
 #pragma unroll
-        for (int y = 0; y < kRT; ++y) {
-          // All ones for a blocked cell (sign of its packed cost half): the
-          // via adds then saturate, so the cell keeps its label.  Arithmetic
-          // rather than a branch keeps both LDS reads unconditional.
-          const uint32_t blk = (y & 1) ? static_cast<uint32_t>(static_cast<int32_t>(cst[y >> 1]) >> 31)
-                                       : static_cast<uint32_t>(static_cast<int32_t>(cst[y >> 1] << 16) >> 31);
-          const uint32_t fb = __builtin_elementwise_add_sat(below[y * kRP + lane], via_b | blk) & ~7u;  // kFromBelow = 0
-          const uint32_t fa = Relax8(above[y * kRP + lane], via_a | blk, kFromAbove);
-          lab[y] = min(lab[y], min(fb, fa));
+          for (int y = 0; y < kRT; ++y) {
+            const uint32_t c8 = (y & 1) ? CostHi(cst[y >> 1]) : CostLo(cst[y >> 1]);
+            const uint32_t wb = __builtin_elementwise_add_sat(via_b, c8);
+            const uint32_t wa = __builtin_elementwise_add_sat(via_a, c8);
+            const uint32_t fb = __builtin_elementwise_add_sat(below[y * kRP + lane], wb) & ~7u;  // kFromBelow = 0
+            const uint32_t fa = Relax8(above[y * kRP + lane], wa, kFromAbove);
+            lab[y] = min(lab[y], min(fb, fa));
+          }
+        } else {
+#pragma unroll
+          for (int y = 0; y < kRT; ++y) {
+            // All ones for a blocked cell (sign of its packed cost half): the
+            // via adds then saturate, so the cell keeps its label.  Arithmetic
+            // rather than a branch keeps both LDS reads unconditional.
+            const uint32_t blk = (y & 1) ? static_cast<uint32_t>(static_cast<int32_t>(cst[y >> 1]) >> 31)
+                                         : static_cast<uint32_t>(static_cast<int32_t>(cst[y >> 1] << 16) >> 31);
+            const uint32_t fb = __builtin_elementwise_add_sat(below[y * kRP + lane], via_b | blk) & ~7u;  // kFromBelow = 0
+            const uint32_t fa = Relax8(above[y * kRP + lane], via_a | blk, kFromAbove);
+            lab[y] = min(lab[y], min(fb, fa));
+          }
         }
       }
       uint32_t new_lo = 0, new_hi = 0;
@@ -1383,6 +1402,7 @@ int apgx::RouteNets(apgx_lattice* lat, const CostOverride* ov, const apgx_net_qu
   a.tile_count = static_cast<uint32_t>(tiles);
   a.min_edge = std::max(lat->min_edge, 1u);
   a.goal_cost = opt.no_goal_bound ? 0u : lat->min_cell_cost;
+  a.via_pays_cell = (ov && ov->via_pays_cell) ? 1u : 0u;
   // Bucket width: about one tile width of distance by default; all-ones
   // disables ordering (every queued tile is processed every round).
   if (opt.frontier_delta == 0xFFFFFFFFu) {

@@ -49,6 +49,8 @@ namespace apgx {
 struct CostOverride {
   const uint32_t* plane;
   uint32_t max_cost;
+  // 1: via moves also pay the entered cell's cost (via + plane[v]).
+  uint32_t via_pays_cell;
 };
 
 int RouteNets(apgx_lattice* lat, const CostOverride* override_plane, const apgx_net_query* nets,

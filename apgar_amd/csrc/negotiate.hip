@@ -348,7 +348,7 @@ extern "C" int apgx_negotiate(apgx_lattice* lat, const uint8_t* capacity, const 
       // Route the batch on the snapshot plane.
       batch_nets.clear();
       for (uint32_t n : members) batch_nets.push_back(nets[n]);
-      const CostOverride ov{b.snap, std::max(host_flags.max_cost, lat->min_cell_cost)};
+      const CostOverride ov{b.snap, std::max(host_flags.max_cost, lat->min_cell_cost), 1u};
       apgx_search* s = nullptr;
       if (int rc = RouteNets(lat, &ov, batch_nets.data(), count, pins, pin_count, &sopt, &s); rc != APGX_OK)
         return rc;

@@ -42,7 +42,7 @@ def parse_args():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", choices=["four_layer_1024", "eight_layer_2048"],
+    p.add_argument("--workload", choices=["four_layer_1024", "eight_layer_2048", "learned_prior"],
                    default="four_layer_1024")
     p.add_argument("--queries", type=int, default=0, help="queries / nets (0: the workload's count)")
     p.add_argument("--cpu-sample", type=int, default=0, help="queries timed on the CPU oracle (0: auto)")
@@ -57,6 +57,14 @@ def parse_args():
     p.add_argument("--no-tune", action="store_true", help="eight_layer_2048: skip the autotuner")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run a device-to-device copy of MIB MiB (HBM counter scale reference)")
+    p.add_argument("--train-seeds", type=int, default=200, help="learned_prior: training boards")
+    p.add_argument("--heldout-seeds", type=int, default=50, help="learned_prior: held-out boards")
+    p.add_argument("--dataset", type=str, default="", help="learned_prior: .npz of collected training "
+                   "data (loaded if present, else collected and written)")
+    p.add_argument("--collect-only", action="store_true", help="learned_prior: stop after collection")
+    p.add_argument("--epochs", type=int, default=300, help="learned_prior: training epochs")
+    p.add_argument("--full-size-boards", type=int, default=0,
+                   help="learned_prior: also compare on this many six_layer_1536 boards")
     return p.parse_args()
 
 
@@ -131,16 +139,111 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def learned_prior(args):
+    """Learned history-cost prior on six_layer_512 boards: plain allocator runs on
+    training seeds 0.. give the targets, the network trains on the GPU, the
+    decode gain is chosen on the last tenth of the training seeds, and held-out
+    seeds 200.. compare time to zero overflow, learned seed vs zero history."""
+    from apgar_amd import prior_study as ps
+    from tests.lattice_cases import SIX_LAYER_1536, SIX_LAYER_512
+
+    spec = SIX_LAYER_512
+    L = spec["layers"]
+    seeds = list(range(args.train_seeds))
+    path = Path(args.dataset) if args.dataset else None
+    t0 = time.perf_counter()
+    if path is not None and path.exists():
+        d = np.load(path, allow_pickle=False)
+        x, y = d["x"], d["y"]
+        plain_ms = d["plain_ms"]
+        log(f"[bench] loaded {x.shape[0]} training samples from {path}")
+    else:
+        x, y, runs = ps.collect(spec, seeds, log=log)
+        plain_ms = np.array([r["wall_ms"] for r in runs])
+        if path is not None:
+            path.parent.mkdir(parents=True, exist_ok=True)
+            np.savez(path, x=x, y=y, plain_ms=plain_ms,
+                     termination=np.array([r["termination"] for r in runs]),
+                     passes=np.array([r["passes"] for r in runs]))
+    collect_s = time.perf_counter() - t0
+    if args.collect_only:
+        print(json.dumps({"collected": int(x.shape[0]), "collect_s": collect_s,
+                          "plain_ms_mean": float(np.mean(plain_ms))}))
+        return
+    n_val = max(1, len(seeds) // 10)
+    n_fit = len(seeds) - n_val
+    net, rep = ps.train(x[:n_fit], y[:n_fit], L, epochs=args.epochs, batch=32)
+    log(f"[bench] trained on {n_fit} boards: {rep}")
+    mse, zero_mse = ps.prediction_error(net, x[n_fit:], y[n_fit:])
+    mean_map = y[:n_fit].mean(axis=0, keepdims=True)
+    mean_mse = float(np.mean((mean_map - y[n_fit:]) ** 2))
+    log(f"[bench] validation MSE {mse:.4f} (zero history {zero_mse:.4f}, mean map {mean_mse:.4f})")
+    gains = (0.5, 1.0, 2.0)
+    val = {}
+    for g in gains:
+        res = [ps.seeded_run(net, spec, s, g) for s in seeds[n_fit:]]
+        val[g] = ps.compare(res)
+        log(f"[bench] validation gain {g}: {val[g]}")
+    def score(c):
+        return (c.get("learned_feasible", 0), -c.get("time_ratio", 9.0))
+    gain = max(gains, key=lambda g: score(val[g]))
+    held = []
+    for s in range(200, 200 + args.heldout_seeds):
+        r = ps.seeded_run(net, spec, s, gain)
+        held.append(r)
+        log(f"[bench] held-out seed {s}: plain {r['plain']['passes']} passes "
+            f"{r['plain']['time_to_zero_overflow_ms']} ms, learned {r['learned']['passes']} passes "
+            f"{r['learned']['time_to_zero_overflow_ms']} ms")
+    cmp = ps.compare(held)
+    full = []
+    for s in range(200, 200 + args.full_size_boards):
+        r = ps.seeded_run(net, SIX_LAYER_1536, s, gain)
+        full.append(r)
+        log(f"[bench] six_layer_1536 seed {s}: plain {r['plain']['passes']} passes "
+            f"{r['plain']['time_to_zero_overflow_ms']} ms, learned {r['learned']['passes']} passes "
+            f"{r['learned']['time_to_zero_overflow_ms']} ms")
+    line = {
+        "metric": "learned-prior vs plain time to 0 overflow",
+        "value": cmp.get("time_ratio"),
+        "unit": "ratio (learned / plain wall time, boards where both reach zero overflow)",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "vs_baseline": None,
+        "dtype": "bf16 (network), u32 (allocator)",
+        "data": "synthetic",
+        "config": {"workload": "six_layer_512 allocator boards (one 32x32-pad BGA field, 200 nets of 2-6 "
+                               "pins, capacity 1), exact-order negotiation, 40-pass limit",
+                   "train_seeds": f"0-{n_fit - 1}", "validation_seeds": f"{n_fit}-{len(seeds) - 1}",
+                   "heldout_seeds": f"200-{199 + args.heldout_seeds}", "decode_gain": gain},
+        "heldout": cmp,
+        "validation": {str(g): v for g, v in val.items()},
+        "prediction_mse": {"network": mse, "zero_history": zero_mse, "training_mean_map": mean_mse},
+        "training": {k: (float(v) if isinstance(v, float) else v) for k, v in rep.items()},
+        "collect_s": collect_s,
+    }
+    if full:
+        line["six_layer_1536"] = ps.compare(full)
+        line["six_layer_1536_runs"] = [{"seed": r["seed"], "plain": r["plain"]["passes"],
+                                        "learned": r["learned"]["passes"],
+                                        "plain_ms": r["plain"]["time_to_zero_overflow_ms"],
+                                        "learned_ms": r["learned"]["time_to_zero_overflow_ms"]} for r in full]
+    print(json.dumps(line))
+    net.close()
+
+
 def main():
     args = parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
-    if args.gpus != 1:
+    if args.gpus != 1 and args.workload != "learned_prior":
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); queries are "
               "independent and would shard by query list, which this build does not launch.",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "learned_prior":
+        learned_prior(args)
+        return
     big = args.workload == "eight_layer_2048"
     spec = EIGHT_LAYER_2048 if big else FOUR_LAYER_1024
     cost = make_lattice(apgx, spec)

@@ -314,6 +314,95 @@ int apgx_negotiation_routes(const apgx_negotiation* negotiation, uint32_t* route
 void apgx_negotiation_release(apgx_negotiation* negotiation);
 
 /* ------------------------------------------------------------------------ */
+/* Learned history-cost prior                                               */
+/* ------------------------------------------------------------------------ */
+/* No reference counterpart: the reference has no learned component.  A small
+ * convolutional encoder-decoder maps per-layer maps of a board, one value per
+ * lattice tile (pin density and blocked fraction), to per-layer history-cost
+ * maps; the decoded integer plane only seeds apgx_negotiate's history
+ * (initial_history), so exact searches and the overuse check still judge
+ * every route and a zero plane is the fallback.
+ *
+ * Network: 4 levels (grid, /2, /4, /8) of two 3x3 convs (32 channels, ReLU),
+ * 2x2 average pooling down, nearest 2x upsampling plus skip on the way up,
+ * and a 3x3 head; 15 convs.  Convs run on bf16 MFMA with fp32 accumulation
+ * (bf16 weights and activations at the matrix-core inputs, fp32 elsewhere).
+ * Parameters, flat fp32, per conv i = 0..14 in order: weights
+ * w[cout][3*3][cin] then bias[cout]; cin is 16 for conv 0 (inputs 2L..15 are
+ * zero planes) and 32 otherwise, cout is 16 for conv 14 (outputs L..15 are
+ * unused) and 32 otherwise.
+ *
+ * Planar layouts: inputs [sample][2L][grid_h][grid_w] with plane 2l the pin
+ * density of layer l (pins in the tile / 64) and 2l+1 the blocked fraction
+ * (lattice cells outside the grid count as blocked); targets and outputs
+ * [sample][L][grid_h][grid_w] = log1p(mean tile history / unit). */
+typedef struct apgx_prior apgx_prior;
+
+typedef struct apgx_prior_config {
+  uint32_t lattice_layers;  /* L, 1..8 */
+  uint32_t grid_h;          /* tiles per column, multiple of 8 */
+  uint32_t grid_w;          /* tiles per row, multiple of 8 */
+  uint32_t reserved;
+  uint64_t seed;            /* He-normal weight initialisation */
+} apgx_prior_config;
+
+typedef struct apgx_prior_train_options {
+  uint32_t epochs;          /* default 200 */
+  uint32_t batch;           /* default 32 */
+  float learning_rate;      /* Adam, default 2e-3 */
+  float beta1;              /* 0.9 */
+  float beta2;              /* 0.999 */
+  float epsilon;            /* 1e-8 */
+  uint32_t augment;         /* 1: random flip / transpose per sample and step (square grids) */
+  uint32_t use_graph;       /* 1: each step replays one captured hipGraph */
+  uint64_t seed;            /* shuffling and augmentation */
+} apgx_prior_train_options;
+
+typedef struct apgx_prior_train_report {
+  uint64_t steps;
+  float first_epoch_loss;   /* mean step loss of the first and last epoch */
+  float last_epoch_loss;
+  double device_ms;         /* hipEvent envelope of all steps */
+  double wall_ms;
+  uint32_t graph;           /* 1 if steps replayed a hipGraph */
+  uint32_t reserved;
+} apgx_prior_train_report;
+
+typedef struct apgx_prior_decode {
+  uint32_t tile;            /* lattice cells per grid cell side (64) */
+  uint32_t max_value;       /* clamp of the decoded history */
+  float unit;               /* history unit of the targets (the historical increment) */
+  float gain;               /* history = gain * unit * expm1(max(output, 0)), rounded */
+} apgx_prior_decode;
+
+int apgx_prior_create(const apgx_prior_config* config, apgx_prior** out);
+void apgx_prior_release(apgx_prior* prior);
+uint64_t apgx_prior_param_count(const apgx_prior* prior);
+int apgx_prior_get_params(const apgx_prior* prior, float* params, uint64_t count);
+int apgx_prior_set_params(apgx_prior* prior, const float* params, uint64_t count);
+/* Outputs for n planar input samples. */
+int apgx_prior_forward(apgx_prior* prior, const float* inputs, uint32_t n, float* outputs);
+/* Mean squared error over the L used outputs of n samples, and its gradient
+ * with respect to every parameter (flat layout above). */
+int apgx_prior_gradients(apgx_prior* prior, const float* inputs, const float* targets, uint32_t n,
+                         float* loss, float* grads);
+void apgx_prior_train_defaults(apgx_prior_train_options* options);
+/* Adam on the whole data set (uploaded once), `epochs` shuffled passes. */
+int apgx_prior_train(apgx_prior* prior, const float* inputs, const float* targets, uint32_t samples,
+                     const apgx_prior_train_options* options, apgx_prior_train_report* report);
+/* Input planes of one board (2L x ceil(H/tile) x ceil(W/tile)), from the
+ * lattice's blocked cells and the given pin cells. */
+int apgx_prior_features(apgx_lattice* lattice, const uint32_t* pins, uint64_t pin_count, uint32_t tile,
+                        float* features);
+/* Training targets from a history plane (L x H x W u32). */
+int apgx_prior_pool_history(const uint32_t* history, uint32_t width, uint32_t height, uint32_t layers,
+                            uint32_t tile, float unit, float* targets);
+/* Features, network and decoding on the device: the integer history plane
+ * (L x H x W u32) that seeds apgx_negotiate for this board. */
+int apgx_prior_seed_history(apgx_prior* prior, apgx_lattice* lattice, const uint32_t* pins,
+                            uint64_t pin_count, const apgx_prior_decode* decode, uint32_t* history);
+
+/* ------------------------------------------------------------------------ */
 /* V1: reference DeviceCompiledBoardV1 / DeviceCandidateBatch records        */
 /* ------------------------------------------------------------------------ */
 /* Byte-identical to include/apgar/gpu/planar_router.h:170-423 (sizes checked
@@ -593,7 +682,8 @@ typedef struct apgx_board_gen_params {
   uint32_t pad_pin_ppm;
   uint32_t scatter_radius;
   uint32_t window_margin;
-  uint32_t reserved;
+  uint32_t pin_clearance;       /* r > 0: no scattered pin within Chebyshev r of another pin on
+                                   its layer or on the cell above a pad; those cells are forced free */
   uint64_t seed;
 } apgx_board_gen_params;
 

@@ -86,6 +86,11 @@ static int cell_free(const orc_lattice* g, uint32_t c) {
   return g->cost[c] != 0 && in_window(g, x, y);
 }
 
+/* Weight of a via move into cell v: via, plus v's cost when via_pays_cell. */
+static uint32_t via_weight(const orc_lattice* g, uint32_t v) {
+  return g->via + (g->via_pays_cell ? g->cost[v] : 0u);
+}
+
 /* Neighbours of c in ascending cell-index order: l-1, y-1, x-1, x+1, y+1, l+1.
  * Writes (cell, weight of the move INTO the neighbour) pairs; returns count. */
 static int neighbours(const orc_lattice* g, uint32_t c, uint32_t* nb, uint32_t* w_into) {
@@ -93,12 +98,12 @@ static int neighbours(const orc_lattice* g, uint32_t c, uint32_t* nb, uint32_t* 
   uint32_t x = c % W, y = (c / W) % H, l = c / (W * H);
   int n = 0;
   uint64_t plane = (uint64_t)W * H;
-  if (g->via && l > 0) { nb[n] = (uint32_t)(c - plane); w_into[n++] = g->via; }
+  if (g->via && l > 0) { nb[n] = (uint32_t)(c - plane); w_into[n++] = via_weight(g, (uint32_t)(c - plane)); }
   if (y > 0) { nb[n] = c - W; w_into[n++] = g->cost[c - W]; }
   if (x > 0) { nb[n] = c - 1; w_into[n++] = g->cost[c - 1]; }
   if (x + 1 < W) { nb[n] = c + 1; w_into[n++] = g->cost[c + 1]; }
   if (y + 1 < H) { nb[n] = c + W; w_into[n++] = g->cost[c + W]; }
-  if (g->via && l + 1 < L) { nb[n] = (uint32_t)(c + plane); w_into[n++] = g->via; }
+  if (g->via && l + 1 < L) { nb[n] = (uint32_t)(c + plane); w_into[n++] = via_weight(g, (uint32_t)(c + plane)); }
   int k = 0;
   for (int i = 0; i < n; ++i) {
     if (cell_free(g, nb[i])) {
@@ -114,7 +119,7 @@ static int neighbours(const orc_lattice* g, uint32_t c, uint32_t* nb, uint32_t* 
 static uint32_t move_weight(const orc_lattice* g, uint32_t u, uint32_t v) {
   uint64_t plane = (uint64_t)g->W * g->H;
   uint64_t d = u > v ? (uint64_t)u - v : (uint64_t)v - u;
-  if (d == plane) return g->via;
+  if (d == plane) return via_weight(g, v);
   return g->cost[v];
 }
 

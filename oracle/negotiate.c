@@ -98,7 +98,7 @@ int orc_negotiate(uint32_t W, uint32_t H, uint32_t L, const uint32_t* base, uint
       }
       for (uint32_t n = b0; n < b1; ++n) {
         const orc_nego_net* q = &nets[n];
-        orc_lattice lat = {W, H, L, snap, via, q->wx0, q->wy0, q->ww, q->wh};
+        orc_lattice lat = {W, H, L, snap, via, q->wx0, q->wy0, q->ww, q->wh, 1u};
         uint64_t used = 0;
         const int r = orc_dense_route_net(&lat, pins + q->pin_offset, q->pin_count, cpin, cdist, clen,
                                           path, path_cap, &used);

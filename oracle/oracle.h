@@ -31,6 +31,7 @@ typedef struct orc_lattice {
   const uint32_t* cost;
   uint32_t via;
   uint32_t wx0, wy0, ww, wh;
+  uint32_t via_pays_cell; /* 1: a via move into v weighs via + cost[v] (allocator searches) */
 } orc_lattice;
 
 /* Exact multi-source Dijkstra (binary heap keyed (dist, cell)); dist has W*H*L

@@ -20,7 +20,8 @@ UNREACHABLE = 0xFFFFFFFF
 class OrcLattice(C.Structure):
     _fields_ = [("W", C.c_uint32), ("H", C.c_uint32), ("L", C.c_uint32),
                 ("cost", C.POINTER(C.c_uint32)), ("via", C.c_uint32), ("wx0", C.c_uint32),
-                ("wy0", C.c_uint32), ("ww", C.c_uint32), ("wh", C.c_uint32)]
+                ("wy0", C.c_uint32), ("ww", C.c_uint32), ("wh", C.c_uint32),
+                ("via_pays_cell", C.c_uint32)]
 
 
 _lib = None
@@ -61,12 +62,12 @@ def _u32(a):
 class Dense:
     """Dense lattice oracle over a (L, H, W) uint32 cost array (0 = blocked)."""
 
-    def __init__(self, cost: np.ndarray, via: int, window=None):
+    def __init__(self, cost: np.ndarray, via: int, window=None, via_pays_cell=False):
         self.cost = np.ascontiguousarray(cost, dtype=np.uint32)
         L, H, W = self.cost.shape
         self.shape = (L, H, W)
         wx0, wy0, ww, wh = window if window is not None else (0, 0, 0, 0)
-        self.g = OrcLattice(W, H, L, _u32(self.cost), via, wx0, wy0, ww, wh)
+        self.g = OrcLattice(W, H, L, _u32(self.cost), via, wx0, wy0, ww, wh, 1 if via_pays_cell else 0)
 
     @property
     def n(self):

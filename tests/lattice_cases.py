@@ -11,8 +11,12 @@ The five lattice workloads of this project (sizes, cost models, seeds):
                      (128 x 128-cell footprints, pads on layer 0), 2000 nets of
                      2-6 pins: pin 0 on a pad, further pins on a pad with
                      probability 0.3, else scattered within 384 cells of the
-                     net's field; windows = pin bounding box + 16; capacity 1
-                     per cell; seeds 0-249 (0-199 training, 200-249 held out)
+                     net's field; windows = pin bounding box + 16; pin clearance 1
+                     (no pin walled in by keepouts or another pin); capacity 1
+                     per cell
+  six_layer_512      the same generator on 6 x 512 x 512 with one field and 200
+                     nets (learned-prior study boards; seeds 0-199 training,
+                     200-249 held out)
 """
 from __future__ import annotations
 
@@ -30,7 +34,13 @@ EIGHT_LAYER_2048 = dict(width=2048, height=2048, layers=8, cost_min=1, cost_max=
 SIX_LAYER_1536 = dict(width=1536, height=1536, layers=6, cost_min=1, cost_max=8,
                       keepout_ppm=100_000, keepout_min_side=4, keepout_max_side=64, nets=2000,
                       min_pins=2, max_pins=6, bga_fields=5, bga_pads=32, bga_pitch=4,
-                      pad_pin_ppm=300_000, scatter_radius=384, window_margin=16, via=20)
+                      pad_pin_ppm=300_000, scatter_radius=384, window_margin=16, pin_clearance=1,
+                      via=20)
+
+# One field of the six_layer_1536 family on a 512 x 512 board at about the same
+# net density (200 nets per 512^2 against 2000 per 1536^2): the scale at which
+# the learned prior's 200 training and 50 held-out allocator runs fit in minutes.
+SIX_LAYER_512 = dict(SIX_LAYER_1536, width=512, height=512, nets=200, bga_fields=1, scatter_radius=192)
 
 
 def make_board(apgx, spec, seed, **overrides):

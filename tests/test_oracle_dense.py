@@ -226,3 +226,30 @@ def test_keepout_generator_coverage(apgx):
     for l in range(2):
         assert (cost[l] == 0).mean() >= 0.15
     assert cost.max() <= 8
+
+
+def test_via_pays_cell_closed_form(oracle):
+    """Allocator searches: a via move into v weighs via + cost[v].  Two layers,
+    one row; layer 0 has a 100-cost cell between the endpoints."""
+    cost = np.array([[[1, 100, 1]], [[1, 1, 1]]], dtype=np.uint32)
+    flat = oracle.Dense(cost, 5).dijkstra([0])
+    pays = oracle.Dense(cost, 5, via_pays_cell=True).dijkstra([0])
+    # flat vias: up 5, two steps 1 + 1, down 5 = 12 (< 101 straight)
+    assert flat[2] == 12 and flat[3] == 5 and flat[1] == 11
+    # paying vias: up 5 + 1, 1 + 1, down 5 + 1 = 14; into (0,0,1) = 6; the
+    # 100-cost cell is cheapest straight (a via into it would add 5 + 100)
+    assert pays[2] == 14 and pays[3] == 6 and pays[1] == 100
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_via_pays_cell_matches_bellman_ford(oracle, seed):
+    rng = np.random.default_rng(50 + seed)
+    cost = random_lattice(rng, 3, 5, 6)
+    L, H, W = cost.shape
+    es = [(u, v, w + (int(cost.reshape(-1)[v]) if abs(u - v) == H * W else 0))
+          for u, v, w in edges(cost, 7)]
+    free = np.flatnonzero(cost.reshape(-1))
+    src = int(free[0])
+    want = bellman_ford(cost.size, es, [src])
+    got = oracle.Dense(cost, 7, via_pays_cell=True).dijkstra([src]).astype(np.uint64)
+    assert np.array_equal(got, want)
