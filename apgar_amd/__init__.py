@@ -119,6 +119,22 @@ class LatticeGenParams(C.Structure):
                 ("keepout_max_side", C.c_uint32), ("reserved", C.c_uint32), ("seed", C.c_uint64)]
 
 
+class PricePolicy(C.Structure):
+    _fields_ = [("initial_present_factor", C.c_uint64), ("present_factor_increment", C.c_uint64),
+                ("historical_price_increment", C.c_uint64)]
+
+
+class PriceLimits(C.Structure):
+    _fields_ = [("maximum_price_entries", C.c_uint64), ("maximum_hot_resources", C.c_uint64),
+                ("maximum_price_value", C.c_uint64), ("maximum_aggregate_price", C.c_uint64)]
+
+
+class PriceEpoch(C.Structure):
+    _fields_ = [("outcome", C.c_uint32), ("reserved", C.c_uint32), ("outcome_cell", C.c_uint64),
+                ("epoch_index", C.c_uint64), ("present_factor", C.c_uint64), ("hot_resources", C.c_uint64),
+                ("price_entries", C.c_uint64), ("aggregate_price", C.c_uint64)]
+
+
 class PriorConfig(C.Structure):
     _fields_ = [("lattice_layers", C.c_uint32), ("grid_h", C.c_uint32), ("grid_w", C.c_uint32),
                 ("reserved", C.c_uint32), ("seed", C.c_uint64)]
@@ -171,6 +187,10 @@ _SIGNATURES = {
     "apgx_negotiation_release": (None, [C.c_void_p]),
     "apgx_autotune": (C.c_int, [C.c_void_p, _P(NetQuery), C.c_uint32, _P(C.c_uint32), C.c_uint64,
                                 C.c_uint32, _P(RouteOptions), _P(RouteOptions), _P(TuneReport)]),
+    "apgx_price_defaults": (None, [_P(PricePolicy), _P(PriceLimits)]),
+    "apgx_price_update": (C.c_int, [C.c_uint64, _P(PricePolicy), _P(PriceLimits), C.c_uint64, _P(C.c_uint32),
+                                    _P(C.c_uint8), _P(C.c_uint32), _P(C.c_uint32), _P(C.c_uint32),
+                                    _P(PriceEpoch)]),
     "apgx_prior_create": (C.c_int, [_P(PriorConfig), _P(C.c_void_p)]),
     "apgx_prior_release": (None, [C.c_void_p]),
     "apgx_prior_param_count": (C.c_uint64, [C.c_void_p]),
@@ -558,3 +578,25 @@ def prior_targets(history: np.ndarray, tile: int = 64, unit: float = 10.0) -> np
     out = np.empty((L, -(-H // tile), -(-W // tile)), dtype=np.float32)
     check(lib().apgx_prior_pool_history(u32p(h), W, H, L, tile, unit, _f32p(out)))
     return out
+
+
+def price_update(epoch: int, usage: np.ndarray, history: np.ndarray, capacity=None, **settings):
+    """Candidate-allocator price epoch (apgx_price_update) on host planes.
+    settings: apgx_price_policy / apgx_price_limits fields.  Returns (epoch
+    summary dict, present, history, total); planes are unchanged copies of
+    the input history (present / total zero) when the epoch fails."""
+    pol, lim = PricePolicy(), PriceLimits()
+    lib().apgx_price_defaults(C.byref(pol), C.byref(lim))
+    for k, v in settings.items():
+        setattr(pol if hasattr(pol, k) else lim, k, v)
+    u = np.ascontiguousarray(usage, dtype=np.uint32).reshape(-1)
+    h = np.ascontiguousarray(history, dtype=np.uint32).reshape(-1).copy()
+    cap = None if capacity is None else np.ascontiguousarray(capacity, dtype=np.uint8).reshape(-1)
+    present = np.zeros_like(h)
+    total = np.zeros_like(h)
+    out = PriceEpoch()
+    check(lib().apgx_price_update(epoch, C.byref(pol), C.byref(lim), u.size, u32p(u),
+                                  cap.ctypes.data_as(_P(C.c_uint8)) if cap is not None else None,
+                                  u32p(h), u32p(present), u32p(total), C.byref(out)))
+    summary = {k: getattr(out, k) for k, _ in PriceEpoch._fields_ if k != "reserved"}
+    return summary, present, h, total

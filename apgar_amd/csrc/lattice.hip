@@ -482,7 +482,9 @@ __device__ __forceinline__ uint32_t CostAt(const uint32_t* pairs) {
   return (I & 1) ? CostHi(pairs[I >> 1]) : CostLo(pairs[I >> 1]);
 }
 
-template <int L>
+// PAYS: vias also pay the entered cell's cost (allocator searches); a
+// template parameter so the plain kernel's register allocation is untouched.
+template <int L, bool PAYS>
 __global__ __launch_bounds__(64 * L) __attribute__((amdgpu_waves_per_eu(2)))
 void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
   extern __shared__ uint32_t smem[];
@@ -666,7 +668,7 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
       for (int y = 0; y < kRT; ++y) lab[y] = T[y * kRP + lane];
       if (a.via != 0) {  // the host passes via = 0 for single-layer lattices
         __syncthreads();  // every layer published in its buffer
-        if (a.via_pays_cell) {
+        if constexpr (PAYS) {
           // Via weight via + cost of the entered cell: the packed cost half
           // (all ones when blocked) saturates the sum for blocked cells.
 #pragma unroll
@@ -777,18 +779,23 @@ size_t RegTileLdsBytes(uint32_t layers) {
   return static_cast<size_t>(layers) * (kRT * kRP + 2 * kRT) * 4;
 }
 
-const void* RegKernelFor(uint32_t layers) {
+template <bool PAYS>
+const void* RegKernelForT(uint32_t layers) {
   switch (layers) {
-    case 1: return reinterpret_cast<const void*>(&RegRoundKernel<1>);
-    case 2: return reinterpret_cast<const void*>(&RegRoundKernel<2>);
-    case 3: return reinterpret_cast<const void*>(&RegRoundKernel<3>);
-    case 4: return reinterpret_cast<const void*>(&RegRoundKernel<4>);
-    case 5: return reinterpret_cast<const void*>(&RegRoundKernel<5>);
-    case 6: return reinterpret_cast<const void*>(&RegRoundKernel<6>);
-    case 7: return reinterpret_cast<const void*>(&RegRoundKernel<7>);
-    case 8: return reinterpret_cast<const void*>(&RegRoundKernel<8>);
+    case 1: return reinterpret_cast<const void*>(&RegRoundKernel<1, PAYS>);
+    case 2: return reinterpret_cast<const void*>(&RegRoundKernel<2, PAYS>);
+    case 3: return reinterpret_cast<const void*>(&RegRoundKernel<3, PAYS>);
+    case 4: return reinterpret_cast<const void*>(&RegRoundKernel<4, PAYS>);
+    case 5: return reinterpret_cast<const void*>(&RegRoundKernel<5, PAYS>);
+    case 6: return reinterpret_cast<const void*>(&RegRoundKernel<6, PAYS>);
+    case 7: return reinterpret_cast<const void*>(&RegRoundKernel<7, PAYS>);
+    case 8: return reinterpret_cast<const void*>(&RegRoundKernel<8, PAYS>);
     default: return nullptr;
   }
+}
+
+const void* RegKernelFor(uint32_t layers, bool via_pays_cell) {
+  return via_pays_cell ? RegKernelForT<true>(layers) : RegKernelForT<false>(layers);
 }
 
 __device__ __forceinline__ uint64_t LocalIndex(const NetStatic& ns, uint32_t W, uint32_t H,
@@ -1250,7 +1257,7 @@ int apgx::RouteNets(apgx_lattice* lat, const CostOverride* ov, const apgx_net_qu
   uint32_t block = kThreads;
   size_t lds = TileLdsBytes(shape, d.layers);
   if (reg_tile) {
-    k.round = RegKernelFor(d.layers);
+    k.round = RegKernelFor(d.layers, ov != nullptr && ov->via_pays_cell);
     block = 64 * d.layers;
     lds = RegTileLdsBytes(d.layers);
   } else if (static_cast<uint64_t>(shape.tx) * shape.ty * d.layers > kMaxTileCells) {

@@ -1,0 +1,347 @@
+// prices.hip — candidate-allocator price epoch on cell resources.
+//
+// Restates the price update of PlanNegotiatedRegeneration
+// (src/allocator/negotiated_regeneration_plan.cc:697-790) with
+// NextPresentFactor (:430-442) for the dense lattice, one resource per cell in
+// cell order (the reference's canonical resource order becomes the cell
+// index):
+//   factor   = initial + epoch * increment                      (checked)
+//   entries  = cells with prior history != 0, plus overused cells
+//   overused cell: present = factor * overuse, history += inc * overuse,
+//                  total = present + history                    (checked)
+//   other entries: present = 0, total = history
+// followed by the reference's bounds in its order: hot-resource count, then
+// per overused cell (cell order) the entry bound and the three overflow
+// checks, then per entry (cell order) the value bound and the running
+// aggregate (overflow, then bound).  The first failing check is the outcome
+// and leaves the planes untouched, as the reference returns an error instead
+// of a snapshot.
+//
+// Kernels: one HBM pass computes per-cell values and per-block partials
+// (counts, clamped sums, first failing cells); the host walks the block
+// partials in order to find where a running count or sum first crosses its
+// bound, a one-block kernel pins the exact cell inside that block, and a
+// second pass writes the planes when every check passes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "apgx.h"
+#include "common.h"
+
+namespace apgx {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kPerThread = 4;
+constexpr int kBlockCells = kThreads * kPerThread;
+constexpr uint64_t kNone = ~0ull;
+constexpr uint64_t kSumClamp = 1ull << 33;  // per-entry clamp of block sums (values above fail the value bound)
+
+struct Params {
+  uint64_t factor, inc, max_value;
+  uint64_t cells;
+};
+
+struct CellEval {
+  bool over_cell;     // usage > capacity
+  bool entry;         // price entry after the update
+  bool new_entry;     // overused, no prior history
+  uint8_t arith;      // 0 ok, else APGX_PRICE_* overflow outcome of this overused cell
+  bool value_fail;
+  uint64_t present, history, total;
+};
+
+__host__ __device__ inline CellEval Evaluate(const Params& p, uint32_t usage, uint32_t cap, uint32_t hist) {
+  CellEval e{};
+  const uint64_t over = usage > cap ? usage - cap : 0u;
+  e.over_cell = over > 0;
+  e.history = hist;
+  e.entry = hist != 0 || e.over_cell;
+  e.new_entry = e.over_cell && hist == 0;
+  if (e.over_cell) {
+    uint64_t present = 0, add = 0, h = 0, t = 0;
+    if (__builtin_mul_overflow(p.factor, over, &present) || __builtin_mul_overflow(p.inc, over, &add)) {
+      e.arith = APGX_PRICE_MULTIPLY_OVERFLOW;
+    } else if (__builtin_add_overflow(static_cast<uint64_t>(hist), add, &h)) {
+      e.arith = APGX_PRICE_HISTORY_OVERFLOW;
+    } else if (__builtin_add_overflow(present, h, &t)) {
+      e.arith = APGX_PRICE_TOTAL_OVERFLOW;
+    }
+    e.present = present;
+    e.history = h;
+    e.total = t;
+  } else {
+    e.total = hist;
+  }
+  e.value_fail = e.entry && e.arith == 0 &&
+                 (e.present > p.max_value || e.history > p.max_value || e.total > p.max_value);
+  return e;
+}
+
+struct BlockPartial {
+  unsigned long long over_count, new_count, entry_count, sum_total;
+  unsigned long long first_arith, first_value;  // cell index or kNone
+  unsigned long long arith_kind, pad;
+};
+
+__device__ inline uint32_t CapAt(const uint8_t* cap, uint64_t c) { return cap ? cap[c] : 1u; }
+
+__global__ __launch_bounds__(kThreads) void PriceScan(Params p, const uint32_t* __restrict__ usage,
+                                                      const uint8_t* __restrict__ cap,
+                                                      const uint32_t* __restrict__ hist,
+                                                      BlockPartial* __restrict__ out) {
+  __shared__ unsigned long long s_over, s_new, s_entry, s_sum, s_arith, s_value;
+  if (threadIdx.x == 0) {
+    s_over = s_new = s_entry = s_sum = 0;
+    s_arith = s_value = kNone;
+  }
+  __syncthreads();
+  unsigned long long over = 0, nw = 0, ent = 0, sum = 0, fa = kNone, fv = kNone;
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kBlockCells;
+  for (int k = 0; k < kPerThread; ++k) {
+    const uint64_t c = base + static_cast<uint64_t>(k) * kThreads + threadIdx.x;  // coalesced
+    if (c >= p.cells) break;
+    const CellEval e = Evaluate(p, usage[c], CapAt(cap, c), hist[c]);
+    over += e.over_cell;
+    nw += e.new_entry;
+    ent += e.entry;
+    if (e.entry) sum += min(e.total, kSumClamp);
+    if (e.arith && c < fa) fa = c;
+    if (e.value_fail && c < fv) fv = c;
+  }
+  atomicAdd(&s_over, over);
+  atomicAdd(&s_new, nw);
+  atomicAdd(&s_entry, ent);
+  atomicAdd(&s_sum, sum);
+  atomicMin(&s_arith, fa);
+  atomicMin(&s_value, fv);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    BlockPartial b{};
+    b.over_count = s_over;
+    b.new_count = s_new;
+    b.entry_count = s_entry;
+    b.sum_total = s_sum;
+    b.first_arith = s_arith;
+    b.first_value = s_value;
+    b.arith_kind = 0;
+    out[blockIdx.x] = b;
+  }
+}
+
+// One thread walks one block's cells in order: result[0] = the cell where the
+// running count of new entries reaches `new_target` (kNone if never), result[1]
+// = the cell where `agg_start` plus the running total exceeds `agg_bound` or
+// overflows, result[2] = 1 if that was an overflow, result[3] = the arithmetic
+// outcome of cell `arith_cell` (if inside this block).
+__global__ void PriceLocate(Params p, const uint32_t* __restrict__ usage, const uint8_t* __restrict__ cap,
+                            const uint32_t* __restrict__ hist, uint64_t block, uint64_t new_target,
+                            uint64_t agg_start, uint64_t agg_bound, uint64_t arith_cell,
+                            unsigned long long* __restrict__ result) {
+  if (threadIdx.x != 0) return;
+  uint64_t seen_new = 0, agg = agg_start;
+  unsigned long long r0 = kNone, r1 = kNone, r2 = 0, r3 = 0;
+  const uint64_t begin = block * kBlockCells, end = min(p.cells, begin + kBlockCells);
+  for (uint64_t c = begin; c < end; ++c) {
+    const CellEval e = Evaluate(p, usage[c], CapAt(cap, c), hist[c]);
+    if (c == arith_cell) r3 = e.arith;
+    if (e.new_entry) {
+      if (r0 == kNone && seen_new == new_target) r0 = c;
+      ++seen_new;
+    }
+    if (e.entry && r1 == kNone) {
+      uint64_t next = 0;
+      if (__builtin_add_overflow(agg, e.total, &next)) {
+        r1 = c;
+        r2 = 1;
+      } else {
+        agg = next;
+        if (agg > agg_bound) r1 = c;
+      }
+    }
+  }
+  result[0] = r0;
+  result[1] = r1;
+  result[2] = r2;
+  result[3] = r3;
+}
+
+__global__ __launch_bounds__(kThreads) void PriceWrite(Params p, const uint32_t* __restrict__ usage,
+                                                       const uint8_t* __restrict__ cap,
+                                                       uint32_t* __restrict__ hist, uint32_t* __restrict__ present,
+                                                       uint32_t* __restrict__ total) {
+  const uint64_t c = blockIdx.x * static_cast<uint64_t>(kThreads) + threadIdx.x;
+  if (c >= p.cells) return;
+  const CellEval e = Evaluate(p, usage[c], CapAt(cap, c), hist[c]);
+  hist[c] = static_cast<uint32_t>(e.history);
+  if (present) present[c] = static_cast<uint32_t>(e.present);
+  if (total) total[c] = static_cast<uint32_t>(e.total);
+}
+
+struct Buffers {
+  uint32_t* usage = nullptr;
+  uint8_t* cap = nullptr;
+  uint32_t* hist = nullptr;
+  uint32_t* present = nullptr;
+  uint32_t* total = nullptr;
+  BlockPartial* part = nullptr;
+  unsigned long long* result = nullptr;
+  ~Buffers() {
+    for (void* q : {static_cast<void*>(usage), static_cast<void*>(cap), static_cast<void*>(hist),
+                    static_cast<void*>(present), static_cast<void*>(total), static_cast<void*>(part),
+                    static_cast<void*>(result)})
+      if (q) (void)hipFree(q);
+  }
+};
+
+}  // namespace
+}  // namespace apgx
+
+extern "C" void apgx_price_defaults(apgx_price_policy* policy, apgx_price_limits* limits) {
+  if (policy) {
+    policy->initial_present_factor = 100;
+    policy->present_factor_increment = 100;
+    policy->historical_price_increment = 10;
+  }
+  if (limits) {
+    limits->maximum_price_entries = 1000000;
+    limits->maximum_hot_resources = 1000000;
+    limits->maximum_price_value = 10000000;
+    limits->maximum_aggregate_price = 1000000000000ull;
+  }
+}
+
+extern "C" int apgx_price_update(uint64_t epoch_index, const apgx_price_policy* policy,
+                                 const apgx_price_limits* limits, uint64_t cells, const uint32_t* usage,
+                                 const uint8_t* capacity, uint32_t* history, uint32_t* present, uint32_t* total,
+                                 apgx_price_epoch* out) {
+  using namespace apgx;
+  if (out == nullptr || (cells && (usage == nullptr || history == nullptr)))
+    return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_price_update: null argument");
+  apgx_price_policy pol;
+  apgx_price_limits lim;
+  apgx_price_defaults(&pol, &lim);
+  if (policy) pol = *policy;
+  if (limits) lim = *limits;
+  if (lim.maximum_price_value > 0xFFFFFFFFull)
+    return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_price_update: maximum_price_value must fit the u32 price planes");
+  std::memset(out, 0, sizeof(*out));
+  out->epoch_index = epoch_index;
+  out->outcome_cell = kNone;
+  Params p{};
+  if (__builtin_mul_overflow(epoch_index, pol.present_factor_increment, &p.factor) ||
+      __builtin_add_overflow(p.factor, pol.initial_present_factor, &p.factor)) {
+    out->outcome = APGX_PRICE_PRESENT_FACTOR_OVERFLOW;
+    ClearError();
+    return APGX_OK;
+  }
+  out->present_factor = p.factor;
+  p.inc = pol.historical_price_increment;
+  p.max_value = lim.maximum_price_value;
+  p.cells = cells;
+  if (cells == 0) {
+    ClearError();
+    return APGX_OK;
+  }
+  const uint64_t blocks = (cells + kBlockCells - 1) / kBlockCells;
+  Buffers b;
+  APGX_HIP(hipMalloc(&b.usage, 4 * cells), "hipMalloc");
+  APGX_HIP(hipMalloc(&b.hist, 4 * cells), "hipMalloc");
+  APGX_HIP(hipMalloc(&b.part, sizeof(BlockPartial) * blocks), "hipMalloc");
+  APGX_HIP(hipMalloc(&b.result, 4 * sizeof(unsigned long long)), "hipMalloc");
+  APGX_HIP(hipMemcpy(b.usage, usage, 4 * cells, hipMemcpyHostToDevice), "usage upload");
+  APGX_HIP(hipMemcpy(b.hist, history, 4 * cells, hipMemcpyHostToDevice), "history upload");
+  if (capacity) {
+    APGX_HIP(hipMalloc(&b.cap, cells), "hipMalloc");
+    APGX_HIP(hipMemcpy(b.cap, capacity, cells, hipMemcpyHostToDevice), "capacity upload");
+  }
+  hipLaunchKernelGGL(PriceScan, dim3(static_cast<unsigned>(blocks)), dim3(kThreads), 0, nullptr, p, b.usage, b.cap,
+                     b.hist, b.part);
+  APGX_HIP(hipGetLastError(), "price scan launch");
+  std::vector<BlockPartial> part(blocks);
+  APGX_HIP(hipMemcpy(part.data(), b.part, sizeof(BlockPartial) * blocks, hipMemcpyDeviceToHost), "partial readback");
+
+  uint64_t over = 0, prior_entries = 0, entries = 0;
+  uint64_t first_arith = kNone, first_value = kNone;
+  for (const BlockPartial& q : part) {
+    over += q.over_count;
+    entries += q.entry_count;
+    prior_entries += q.entry_count - q.new_count;
+    first_arith = std::min<uint64_t>(first_arith, q.first_arith);
+    first_value = std::min<uint64_t>(first_value, q.first_value);
+  }
+  out->hot_resources = over;
+  out->price_entries = entries;
+  auto locate = [&](uint64_t block, uint64_t new_target, uint64_t agg_start, uint64_t arith_cell,
+                    unsigned long long r[4]) -> int {
+    hipLaunchKernelGGL(PriceLocate, dim3(1), dim3(64), 0, nullptr, p, b.usage, b.cap, b.hist, block, new_target,
+                       agg_start, lim.maximum_aggregate_price, arith_cell, b.result);
+    APGX_HIP(hipGetLastError(), "price locate launch");
+    APGX_HIP(hipMemcpy(r, b.result, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost), "locate readback");
+    return APGX_OK;
+  };
+  auto finish = [&](uint32_t outcome, uint64_t cell) {
+    out->outcome = outcome;
+    out->outcome_cell = cell;
+    ClearError();
+    return APGX_OK;
+  };
+  // 1. hot-resource bound (checked before any price, negotiated_regeneration_plan.cc:690-697)
+  if (over > lim.maximum_hot_resources) return finish(APGX_PRICE_HOT_RESOURCE_BOUND, kNone);
+  // 2. overused cells in order: entry bound (the k-th new entry with
+  //    prior + k >= bound), then that cell's overflow checks.
+  uint64_t entry_cell = kNone;
+  const uint64_t new_entries = entries - prior_entries;
+  if (new_entries > 0 && prior_entries + new_entries > lim.maximum_price_entries) {
+    const uint64_t k = lim.maximum_price_entries > prior_entries ? lim.maximum_price_entries - prior_entries : 0;
+    uint64_t seen = 0;
+    for (uint64_t i = 0; i < blocks; ++i) {
+      if (seen + part[i].new_count > k) {
+        unsigned long long r[4];
+        if (int rc = locate(i, k - seen, 0, kNone, r); rc != APGX_OK) return rc;
+        entry_cell = r[0];
+        break;
+      }
+      seen += part[i].new_count;
+    }
+  }
+  if (entry_cell != kNone && entry_cell <= first_arith) return finish(APGX_PRICE_ENTRY_BOUND, entry_cell);
+  if (first_arith != kNone) {
+    unsigned long long r[4];
+    if (int rc = locate(first_arith / kBlockCells, kNone, 0, first_arith, r); rc != APGX_OK) return rc;
+    return finish(static_cast<uint32_t>(r[3]), first_arith);
+  }
+  // 3. entries in order: value bound, then the running aggregate.
+  uint64_t agg = 0, agg_cell = kNone;
+  uint32_t agg_kind = 0;
+  for (uint64_t i = 0; i < blocks; ++i) {
+    uint64_t next = 0;
+    const bool ovf = __builtin_add_overflow(agg, static_cast<uint64_t>(part[i].sum_total), &next);
+    if (ovf || next > lim.maximum_aggregate_price) {
+      unsigned long long r[4];
+      if (int rc = locate(i, kNone, agg, kNone, r); rc != APGX_OK) return rc;
+      agg_cell = r[1];
+      agg_kind = r[2] ? APGX_PRICE_AGGREGATE_OVERFLOW : APGX_PRICE_AGGREGATE_BOUND;
+      break;
+    }
+    agg = next;
+  }
+  if (first_value != kNone && first_value <= agg_cell) return finish(APGX_PRICE_VALUE_BOUND, first_value);
+  if (agg_cell != kNone) return finish(agg_kind, agg_cell);
+  out->aggregate_price = agg;
+  // 4. every check passed: write the planes.
+  if (present) APGX_HIP(hipMalloc(&b.present, 4 * cells), "hipMalloc");
+  if (total) APGX_HIP(hipMalloc(&b.total, 4 * cells), "hipMalloc");
+  hipLaunchKernelGGL(PriceWrite, dim3(static_cast<unsigned>((cells + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                     nullptr, p, b.usage, b.cap, b.hist, b.present, b.total);
+  APGX_HIP(hipGetLastError(), "price write launch");
+  APGX_HIP(hipMemcpy(history, b.hist, 4 * cells, hipMemcpyDeviceToHost), "history readback");
+  if (present) APGX_HIP(hipMemcpy(present, b.present, 4 * cells, hipMemcpyDeviceToHost), "present readback");
+  if (total) APGX_HIP(hipMemcpy(total, b.total, 4 * cells, hipMemcpyDeviceToHost), "total readback");
+  out->outcome = APGX_PRICE_OK;
+  ClearError();
+  return APGX_OK;
+}

@@ -68,11 +68,12 @@ def train(x, y, layers, seed=1, **options):
     return net, report
 
 
-def seeded_run(net, spec, seed, gain, **negotiate_kw):
-    """One board: plain and learned arms (same policy), learned seeding timed."""
+def seeded_run(net, spec, seed, gain, plain=None, **negotiate_kw):
+    """One board: plain and learned arms (same policy), learned seeding timed.
+    `plain`: a summary of this board's plain run already measured (skips it)."""
     cost, nets, pins = board(spec, seed)
     lat = apgx.Lattice(cost, via_cost=spec["via"], cost_bytes=2)
-    plain = negotiate(lat, nets, pins, **negotiate_kw)
+    plain_summary = plain if plain is not None else summary(negotiate(lat, nets, pins, **negotiate_kw))
     prior_net = net
     if net.grid != (cost.shape[1] // TILE, cost.shape[2] // TILE):
         prior_net = apgx.Prior(net.layers, cost.shape[1] // TILE, cost.shape[2] // TILE)
@@ -84,7 +85,7 @@ def seeded_run(net, spec, seed, gain, **negotiate_kw):
     lat.close()
     if prior_net is not net:
         prior_net.close()
-    return {"seed": seed, "plain": summary(plain), "learned": summary(learned, seed_ms), "seed_ms": seed_ms,
+    return {"seed": seed, "plain": plain_summary, "learned": summary(learned, seed_ms), "seed_ms": seed_ms,
             "seed_history_sum": int(hist.sum(dtype=np.uint64))}
 
 

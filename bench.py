@@ -63,6 +63,7 @@ def parse_args():
                    "data (loaded if present, else collected and written)")
     p.add_argument("--collect-only", action="store_true", help="learned_prior: stop after collection")
     p.add_argument("--epochs", type=int, default=300, help="learned_prior: training epochs")
+    p.add_argument("--gains", type=str, default="1,4,16", help="learned_prior: decode gains to validate")
     p.add_argument("--full-size-boards", type=int, default=0,
                    help="learned_prior: also compare on this many six_layer_1536 boards")
     return p.parse_args()
@@ -178,10 +179,15 @@ def learned_prior(args):
     mean_map = y[:n_fit].mean(axis=0, keepdims=True)
     mean_mse = float(np.mean((mean_map - y[n_fit:]) ** 2))
     log(f"[bench] validation MSE {mse:.4f} (zero history {zero_mse:.4f}, mean map {mean_mse:.4f})")
-    gains = (0.5, 1.0, 2.0)
+    gains = tuple(float(g) for g in args.gains.split(","))
     val = {}
+    plain_val = {}
     for g in gains:
-        res = [ps.seeded_run(net, spec, s, g) for s in seeds[n_fit:]]
+        res = []
+        for s in seeds[n_fit:]:
+            r = ps.seeded_run(net, spec, s, g, plain=plain_val.get(s))
+            plain_val[s] = r["plain"]
+            res.append(r)
         val[g] = ps.compare(res)
         log(f"[bench] validation gain {g}: {val[g]}")
     def score(c):

@@ -314,6 +314,69 @@ int apgx_negotiation_routes(const apgx_negotiation* negotiation, uint32_t* route
 void apgx_negotiation_release(apgx_negotiation* negotiation);
 
 /* ------------------------------------------------------------------------ */
+/* Candidate-allocator price epoch                                          */
+/* ------------------------------------------------------------------------ */
+/* Replaces the price update of PlanNegotiatedRegeneration
+ * (src/allocator/negotiated_regeneration_plan.cc:697-790) and NextPresentFactor
+ * (:430-442) on cell resources: resource order is cell order, capacity is per
+ * cell (NULL: 1), usage per cell.  Policy and limit defaults are
+ * NegotiatedPriceUpdatePolicyV1 and NegotiatedRegenerationPlanLimits
+ * (include/apgar/allocator/negotiated_regeneration_plan.h:28-35, 130-140).
+ *   factor = initial_present_factor + epoch_index * present_factor_increment
+ *   overused cell: present = factor * overuse, history += increment * overuse
+ *   every entry (history != 0 or overused): total = present + history
+ * Checks, first failure wins, in the reference's order: the factor's
+ * arithmetic; hot resources (overused cells) <= maximum_hot_resources; per
+ * overused cell in cell order the entry bound (prior entries + new entries
+ * so far < maximum_price_entries) and the multiply / history / total
+ * overflows; per entry in cell order the value bound on present, history and
+ * total, then the running aggregate of totals (overflow, then bound).  A
+ * failing epoch reports its outcome and cell and leaves the planes untouched;
+ * the call still returns APGX_OK.  Planes are host memory, u32 per cell
+ * (maximum_price_value must fit in u32); the update runs on the device. */
+#define APGX_PRICE_OK 0u
+#define APGX_PRICE_PRESENT_FACTOR_OVERFLOW 1u
+#define APGX_PRICE_HOT_RESOURCE_BOUND 2u
+#define APGX_PRICE_ENTRY_BOUND 3u
+#define APGX_PRICE_MULTIPLY_OVERFLOW 4u
+#define APGX_PRICE_HISTORY_OVERFLOW 5u
+#define APGX_PRICE_TOTAL_OVERFLOW 6u
+#define APGX_PRICE_VALUE_BOUND 7u
+#define APGX_PRICE_AGGREGATE_OVERFLOW 8u
+#define APGX_PRICE_AGGREGATE_BOUND 9u
+
+typedef struct apgx_price_policy {
+  uint64_t initial_present_factor;      /* 100 */
+  uint64_t present_factor_increment;    /* 100 */
+  uint64_t historical_price_increment;  /* 10 */
+} apgx_price_policy;
+
+typedef struct apgx_price_limits {
+  uint64_t maximum_price_entries;       /* 1,000,000 */
+  uint64_t maximum_hot_resources;       /* 1,000,000 */
+  uint64_t maximum_price_value;         /* 10,000,000 */
+  uint64_t maximum_aggregate_price;     /* 1,000,000,000,000 */
+} apgx_price_limits;
+
+typedef struct apgx_price_epoch {
+  uint32_t outcome;                     /* APGX_PRICE_* */
+  uint32_t reserved;
+  uint64_t outcome_cell;                /* failing cell, or UINT64_MAX */
+  uint64_t epoch_index;
+  uint64_t present_factor;
+  uint64_t hot_resources;               /* overused cells */
+  uint64_t price_entries;               /* cells with a price after the update */
+  uint64_t aggregate_price;             /* sum of totals (when the epoch passes) */
+} apgx_price_epoch;
+
+void apgx_price_defaults(apgx_price_policy* policy, apgx_price_limits* limits);
+/* history: in/out; present and total (may be NULL): out. */
+int apgx_price_update(uint64_t epoch_index, const apgx_price_policy* policy,
+                      const apgx_price_limits* limits, uint64_t cells, const uint32_t* usage,
+                      const uint8_t* capacity, uint32_t* history, uint32_t* present, uint32_t* total,
+                      apgx_price_epoch* out);
+
+/* ------------------------------------------------------------------------ */
 /* Learned history-cost prior                                               */
 /* ------------------------------------------------------------------------ */
 /* No reference counterpart: the reference has no learned component.  A small
