@@ -68,12 +68,16 @@ def train(x, y, layers, seed=1, **options):
     return net, report
 
 
-def seeded_run(net, spec, seed, gain, plain=None, **negotiate_kw):
+def seeded_run(net, spec, seed, gain, plain=None, log=None, **negotiate_kw):
     """One board: plain and learned arms (same policy), learned seeding timed.
-    `plain`: a summary of this board's plain run already measured (skips it)."""
+    `plain`: a summary of this board's plain run already measured (skips it);
+    `log`: called with a line after each arm."""
     cost, nets, pins = board(spec, seed)
     lat = apgx.Lattice(cost, via_cost=spec["via"], cost_bytes=2)
     plain_summary = plain if plain is not None else summary(negotiate(lat, nets, pins, **negotiate_kw))
+    if log:
+        log(f"[prior] seed {seed} plain: {plain_summary['passes']} passes, termination "
+            f"{plain_summary['termination']}, {plain_summary['wall_ms']:.0f} ms")
     prior_net = net
     if net.grid != (cost.shape[1] // TILE, cost.shape[2] // TILE):
         prior_net = apgx.Prior(net.layers, cost.shape[1] // TILE, cost.shape[2] // TILE)
@@ -82,11 +86,32 @@ def seeded_run(net, spec, seed, gain, plain=None, **negotiate_kw):
     hist = prior_net.seed_history(lat, pins, gain=gain)
     seed_ms = (time.perf_counter() - t0) * 1e3
     learned = negotiate(lat, nets, pins, history=hist, **negotiate_kw)
+    if log:
+        log(f"[prior] seed {seed} learned: {learned['passes']} passes, termination "
+            f"{learned['termination']}, {learned['wall_ms'] + seed_ms:.0f} ms")
     lat.close()
     if prior_net is not net:
         prior_net.close()
     return {"seed": seed, "plain": plain_summary, "learned": summary(learned, seed_ms), "seed_ms": seed_ms,
             "seed_history_sum": int(hist.sum(dtype=np.uint64))}
+
+
+def self_seeded_run(spec, seed, **negotiate_kw):
+    """Upper bounds for any history prior on one board: the plain run, then the
+    same board seeded with that run's own final history, per cell and as the
+    64-cell tile mean (the resolution the network predicts)."""
+    cost, nets, pins = board(spec, seed)
+    lat = apgx.Lattice(cost, via_cost=spec["via"], cost_bytes=2)
+    plain = negotiate(lat, nets, pins, **negotiate_kw)
+    hist = plain["history"]
+    L, H, W = hist.shape
+    tiles = apgx.prior_targets(hist, TILE, UNIT)
+    tile_mean = np.repeat(np.repeat(np.rint(UNIT * np.expm1(tiles)), TILE, axis=1), TILE, axis=2)
+    tile_mean = tile_mean[:, :H, :W].astype(np.uint32)
+    per_cell = negotiate(lat, nets, pins, history=hist, **negotiate_kw)
+    per_tile = negotiate(lat, nets, pins, history=tile_mean, **negotiate_kw)
+    lat.close()
+    return {"seed": seed, "plain": summary(plain), "learned": summary(per_cell), "tile_mean": summary(per_tile)}
 
 
 def compare(results):

@@ -64,6 +64,10 @@ def parse_args():
     p.add_argument("--collect-only", action="store_true", help="learned_prior: stop after collection")
     p.add_argument("--epochs", type=int, default=300, help="learned_prior: training epochs")
     p.add_argument("--gains", type=str, default="1,4,16", help="learned_prior: decode gains to validate")
+    p.add_argument("--gain", type=float, default=0.0, help="learned_prior: use this decode gain (skips validation)")
+    p.add_argument("--upper-bound", type=int, default=0, metavar="N",
+                   help="learned_prior: only seed N held-out boards with their own plain-run history "
+                        "(per cell and per tile), the best any history prior could do")
     p.add_argument("--full-size-boards", type=int, default=0,
                    help="learned_prior: also compare on this many six_layer_1536 boards")
     return p.parse_args()
@@ -150,6 +154,19 @@ def learned_prior(args):
 
     spec = SIX_LAYER_512
     L = spec["layers"]
+    if args.upper_bound:
+        res = []
+        for s in range(200, 200 + args.upper_bound):
+            r = ps.self_seeded_run(spec, s)
+            res.append(r)
+            log(f"[bench] upper bound seed {s}: plain {r['plain']['passes']} passes "
+                f"{r['plain']['time_to_zero_overflow_ms']} ms, own history per cell {r['learned']['passes']} "
+                f"passes {r['learned']['time_to_zero_overflow_ms']} ms, per tile {r['tile_mean']['passes']} "
+                f"passes {r['tile_mean']['time_to_zero_overflow_ms']} ms")
+        tile = [dict(r, learned=r["tile_mean"]) for r in res]
+        print(json.dumps({"metric": "time to 0 overflow seeded with the board's own final history / plain",
+                          "boards": len(res), "per_cell": ps.compare(res), "per_tile": ps.compare(tile)}))
+        return
     seeds = list(range(args.train_seeds))
     path = Path(args.dataset) if args.dataset else None
     t0 = time.perf_counter()
@@ -179,10 +196,10 @@ def learned_prior(args):
     mean_map = y[:n_fit].mean(axis=0, keepdims=True)
     mean_mse = float(np.mean((mean_map - y[n_fit:]) ** 2))
     log(f"[bench] validation MSE {mse:.4f} (zero history {zero_mse:.4f}, mean map {mean_mse:.4f})")
-    gains = tuple(float(g) for g in args.gains.split(","))
+    gains = (args.gain,) if args.gain > 0 else tuple(float(g) for g in args.gains.split(","))
     val = {}
     plain_val = {}
-    for g in gains:
+    for g in gains if len(gains) > 1 else ():
         res = []
         for s in seeds[n_fit:]:
             r = ps.seeded_run(net, spec, s, g, plain=plain_val.get(s))
@@ -192,7 +209,7 @@ def learned_prior(args):
         log(f"[bench] validation gain {g}: {val[g]}")
     def score(c):
         return (c.get("learned_feasible", 0), -c.get("time_ratio", 9.0))
-    gain = max(gains, key=lambda g: score(val[g]))
+    gain = gains[0] if len(gains) == 1 else max(gains, key=lambda g: score(val[g]))
     held = []
     for s in range(200, 200 + args.heldout_seeds):
         r = ps.seeded_run(net, spec, s, gain)
@@ -203,7 +220,7 @@ def learned_prior(args):
     cmp = ps.compare(held)
     full = []
     for s in range(200, 200 + args.full_size_boards):
-        r = ps.seeded_run(net, SIX_LAYER_1536, s, gain)
+        r = ps.seeded_run(net, SIX_LAYER_1536, s, gain, log=log, maximum_passes=8)
         full.append(r)
         log(f"[bench] six_layer_1536 seed {s}: plain {r['plain']['passes']} passes "
             f"{r['plain']['time_to_zero_overflow_ms']} ms, learned {r['learned']['passes']} passes "
@@ -359,10 +376,10 @@ def main():
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         if big:
             line["cpu_baseline"] = cpu_nets_baseline(cost, spec["via"], nets, pins,
-                                                     args.cpu_sample or 16, threads)
+                                                     args.cpu_sample or 64, threads)
         else:
             line["cpu_baseline"] = cpu_pairs_baseline(cost, spec["via"], pairs,
-                                                      args.cpu_sample or 32, threads)
+                                                      args.cpu_sample or 512, threads)
     print(json.dumps(line))
     lat.close()
 
