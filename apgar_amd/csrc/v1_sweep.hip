@@ -1,4 +1,5 @@
-// v1_sweep.hip — heading-aware sweep over reference V1 device views.
+// v1_sweep.hip — heading-aware sweep and bucketed frontier over reference V1
+// device views.
 //
 // Semantics follow the reference batch sweep exactly (departures,
 // segmented run scans, round convergence, goal tie-break and lowest
@@ -27,6 +28,7 @@ constexpr uint64_t kInf64 = ~0ull;
 constexpr uint32_t kInvalid = APGX_V1_INVALID_INDEX;
 constexpr uint32_t kRunning = 0xFFu;
 constexpr uint32_t kChunkRounds = 8;  // kCandidateSweepChunkRounds
+constexpr uint32_t kFrontierChunkRounds = 32;  // kCandidateFrontierChunkRounds
 constexpr int kThreads = 256;
 
 struct V1Args {
@@ -196,6 +198,109 @@ __global__ __launch_bounds__(kThreads) void V1SweepKernel(V1Args a, uint32_t chu
   }
   if (turns) atomicAdd(reinterpret_cast<unsigned long long*>(&res.heading_turn_relaxations),
                        static_cast<unsigned long long>(turns));
+}
+
+// Bucketed-frontier generator (CandidateFrontierChunk,
+// gpu/cuda/planar_router.cu:715-845): each round one workgroup takes the open
+// state with the least (f, g, state) -- f = g + the admissible heuristic
+// max(|dx|, |dy|) * min(orthogonal, diagonal step) of
+// DeviceAdmissibleHeuristic (:590-608), winner order DeviceFrontierWinnerLess
+// (:610-619) -- closes it, finishes at the goal node, and otherwise relaxes
+// its legal edges (each counted as examined work).  The argmin is a
+// workgroup reduction; one thread relaxes the <= 8 edges.  closed flags live
+// in the query's departure slice (the sweep's scratch).
+__device__ __forceinline__ uint64_t FrontierHeuristic(const V1Args& a, const apgx_v1_query& q, uint32_t node,
+                                                     int64_t gx, int64_t gy) {
+  const int64_t x = a.nodes[node].lattice_x, y = a.nodes[node].lattice_y;
+  const uint64_t dx = x >= gx ? static_cast<uint64_t>(x) - static_cast<uint64_t>(gx)
+                              : static_cast<uint64_t>(gx) - static_cast<uint64_t>(x);
+  const uint64_t dy = y >= gy ? static_cast<uint64_t>(y) - static_cast<uint64_t>(gy)
+                              : static_cast<uint64_t>(gy) - static_cast<uint64_t>(y);
+  const uint64_t steps = dx > dy ? dx : dy;
+  const uint64_t orth = static_cast<uint64_t>(a.orth) + q.orthogonal_step_surcharge;
+  const uint64_t diag = static_cast<uint64_t>(a.diag) + q.diagonal_step_surcharge;
+  const uint64_t step = orth < diag ? orth : diag;
+  if (steps != 0 && step > (kInf64 - 1) / steps) return kInf64 - 1;
+  return steps * step;
+}
+
+__device__ __forceinline__ bool WinnerLess(uint64_t f, uint64_t g, uint32_t s, uint64_t bf, uint64_t bg,
+                                           uint32_t bs) {
+  return f < bf || (f == bf && (g < bg || (g == bg && s < bs)));
+}
+
+__global__ __launch_bounds__(kThreads) void V1FrontierKernel(V1Args a, uint32_t chunk, uint32_t round_budget) {
+  const uint32_t qi = blockIdx.x;
+  apgx_v1_result& res = a.results[qi];
+  const apgx_v1_query q = a.queries[qi];
+  const uint64_t states = a.node_count * 9;
+  uint64_t* lab = a.labels + static_cast<uint64_t>(qi) * states;
+  uint8_t* closed = reinterpret_cast<uint8_t*>(a.departures + static_cast<uint64_t>(qi) * a.node_count * 8);
+  const int64_t gx = a.nodes[q.goal_node].lattice_x, gy = a.nodes[q.goal_node].lattice_y;
+  __shared__ uint64_t wf[kThreads], wg[kThreads];
+  __shared__ uint32_t ws[kThreads];
+  __shared__ uint32_t s_run;
+  for (uint32_t k = 0; k < chunk; ++k) {
+    if (threadIdx.x == 0) s_run = (res.reserved[0] == 0 && res.rounds < round_budget) ? 1u : 0u;
+    __syncthreads();
+    if (s_run == 0) break;
+    uint64_t bf = kInf64, bg = kInf64;
+    uint32_t bs = kInvalid;
+    for (uint64_t st = threadIdx.x; st < states; st += kThreads) {
+      if (closed[st]) continue;
+      const uint64_t g = lab[st];
+      if (g == kInf64) continue;
+      uint64_t f;
+      if (!CheckedAdd(g, FrontierHeuristic(a, q, static_cast<uint32_t>(st / 9), gx, gy), &f)) f = kInf64 - 1;
+      if (WinnerLess(f, g, static_cast<uint32_t>(st), bf, bg, bs)) {
+        bf = f;
+        bg = g;
+        bs = static_cast<uint32_t>(st);
+      }
+    }
+    wf[threadIdx.x] = bf;
+    wg[threadIdx.x] = bg;
+    ws[threadIdx.x] = bs;
+    __syncthreads();
+    for (uint32_t stride = kThreads / 2; stride != 0; stride /= 2) {
+      if (threadIdx.x < stride &&
+          WinnerLess(wf[threadIdx.x + stride], wg[threadIdx.x + stride], ws[threadIdx.x + stride], wf[threadIdx.x],
+                     wg[threadIdx.x], ws[threadIdx.x])) {
+        wf[threadIdx.x] = wf[threadIdx.x + stride];
+        wg[threadIdx.x] = wg[threadIdx.x + stride];
+        ws[threadIdx.x] = ws[threadIdx.x + stride];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      res.rounds += 1;
+      const uint32_t sel = ws[0];
+      if (sel == kInvalid) {
+        res.completion = APGX_V1_DISCONNECTED;
+        res.reserved[0] = 1;
+      } else {
+        closed[sel] = 1;
+        const uint32_t node = sel / 9u, incoming = sel % 9u;
+        if (node == q.goal_node) {
+          res.goal_state = sel;
+          res.completion = APGX_V1_REACHED;
+          res.reserved[0] = 1;
+        } else {
+          const apgx_v1_node nd = a.nodes[node];
+          for (uint32_t dir = 0; dir < 8; ++dir) {
+            if ((nd.legal_edges & (1u << dir)) == 0) continue;
+            res.examined_work += 1;
+            uint64_t w, cand;
+            if (!Transition(a, q, node, dir, incoming, &w)) continue;
+            if (!CheckedAdd(wg[0], w, &cand)) continue;
+            uint64_t& slot = lab[nd.neighbors[dir] * 9ull + dir];
+            if (cand < slot) slot = cand;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
 }
 
 // Lowest predecessor state walk from the goal state (BuildQueryMajorCandidateSweepPaths).
@@ -414,12 +519,15 @@ extern "C" int apgx_v1_batch_execute(apgx_v1_view* view, const apgx_v1_batch_req
     return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_v1_batch_execute: current device differs from the upload device");
   if (req->schema_version != APGX_V1_SCHEMA_VERSION)
     return Fail(APGX_E_INTERNAL_INVARIANT, "candidate batch schema version mismatch");
-  if (req->generator != APGX_V1_GENERATOR_HEADING_AWARE_SWEEP)
-    return Fail(APGX_E_UNSUPPORTED, "only the heading-aware sweep generator is implemented on HIP");
+  if (req->generator != APGX_V1_GENERATOR_HEADING_AWARE_SWEEP && req->generator != APGX_V1_GENERATOR_BUCKETED_FRONTIER)
+    return Fail(APGX_E_UNSUPPORTED, "unknown candidate generator");
+  const bool frontier = req->generator == APGX_V1_GENERATOR_BUCKETED_FRONTIER;
   if (req->maximum_rounds == 0) return Fail(APGX_E_INTERNAL_INVARIANT, "candidate batch round budget must be positive");
   const uint64_t nn = view->node_count, S = nn * 9;
   if (S > req->maximum_workspace_states_per_query)
     return Fail(APGX_E_RESOURCE_EXHAUSTED, "query workspace exceeds maximum_workspace_states_per_query");
+  if (frontier && S > req->maximum_frontier_states_per_query)
+    return Fail(APGX_E_RESOURCE_EXHAUSTED, "query frontier exceeds maximum_frontier_states_per_query");
   // Request shape (ValidateCandidateBatchExecution, gpu/cuda/planar_router.cu:1731-1909).
   for (uint32_t i = 0; i < qn; ++i) {
     const apgx_v1_query& q = queries[i];
@@ -427,7 +535,7 @@ extern "C" int apgx_v1_batch_execute(apgx_v1_view* view, const apgx_v1_batch_req
         (i > 0 && queries[i - 1].query_id >= q.query_id) || q.workspace_owner != q.query_id ||
         q.start_node >= nn || q.goal_node >= nn || q.start_node == q.goal_node ||
         q.workspace_offset != static_cast<uint64_t>(i) * S || q.workspace_state_count != S ||
-        q.frontier_state_capacity != 0 || q.generator != req->generator ||
+        q.frontier_state_capacity != (frontier ? S : 0) || q.generator != req->generator ||
         static_cast<uint64_t>(q.policy_offset) + q.policy_count > pn)
       return Fail(APGX_E_INTERNAL_INVARIANT, "malformed candidate batch query record");
     for (uint32_t k = 1; k < q.policy_count; ++k)
@@ -471,7 +579,8 @@ extern "C" int apgx_v1_batch_execute(apgx_v1_view* view, const apgx_v1_batch_req
   t.batch_device_bytes = batch_bytes;
   t.workspace_capacity_device_bytes = batch_bytes;
   t.peak_device_bytes = batch_bytes + view->persistent_bytes;
-  t.chunk_rounds = kChunkRounds;
+  const uint32_t chunk_rounds = frontier ? kFrontierChunkRounds : kChunkRounds;
+  t.chunk_rounds = chunk_rounds;
   if (qn == 0) {
     *out = pend;
     ClearError();
@@ -522,12 +631,16 @@ extern "C" int apgx_v1_batch_execute(apgx_v1_view* view, const apgx_v1_batch_req
     hipLaunchKernelGGL(V1InitKernel, dim3(static_cast<uint32_t>((total_states + 255) / 256)), dim3(256), 0, stream, a, total_states);
     check(hipGetLastError(), "V1 init launch");
     ++launches;
+    if (frontier) check(hipMemsetAsync(a.departures, 0, dep_bytes, stream), "closed-state clear");
     std::vector<apgx_v1_result> snapshot(qn);
     while (status == APGX_OK && dispatched < req->maximum_rounds) {
       if (cancel != nullptr && *cancel != 0) { cancelled = true; break; }
-      const uint32_t chunk = std::min(kChunkRounds, req->maximum_rounds - dispatched);
-      hipLaunchKernelGGL(V1SweepKernel, dim3(qn), dim3(kThreads), 0, stream, a, chunk, req->maximum_rounds);
-      if (!check(hipGetLastError(), "V1 sweep launch")) break;
+      const uint32_t chunk = std::min(chunk_rounds, req->maximum_rounds - dispatched);
+      if (frontier)
+        hipLaunchKernelGGL(V1FrontierKernel, dim3(qn), dim3(kThreads), 0, stream, a, chunk, req->maximum_rounds);
+      else
+        hipLaunchKernelGGL(V1SweepKernel, dim3(qn), dim3(kThreads), 0, stream, a, chunk, req->maximum_rounds);
+      if (!check(hipGetLastError(), "V1 generator launch")) break;
       ++launches;
       dispatched += chunk;
       check(hipMemcpyAsync(snapshot.data(), a.results, qn * sizeof(apgx_v1_result), hipMemcpyDeviceToHost, stream), "status readback");

@@ -250,6 +250,7 @@ def make_query(i, start, goal, board: Board, *, batch_id=1, policy_offset=0, pol
     q.orthogonal_step_surcharge, q.diagonal_step_surcharge, q.bend_surcharge = surcharges
     q.maximum_rounds = maximum_rounds
     q.generator = generator
+    q.frontier_state_capacity = S if generator == GENERATOR_FRONTIER else 0
     return q
 
 

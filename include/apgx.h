@@ -676,8 +676,11 @@ int apgx_v1_view_upload(const apgx_v1_header* header, const apgx_v1_layer_range*
 void apgx_v1_view_release(apgx_v1_view* view);
 
 /* Replaces IPlanarRouteBackend::ExecuteCandidateBatch (include/apgar/gpu/planar_router.h:556-557)
- * for the heading-aware sweep generator.  `cancel` (may be NULL) is sampled
- * between round chunks; a nonzero value cancels the unfinished queries. */
+ * for both generators: the heading-aware sweep (8 rounds per chunk) and the
+ * bucketed frontier (CandidateFrontierChunk semantics, 32 rounds per chunk;
+ * queries carry frontier_state_capacity = represented states).  `cancel`
+ * (may be NULL) is sampled between round chunks; a nonzero value cancels the
+ * unfinished queries.  Results are read back in compact-path form for both. */
 int apgx_v1_batch_execute(apgx_v1_view* view, const apgx_v1_batch_request* request,
                           const apgx_v1_query* queries, uint32_t query_count,
                           const apgx_v1_policy_edge* policy_edges, uint64_t policy_edge_count,

@@ -60,6 +60,16 @@ int orc_v1_canonical(const orc_v1_node* nodes, uint64_t n, const orc_v1_profile*
                      int64_t gy, uint64_t* cost_out, uint32_t* states_out, uint64_t cap,
                      uint64_t* len_out);
 
+/* Batch frontier generator restatement (CandidateFrontierChunk): rounds,
+ * examined work, goal state and the lowest-predecessor reverse path on its
+ * labels, all in flattened numbering.  Returns 0 reached, 1 disconnected,
+ * 2 budget exhausted, -1 error. */
+int orc_v1_frontier(const orc_v1_node* nodes, uint64_t n, const orc_v1_profile* pr,
+                    const orc_v1_policy* po, uint32_t layer, int64_t sx, int64_t sy, int64_t gx,
+                    int64_t gy, uint32_t max_rounds, uint64_t* cost_out, uint32_t* goal_state_out,
+                    uint32_t* rounds_out, uint64_t* examined_out, uint32_t* states_out, uint64_t cap,
+                    uint64_t* len_out);
+
 #ifdef __cplusplus
 }
 #endif

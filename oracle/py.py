@@ -222,6 +222,11 @@ def _v1_setup():
     h.orc_v1_canonical.argtypes = [P(V1Node), C.c_uint64, P(V1Profile), P(V1Policy), C.c_uint32,
                                    C.c_int64, C.c_int64, C.c_int64, C.c_int64, P(C.c_uint64),
                                    P(C.c_uint32), C.c_uint64, P(C.c_uint64)]
+    h.orc_v1_frontier.restype = C.c_int
+    h.orc_v1_frontier.argtypes = [P(V1Node), C.c_uint64, P(V1Profile), P(V1Policy), C.c_uint32,
+                                  C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_uint32, P(C.c_uint64),
+                                  P(C.c_uint32), P(C.c_uint32), P(C.c_uint64), P(C.c_uint32), C.c_uint64,
+                                  P(C.c_uint64)]
     h._v1_ready = True
     return h
 
@@ -276,3 +281,22 @@ class V1:
         if rc < 0:
             raise RuntimeError("oracle v1 canonical error")
         return (None if rc == 1 else cost.value), list(states)[: plen.value]
+
+
+def v1_frontier(o: "V1", start, goal, policy=None, layer=0, max_rounds=100_000):
+    """Batch frontier restatement: dict(status 0 reached / 1 disconnected / 2
+    budget, cost, goal_state, rounds, examined_work, states) in flattened
+    numbering."""
+    h = _v1_setup()
+    cost, plen, examined = C.c_uint64(0), C.c_uint64(0), C.c_uint64(0)
+    goal_state, rounds = C.c_uint32(0), C.c_uint32(0)
+    states = (C.c_uint32 * (9 * o.n + 2))()
+    rc = h.orc_v1_frontier(o.nodes, o.n, C.byref(o.profile), C.byref(policy) if policy is not None else None,
+                           layer, start[0], start[1], goal[0], goal[1], max_rounds, C.byref(cost),
+                           C.byref(goal_state), C.byref(rounds), C.byref(examined), states, 9 * o.n + 2,
+                           C.byref(plen))
+    if rc < 0:
+        raise RuntimeError("oracle v1 frontier error")
+    return {"status": rc, "cost": cost.value if rc == 0 else None, "goal_state": goal_state.value,
+            "rounds": rounds.value, "examined_work": examined.value,
+            "states": list(states)[: plen.value] if rc == 0 else []}

@@ -419,3 +419,112 @@ done:
   free(flat);
   return rc;
 }
+
+/* Restatement of the batch frontier generator (CandidateFrontierChunk,
+ * gpu/cuda/planar_router.cu:715-845) in flattened numbering: each round takes
+ * the open finite state with the least (f, g, state), f = g + max(|dx|,|dy|) *
+ * min(orth + s_o, diag + s_d) (DeviceAdmissibleHeuristic :590-608, winner
+ * order :610-619), counts the round, closes it, stops at the goal node, and
+ * otherwise relaxes every legal direction (each counted as examined work;
+ * labels drop only on strict improvement).  Then the lowest-predecessor walk
+ * of :1084-1165 on the resulting labels.  Returns 0 reached, 1 disconnected,
+ * 2 round budget exhausted, -1 error. */
+int orc_v1_frontier(const orc_v1_node* nodes, uint64_t n, const orc_v1_profile* pr,
+                    const orc_v1_policy* po, uint32_t layer, int64_t sx, int64_t sy, int64_t gx,
+                    int64_t gy, uint32_t max_rounds, uint64_t* cost_out, uint32_t* goal_state_out,
+                    uint32_t* rounds_out, uint64_t* examined_out, uint32_t* states_out, uint64_t cap,
+                    uint64_t* len_out) {
+  uint32_t* order = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+  orc_v1_node* flat = (orc_v1_node*)malloc(sizeof(orc_v1_node) * (n ? n : 1));
+  if (!order || !flat || orc_v1_flatten_order(nodes, n, pr->tile_w, pr->tile_h, order)) {
+    free(order); free(flat); return -1;
+  }
+  for (uint64_t i = 0; i < n; ++i) flat[i] = nodes[order[i]];
+  lookup_t lk;
+  if (lookup_init(&lk, flat, n)) { free(order); free(flat); return -1; }
+  int64_t s = lookup(&lk, layer, sx, sy), g = lookup(&lk, layer, gx, gy);
+  uint64_t S = n * 9;
+  uint64_t* L = (uint64_t*)malloc(sizeof(uint64_t) * (S ? S : 1));
+  uint8_t* closed = (uint8_t*)calloc(S ? S : 1, 1);
+  int rc = 2;
+  uint32_t rounds = 0, goal_state = UINT32_MAX;
+  uint64_t examined = 0;
+  if (s < 0 || g < 0 || !L || !closed) { rc = -1; goto done; }
+  for (uint64_t i = 0; i < S; ++i) L[i] = INF64;
+  L[s * 9 + 8] = 0;
+  const uint64_t orth = (uint64_t)pr->orth + (po ? po->orth_s : 0);
+  const uint64_t diag = (uint64_t)pr->diag + (po ? po->diag_s : 0);
+  const uint64_t step = orth < diag ? orth : diag;
+  const int64_t goal_x = flat[g].x, goal_y = flat[g].y;
+  while (rounds < max_rounds) {
+    uint64_t bf = INF64, bg = INF64, bs = UINT64_MAX;
+    for (uint64_t st = 0; st < S; ++st) {
+      if (closed[st] || L[st] == INF64) continue;
+      const int64_t x = flat[st / 9].x, y = flat[st / 9].y;
+      const uint64_t dx = x >= goal_x ? (uint64_t)(x - goal_x) : (uint64_t)(goal_x - x);
+      const uint64_t dy = y >= goal_y ? (uint64_t)(y - goal_y) : (uint64_t)(goal_y - y);
+      const uint64_t steps = dx > dy ? dx : dy;
+      const uint64_t h = (steps != 0 && step > (INF64 - 1) / steps) ? INF64 - 1 : steps * step;
+      uint64_t f;
+      if (checked_add(L[st], h, &f)) f = INF64 - 1;
+      if (f < bf || (f == bf && (L[st] < bg || (L[st] == bg && st < bs)))) {
+        bf = f; bg = L[st]; bs = st;
+      }
+    }
+    ++rounds;
+    if (bs == UINT64_MAX) { rc = 1; break; }
+    closed[bs] = 1;
+    const uint64_t u = bs / 9;
+    const int inc = (int)(bs % 9);
+    if ((int64_t)u == g) { goal_state = (uint32_t)bs; rc = 0; break; }
+    for (int d = 0; d < 8; ++d) {
+      if (!(flat[u].mask & (1u << d))) continue;
+      ++examined;
+      uint64_t w, c;
+      if (transition(pr, po, flat[u].x, flat[u].y, d, inc, &w)) continue;
+      if (checked_add(bg, w, &c)) continue;
+      int64_t v = lookup(&lk, layer, flat[u].x + DX[d], flat[u].y + DY[d]);
+      if (v < 0) { rc = -1; goto done; }
+      uint64_t vs = (uint64_t)v * 9 + d;
+      if (c < L[vs]) L[vs] = c;
+    }
+  }
+  if (rc == 0) {
+    if (cost_out) *cost_out = L[goal_state];
+    uint64_t k = 0, cur = goal_state, start_state = (uint64_t)s * 9 + 8;
+    for (;;) {
+      if (k >= cap || k > S) { rc = -1; break; }
+      states_out[k++] = (uint32_t)cur;
+      if (cur == start_state) break;
+      uint64_t v = cur / 9;
+      int in = (int)(cur % 9);
+      if (in == 8) { rc = -1; break; }
+      int opp = (in + 4) % 8;
+      int64_t p = lookup(&lk, layer, flat[v].x + DX[opp], flat[v].y + DY[opp]);
+      if (p < 0) { rc = -1; break; }
+      uint64_t chosen = INF64;
+      for (int ph = 0; ph < 9; ++ph) {
+        if (ph == 8 && p != s) continue;
+        uint64_t ps = (uint64_t)p * 9 + ph;
+        if (L[ps] == INF64) continue;
+        uint64_t w, c;
+        if (transition(pr, po, flat[p].x, flat[p].y, in, ph, &w)) continue;
+        if (checked_add(L[ps], w, &c)) continue;
+        if (c == L[cur] && ps < chosen) chosen = ps;
+      }
+      if (chosen == INF64) { rc = -1; break; }
+      cur = chosen;
+    }
+    if (len_out) *len_out = k;
+  }
+done:
+  if (rounds_out) *rounds_out = rounds;
+  if (examined_out) *examined_out = examined;
+  if (goal_state_out) *goal_state_out = goal_state;
+  free(L);
+  free(closed);
+  free(lk.keys);
+  free(order);
+  free(flat);
+  return rc;
+}

@@ -86,3 +86,37 @@ def test_stable_hash_string_matches_replay_checksum():
     payload, _, last = text.rstrip(b"\n").rpartition(b"\n")
     assert last.startswith(b"checksum_fnv1a64=")
     assert vc.stable_hash_string(payload + b"\n") == int(last.split(b"=")[1])
+
+
+def test_frontier_restatement_hand_worked(oracle):
+    """Horizontal corridor of 11 nodes (orth 10, bend 3), start x=0, goal x=10:
+    the frontier closes the start state, the nine interior (node, E) states and
+    the goal's (node, E) state in order of f = 100 (ties broken by lower g),
+    examining 1 edge at the start and 2 (E and W) at each interior node:
+    11 rounds, 19 examined edges; a 4-round budget stops after 1 + 2 + 2 + 2."""
+    p, s, g = vc.known_answer_boards()["horizontal_corridor"]
+    _, o = _oracle_for(oracle, p)
+    f = oracle.v1_frontier(o, vc.lattice(p, s), vc.lattice(p, g))
+    assert (f["status"], f["cost"], f["rounds"], f["examined_work"]) == (0, 100, 11, 19)
+    assert f["goal_state"] == 10 * 9 + 0 and f["states"] == [9 * k for k in range(10, 0, -1)] + [8]
+    f = oracle.v1_frontier(o, vc.lattice(p, s), vc.lattice(p, g), max_rounds=4)
+    assert (f["status"], f["rounds"], f["examined_work"]) == (2, 4, 7)
+
+
+@pytest.mark.parametrize("name", sorted(vc.corpus()))
+def test_frontier_restatement_costs_and_paths(oracle, name):
+    """The frontier's costs are the reference's published optimal costs and its
+    lowest-predecessor paths equal the exact-label (Dijkstra) oracle's: with a
+    consistent heuristic every tight predecessor is closed before the goal."""
+    profile, start, goal = vc.corpus()[name]
+    _, o = _oracle_for(oracle, profile)
+    s, g = vc.lattice(profile, start), vc.lattice(profile, goal)
+    f = oracle.v1_frontier(o, s, g)
+    cost, states = o.canonical(s, g)
+    gold = GOLDEN[name]
+    want = gold.get("cuda_frontier_path_cost", gold.get("base_policy_scalar_cost"))
+    if name == "disconnected_fields":
+        assert f["status"] == 1 and cost is None
+    else:
+        assert f["status"] == 0 and f["cost"] == cost == want and f["states"] == states
+        assert f["rounds"] <= 9 * len(o.nodes)

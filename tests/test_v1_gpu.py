@@ -159,3 +159,103 @@ def test_cancel_and_budget(apgx, gpu):
     assert r[0].completion == v1.BUDGET_EXHAUSTED and r[0].rounds == 2
     assert tel.finalization_launch_count == 1
     view.close()
+
+
+def _frontier_check(b, o, s_xy, g_xy, r, p, states, policy=None):
+    f = vc_frontier(o, s_xy, g_xy, policy)
+    assert r.rounds == f["rounds"] and r.examined_work == f["examined_work"]
+    assert r.heading_turn_relaxations == 0
+    if f["status"] == 1:
+        assert r.completion == v1.DISCONNECTED and p.state_count == 0
+        return None
+    assert r.completion == v1.REACHED and r.goal_state == f["goal_state"]
+    got = states[p.state_offset:p.state_offset + p.state_count]
+    assert got == f["states"]
+    return got
+
+
+def vc_frontier(o, s_xy, g_xy, policy=None):
+    from oracle import py as oracle_py
+
+    return oracle_py.v1_frontier(o, s_xy, g_xy, policy)
+
+
+@pytest.mark.parametrize("name", sorted(vc.corpus()))
+def test_frontier_corpus_case(apgx, oracle, gpu, name):
+    """Bucketed-frontier generator: rounds, examined work, goal state and path
+    bit-exact against the frontier restatement; costs against the published
+    optimal costs; results identical to the sweep's."""
+    print(f"[case] v1 frontier corpus {name}", flush=True)
+    profile, start, goal = vc.corpus()[name]
+    nodes, b = board_for(profile)
+    s_xy, g_xy = vc.lattice(profile, start), vc.lattice(profile, goal)
+    s, g = b.node_index(*s_xy), b.node_index(*g_xy)
+    view = v1.View(b)
+    q = v1.make_query(0, s, g, b, generator=v1.GENERATOR_FRONTIER)
+    results, paths, states, tel = view.execute([q], generator=v1.GENERATOR_FRONTIER)
+    o = oracle.V1(nodes, profile.costs, profile.heading, profile.tile_w, profile.tile_h)
+    got = _frontier_check(b, o, s_xy, g_xy, results[0], paths[0], states)
+    sres, spaths, sstates, _ = view.execute([v1.make_query(0, s, g, b)])
+    if got is not None:
+        gold = GOLDEN[name]
+        want = gold.get("cuda_frontier_path_cost", gold.get("base_policy_scalar_cost"))
+        assert path_cost(b, got, profile.costs) == want
+        sp = spaths[0]
+        assert got == sstates[sp.state_offset:sp.state_offset + sp.state_count]
+        assert results[0].goal_state == sres[0].goal_state
+    assert tel.chunk_rounds == 32
+    view.close()
+
+
+def test_frontier_batch_with_policies(apgx, oracle, gpu):
+    """64 random queries with surcharges, penalties and a ban: every query's
+    telemetry and path against the restatement, costs equal to the sweep's."""
+    print("[case] v1 frontier batch with policies", flush=True)
+    profile, _, _ = vc.corpus()["dense_corridors"]
+    nodes, b = board_for(profile)
+    rng = np.random.default_rng(8)
+    n = len(b.nodes)
+    pen_edges = sorted({int(rng.integers(0, n)) * 8 + int(rng.integers(0, 4)) for _ in range(24)})
+    pen_edges = [e for e in pen_edges if b.nodes[e // 8].neighbors[e % 8] != 0xFFFFFFFF]
+    edges = [v1.PolicyEdge(e, 0, int(rng.integers(1, 60))) for e in pen_edges]
+    edges[3].adjustment = v1.BANNED
+    qs_f, qs_s, pairs = [], [], []
+    for i in range(64):
+        s, g = (int(v) for v in rng.choice(n, size=2, replace=False))
+        sur = tuple(int(v) for v in rng.integers(0, 5, 3))
+        pc = len(edges) if i % 2 else 0
+        pairs.append((s, g, sur, pc))
+        qs_f.append(v1.make_query(i, s, g, b, policy_count=pc, surcharges=sur, generator=v1.GENERATOR_FRONTIER))
+        qs_s.append(v1.make_query(i, s, g, b, policy_count=pc, surcharges=sur))
+    view = v1.View(b)
+    fr, fp, fs, _ = view.execute(qs_f, edges, generator=v1.GENERATOR_FRONTIER)
+    sr, sp, ss, _ = view.execute(qs_s, edges)
+    o = oracle.V1(nodes, profile.costs, profile.heading, profile.tile_w, profile.tile_h)
+    for i, (s, g, sur, pc) in enumerate(pairs):
+        s_xy = (b.nodes[s].lattice_x, b.nodes[s].lattice_y)
+        g_xy = (b.nodes[g].lattice_x, b.nodes[g].lattice_y)
+        bans, pens = [], []
+        for e in edges[:pc]:
+            node, d = divmod(e.physical_edge_index, 8)
+            key = (b.nodes[node].lattice_x, b.nodes[node].lattice_y, d)
+            (bans.append(key) if e.adjustment == v1.BANNED else pens.append((key, e.adjustment)))
+        pol = oracle.V1.policy(*sur, bans=bans, penalties=pens)
+        got = _frontier_check(b, o, s_xy, g_xy, fr[i], fp[i], fs, pol)
+        assert fr[i].completion == sr[i].completion
+        if got is not None:
+            sw = ss[sp[i].state_offset:sp[i].state_offset + sp[i].state_count]
+            adj = {e.physical_edge_index: e.adjustment for e in edges[:pc]}
+            assert path_cost(b, got, profile.costs, sur, adj) == path_cost(b, sw, profile.costs, sur, adj)
+    view.close()
+
+
+def test_frontier_budget(apgx, gpu):
+    profile, start, goal = vc.corpus()["high_turn_maze"]
+    _, b = board_for(profile)
+    s, g = b.node_index(*vc.lattice(profile, start)), b.node_index(*vc.lattice(profile, goal))
+    view = v1.View(b)
+    q = v1.make_query(0, s, g, b, maximum_rounds=40, generator=v1.GENERATOR_FRONTIER)
+    r, p, _, tel = view.execute([q], maximum_rounds=40, generator=v1.GENERATOR_FRONTIER)
+    assert r[0].completion == v1.BUDGET_EXHAUSTED and r[0].rounds == 40 and p[0].state_count == 0
+    assert tel.finalization_launch_count == 1 and tel.dispatched_rounds == 40
+    view.close()
