@@ -69,7 +69,7 @@ class RouteOptions(C.Structure):
                 ("rounds_per_sync", C.c_uint32), ("max_device_bytes", C.c_uint64),
                 ("use_graph", C.c_uint32), ("profile_rounds", C.c_uint32),
                 ("frontier_delta", C.c_uint32), ("max_sweeps", C.c_uint32),
-                ("no_goal_bound", C.c_uint32)]
+                ("no_goal_bound", C.c_uint32), ("sweep_schedule", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class RouteStats(C.Structure):
@@ -338,12 +338,12 @@ class Lattice:
 
     def route(self, nets, pins: np.ndarray, *, max_rounds=0, tile=(0, 0), rounds_per_sync=0,
               max_device_bytes=0, use_graph=0, profile_rounds=0, frontier_delta=0, max_sweeps=0,
-              goal_bound=True):
+              goal_bound=True, sweep_schedule=0):
         """Routes nets (a ctypes NetQuery array) and returns (routes, stats)."""
         pins = np.ascontiguousarray(pins, dtype=np.uint32)
         opts = RouteOptions(max_rounds, tile[0], tile[1], rounds_per_sync, max_device_bytes,
                             use_graph, profile_rounds, frontier_delta, max_sweeps,
-                            0 if goal_bound else 1)
+                            0 if goal_bound else 1, sweep_schedule)
         search = C.c_void_p()
         check(lib().apgx_route_nets(self.handle, nets, len(nets), u32p(pins), pins.size,
                                     C.byref(opts), C.byref(search)))

@@ -54,6 +54,7 @@ constexpr uint32_t kFromBelow = 0, kFromSouth = 1, kFromWest = 2, kFromEast = 3,
 constexpr uint32_t kSearching = 0xFFu;
 constexpr uint32_t kFinalWalk = 0xFEu;  // last pin settled; path written after the round loop
 constexpr int kThreads = 256;
+constexpr uint32_t kDefaultSweepSchedule = 2;  // apgx_route_options.sweep_schedule 0
 constexpr int kMaxTileCells = 16384;
 constexpr int kKeep = kMaxTileCells / kThreads;  // labels kept in registers per thread
 
@@ -122,6 +123,9 @@ struct SearchArgs {
   // 1: a via move into cell v weighs via + cost[v] (every entry into a cell pays
   // that cell's cost, as the allocator's per-cell congestion needs); 0: via.
   uint32_t via_pays_cell;
+  // Register tiles: every sweep_mode-th sweep (0, k, 2k, ...) is full (via
+  // exchange and stability vote); the others relax in-layer edges only.
+  uint32_t sweep_mode;
 };
 
 template <typename CostT>
@@ -491,7 +495,7 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
   uint32_t* s_t = smem;                       // L transpose/publish buffers, kRT * kRP each
   __shared__ uint32_t s_item[2];  // current / prefetched item (parity of the iteration)
   __shared__ uint32_t s_side[5];   // smallest changed border distance per side (W, E, S, N), tile min
-  __shared__ uint32_t s_changed[L];
+  __shared__ uint32_t s_changed[2 * L];  // stability votes, double-buffered by vote parity
   const int l = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));  // wave-uniform layer
   const uint32_t r = __builtin_amdgcn_readfirstlane(*a.round_ptr);
   const uint32_t count = a.counters[2 * r];
@@ -619,8 +623,9 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
     const uint32_t* below = s_t + (l > 0 ? l - 1 : l) * kRT * kRP;
     const uint32_t* above = s_t + (l + 1 < L ? l + 1 : l) * kRT * kRP;
 
-    uint32_t sweeps = 0;
+    uint32_t sweeps = 0, votes = 0;
     bool converged = false;
+    const uint32_t period = max(a.sweep_mode, 1u);
 #pragma unroll 1
     while (sweeps < a.max_sweeps) {
       // Keep the packed costs packed: without this the compiler unpacks all
@@ -666,6 +671,15 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
       WaveSync();
 #pragma unroll
       for (int y = 0; y < kRT; ++y) lab[y] = T[y * kRP + lane];
+      // In-layer-only sweep (sweeps not divisible by the period): no via
+      // exchange, no vote and no barrier.  Other waves read this wave's buffer
+      // only between the barriers of a full sweep, so overwriting it here is
+      // safe; the next full sweep's vote compares against the sums of the last
+      // full sweep and so covers this sweep's changes too.
+      if (sweeps % period != 0) {
+        ++sweeps;
+        continue;
+      }
       if (a.via != 0) {  // the host passes via = 0 for single-layer lattices
         __syncthreads();  // every layer published in its buffer
         if constexpr (PAYS) {
@@ -705,19 +719,25 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
       sum_hi = new_hi;
       bool any;
       {
+        // One barrier per vote: every wave finished its via reads before it,
+        // so the next sweep may overwrite the published buffers, and the
+        // next vote writes the other half of s_changed (a wave cannot reach
+        // that vote before every wave has read this one: the next full
+        // sweep's first barrier holds it).
+        uint32_t* flags = s_changed + (votes & 1u) * L;
         const bool wave_any = __any(lane_changed);
-        if (lane == 0) s_changed[l] = wave_any ? 1u : 0u;
+        if (lane == 0) flags[l] = wave_any ? 1u : 0u;
         __syncthreads();
         any = false;
 #pragma unroll
-        for (int k = 0; k < L; ++k) any |= s_changed[k] != 0;
+        for (int k = 0; k < L; ++k) any |= flags[k] != 0;
+        ++votes;
       }
       ++sweeps;
       if (!any) {
         converged = true;
         break;
       }
-      __syncthreads();  // flags and buffers are reused next sweep
     }
 
     // Write back through the tile buffer; detect border distance changes.
@@ -1410,6 +1430,7 @@ int apgx::RouteNets(apgx_lattice* lat, const CostOverride* ov, const apgx_net_qu
   a.min_edge = std::max(lat->min_edge, 1u);
   a.goal_cost = opt.no_goal_bound ? 0u : lat->min_cell_cost;
   a.via_pays_cell = (ov && ov->via_pays_cell) ? 1u : 0u;
+  a.sweep_mode = opt.sweep_schedule == 0 ? kDefaultSweepSchedule : opt.sweep_schedule;
   // Bucket width: about one tile width of distance by default; all-ones
   // disables ordering (every queued tile is processed every round).
   if (opt.frontier_delta == 0xFFFFFFFFu) {

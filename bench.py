@@ -52,6 +52,8 @@ def parse_args():
     p.add_argument("--delta", type=int, default=0, help="tile scheduling bucket width (0: default)")
     p.add_argument("--max-sweeps", type=int, default=0)
     p.add_argument("--no-goal-bound", action="store_true", help="order tiles by distance only")
+    p.add_argument("--sweep-schedule", type=int, default=0,
+                   help="register tiles: 1 full sweeps, 2 alternate in-layer-only and full (0: default)")
     p.add_argument("--graph", type=int, default=-1, help="1/0: replay round chunks as hipGraphs "
                    "(default: on for eight_layer_2048)")
     p.add_argument("--no-tune", action="store_true", help="eight_layer_2048: skip the autotuner")
@@ -273,6 +275,7 @@ def main():
     lat = apgx.Lattice(cost, via_cost=spec["via"], cost_bytes=2)
     opts = dict(tile=tuple(int(v) for v in args.tile.split("x")), frontier_delta=args.delta,
                 max_sweeps=args.max_sweeps, goal_bound=not args.no_goal_bound,
+                sweep_schedule=args.sweep_schedule,
                 use_graph=(1 if big else 0) if args.graph < 0 else args.graph)
     tune = None
     if big:

@@ -172,6 +172,10 @@ typedef struct apgx_route_options {
   uint32_t max_sweeps;        /* LDS/register sweeps per tile visit before re-queueing; 0: 64 */
   uint32_t no_goal_bound;     /* 1: order tiles by distance only; 0: distance plus the
                                  consistent lower bound to the nearest unconnected pin */
+  uint32_t sweep_schedule;    /* register tiles: k = one full sweep (via exchange and
+                                 stability test) every k sweeps, the rest in-layer only;
+                                 1 every sweep full; 0 default (2) */
+  uint32_t reserved;
 } apgx_route_options;
 
 typedef struct apgx_route_stats {

@@ -281,3 +281,33 @@ def test_autotune_picks_a_measured_candidate(apgx, gpu):
         assert (x.status, x.cost) == (y.status, y.cost)
         assert all(np.array_equal(p[2], q[2]) for p, q in zip(x.connections, y.connections))
     lat.close()
+
+
+@pytest.mark.parametrize("schedule,max_sweeps,seed", [(1, 0, 0), (2, 0, 0), (2, 0, 1), (2, 3, 2), (2, 1, 3),
+                                                      (1, 2, 4), (2, 0, 5), (3, 0, 1), (4, 5, 3), (3, 2, 0)])
+def test_register_tiles_sweep_schedules(apgx, oracle, gpu, schedule, max_sweeps, seed):
+    """Register-resident 64 x 64 x L tiles under both sweep schedules (every
+    sweep full, or in-layer-only sweeps alternating with full ones) and small
+    sweep caps that end visits on either kind of sweep: distances and canonical
+    paths bit-exact against the oracle, vias on."""
+    rng = np.random.default_rng(3000 + seed)
+    L = [2, 3, 4, 6, 4, 5][seed % 6]
+    H, W = [(130, 70), (64, 128), (100, 100), (70, 129), (128, 64), (65, 65)][seed % 6]
+    cost = random_lattice(rng, L, H, W, blocked=0.2, cmax=[8, 3, 16, 8, 5, 2][seed % 6])
+    via = [20, 1, 7, 3, 20, 5][seed % 6]
+    _say(f"[case] register tiles schedule={schedule} max_sweeps={max_sweeps} shape={(L, H, W)} via={via}")
+    free = np.flatnonzero(cost.reshape(-1))
+    pairs = rng.choice(free, size=(10, 2), replace=True)
+    pairs = pairs[pairs[:, 0] != pairs[:, 1]]
+    lat = apgx.Lattice(cost, via_cost=via)
+    routes, st = lat.route_pairs(pairs, tile=(64, 64), max_sweeps=max_sweeps, sweep_schedule=schedule)
+    assert (st.tile_w, st.tile_h) == (64, 64)
+    o = oracle.Dense(cost, via)
+    for (s, t), r in zip(pairs, routes):
+        dist = o.dijkstra([int(s)])
+        if dist[t] == UNR:
+            assert r.status == apgx.NET_DISCONNECTED
+            continue
+        assert r.status == apgx.NET_REACHED and r.cost == dist[t]
+        assert np.array_equal(r.connections[0][2], o.canonical_path(dist, int(t)))
+    lat.close()
