@@ -104,6 +104,10 @@ class CompiledNode(C.Structure):
                 ("reserved", U8 * 3)]
 
 
+class Validation(C.Structure):
+    _fields_ = [("valid", U32), ("query_index", U32), ("invariant", C.c_char * 64), ("detail", C.c_char * 128)]
+
+
 _SIGS = {
     "apgx_v1_build_board": (C.c_int, [P(CompiledProfile), P(CompiledNode), U64, P(C.c_void_p)]),
     "apgx_v1_board_sizes": (C.c_int, [C.c_void_p, P(U64), P(U64), P(U64), P(U64)]),
@@ -119,6 +123,9 @@ _SIGS = {
     "apgx_v1_batch_readback": (C.c_int, [C.c_void_p, P(Result), P(CompactPath), P(U32), U64,
                                          P(U64), P(BatchTelemetry)]),
     "apgx_v1_pending_release": (None, [C.c_void_p]),
+    "apgx_v1_validate_batch": (C.c_int, [P(Header), P(Node), U64, P(BatchRequest), P(Query), U32, P(PolicyEdge),
+                                         U64, P(Result), P(CompactPath), P(U32), U64, P(BatchTelemetry), P(U64),
+                                         P(Validation)]),
 }
 
 
@@ -257,3 +264,27 @@ def make_query(i, start, goal, board: Board, *, batch_id=1, policy_offset=0, pol
 def physical_edge_index(board: Board, x, y, direction, layer=0):
     """Canonical physical edge id node*8 + dir (< 4) of a canonical (x, y, dir) resource."""
     return board.node_index(x, y, layer) * 8 + direction
+
+
+def request_for(board: Board, *, batch_id=1, maximum_rounds=100_000, generator=GENERATOR_SWEEP,
+                maximum_device_bytes=(1 << 63)):
+    """The BatchRequest View.execute sends for these arguments."""
+    S = len(board.nodes) * 9
+    return BatchRequest(1, generator, batch_id, maximum_rounds, 0, S, S, maximum_device_bytes, 1 << 63)
+
+
+def validate(board: Board, request, queries, policy_edges, results, paths, states, telemetry):
+    """Host trust-boundary check of a readback (apgx_v1_validate_batch).
+    Returns (valid, query_index, invariant id, detail, path costs)."""
+    n = len(queries)
+    qarr = (Query * max(n, 1))(*queries)
+    parr = (PolicyEdge * max(len(policy_edges), 1))(*policy_edges)
+    rarr = (Result * max(n, 1))(*results)
+    carr = (CompactPath * max(n, 1))(*paths)
+    sarr = (U32 * max(len(states), 1))(*states)
+    costs = (U64 * max(n, 1))()
+    out = Validation()
+    check(_lib().apgx_v1_validate_batch(C.byref(board.header), board.nodes, len(board.nodes), C.byref(request),
+                                        qarr, n, parr, len(policy_edges), rarr, carr, sarr, len(states),
+                                        C.byref(telemetry), costs, C.byref(out)))
+    return (bool(out.valid), out.query_index, out.invariant.decode(), out.detail.decode(), list(costs)[:n])

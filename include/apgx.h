@@ -697,6 +697,33 @@ int apgx_v1_batch_readback(apgx_v1_pending* pending, apgx_v1_result* results,
                            uint64_t* states_used, apgx_v1_batch_telemetry* telemetry);
 void apgx_v1_pending_release(apgx_v1_pending* pending);
 
+/* Host trust boundary: re-checks an untrusted compact readback before use, as
+ * RouteCandidateBatchWithPreparedPreflight does (src/gpu/planar_router.cc:
+ * 969-1131 ValidateAndReconstructCompactGpuPath, 1808-1955 query and batch
+ * telemetry, 629-667 CPU confirmation of a reported disconnection).  Checks, in
+ * order: generator; batch telemetry identities (chunk size, one status readback
+ * per chunk, launches = init + chunks + 3 path kernels); byte accounting; per
+ * query the header echo, completion, rounds / examined work, path bounds, a
+ * disconnection the CPU search contradicts, endpoints, cycles, headings,
+ * edges and banned resources.  `valid` = 0 names the first failing query and
+ * the reference's invariant identifier (e.g. "gpu.batch.compact_path.cycle.v1");
+ * path_costs (may be NULL) receives each reached path's recomputed cost
+ * (UINT64_MAX otherwise).  The call returns APGX_OK unless an argument is null. */
+typedef struct apgx_v1_validation {
+  uint32_t valid;
+  uint32_t query_index;                    /* APGX_V1_INVALID_INDEX: batch-level */
+  char invariant[64];
+  char detail[128];
+} apgx_v1_validation;
+
+int apgx_v1_validate_batch(const apgx_v1_header* header, const apgx_v1_node* nodes, uint64_t node_count,
+                           const apgx_v1_batch_request* request, const apgx_v1_query* queries,
+                           uint32_t query_count, const apgx_v1_policy_edge* policy_edges,
+                           uint64_t policy_edge_count, const apgx_v1_result* results,
+                           const apgx_v1_compact_path* paths, const uint32_t* states,
+                           uint64_t state_count, const apgx_v1_batch_telemetry* telemetry,
+                           uint64_t* path_costs, apgx_v1_validation* out);
+
 /* ------------------------------------------------------------------------ */
 /* Synthetic lattice generators (deterministic xoshiro256** streams)         */
 /* ------------------------------------------------------------------------ */
