@@ -9,6 +9,11 @@
 // launch (departure pass, barrier, run pass, barrier-or for convergence), so a
 // chunk costs one launch instead of four launches per round, and the host
 // samples cancellation and status once per chunk.
+//
+// Readback follows the reference's per-generator contract: the sweep returns
+// compact canonical paths; the bucketed frontier returns its full query-major
+// workspace (labels, lowest tight predecessors and the two ownership stamps
+// of every state), from which the host reconstructs and re-validates routes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -27,8 +32,6 @@ namespace {
 constexpr uint64_t kInf64 = ~0ull;
 constexpr uint32_t kInvalid = APGX_V1_INVALID_INDEX;
 constexpr uint32_t kRunning = 0xFFu;
-constexpr uint32_t kChunkRounds = 8;  // kCandidateSweepChunkRounds
-constexpr uint32_t kFrontierChunkRounds = 32;  // kCandidateFrontierChunkRounds
 constexpr int kThreads = 256;
 
 struct V1Args {
@@ -42,7 +45,11 @@ struct V1Args {
   const apgx_v1_policy_edge* policy;
   uint32_t query_count;
   uint64_t* labels;      // query-major, 9 * node_count per query
-  uint64_t* departures;  // query-major, 8 * node_count per query (path scratch afterwards)
+  uint64_t* departures;  // sweep: query-major, 8 * node_count per query (path scratch afterwards)
+  uint32_t* predecessors;        // frontier: lowest tight predecessor of every state
+  uint64_t* state_owners;        // frontier: ownership stamp of every label slot
+  uint64_t* predecessor_owners;  // frontier: ownership stamp of every predecessor slot
+  uint8_t* closed;               // frontier: closed flag of every state
   apgx_v1_result* results;
   apgx_v1_compact_path* paths;
   uint32_t* packed;
@@ -101,6 +108,24 @@ __global__ void V1InitKernel(V1Args a, uint64_t total_states) {
   const uint32_t qi = static_cast<uint32_t>(slot / states);
   const uint64_t local = slot - qi * states;
   a.labels[slot] = local == static_cast<uint64_t>(a.queries[qi].start_node) * 9 + 8 ? 0 : kInf64;
+  if (a.state_owners != nullptr) {  // frontier workspace (InitializeCandidateBatch)
+    const uint64_t owner = a.queries[qi].workspace_owner;
+    a.state_owners[slot] = owner;
+    a.predecessor_owners[slot] = owner;
+    a.predecessors[slot] = kInvalid;
+    a.closed[slot] = 0;
+  }
+}
+
+// Completion of the queries still running when the round loop stops: budget
+// exhausted, or cancelled at a chunk boundary (FinalizeCandidateBatchStatus).
+__global__ void V1FinalizeKernel(V1Args a, uint32_t completion) {
+  const uint32_t qi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (qi >= a.query_count) return;
+  apgx_v1_result& res = a.results[qi];
+  if (res.reserved[0] != 0) return;
+  res.completion = static_cast<uint8_t>(completion);
+  res.reserved[0] = 1;
 }
 
 // Up to `chunk` rounds for one query per workgroup.
@@ -207,8 +232,11 @@ __global__ __launch_bounds__(kThreads) void V1SweepKernel(V1Args a, uint32_t chu
 // DeviceAdmissibleHeuristic (:590-608), winner order DeviceFrontierWinnerLess
 // (:610-619) -- closes it, finishes at the goal node, and otherwise relaxes
 // its legal edges (each counted as examined work).  The argmin is a
-// workgroup reduction; one thread relaxes the <= 8 edges.  closed flags live
-// in the query's departure slice (the sweep's scratch).
+// workgroup reduction; one thread relaxes the <= 8 edges.  Every close and
+// every label write first checks the slot's ownership stamp (the query's
+// workspace_owner, written at init); a foreign stamp skips the write and
+// zeroes the result's workspace_owner, so the host rejects the query instead
+// of trusting contaminated state (DeviceClaimCandidateState, :620-637).
 __device__ __forceinline__ uint64_t FrontierHeuristic(const V1Args& a, const apgx_v1_query& q, uint32_t node,
                                                      int64_t gx, int64_t gy) {
   const int64_t x = a.nodes[node].lattice_x, y = a.nodes[node].lattice_y;
@@ -235,7 +263,8 @@ __global__ __launch_bounds__(kThreads) void V1FrontierKernel(V1Args a, uint32_t 
   const apgx_v1_query q = a.queries[qi];
   const uint64_t states = a.node_count * 9;
   uint64_t* lab = a.labels + static_cast<uint64_t>(qi) * states;
-  uint8_t* closed = reinterpret_cast<uint8_t*>(a.departures + static_cast<uint64_t>(qi) * a.node_count * 8);
+  uint8_t* closed = a.closed + static_cast<uint64_t>(qi) * states;
+  const uint64_t* own = a.state_owners + static_cast<uint64_t>(qi) * states;
   const int64_t gx = a.nodes[q.goal_node].lattice_x, gy = a.nodes[q.goal_node].lattice_y;
   __shared__ uint64_t wf[kThreads], wg[kThreads];
   __shared__ uint32_t ws[kThreads];
@@ -278,6 +307,10 @@ __global__ __launch_bounds__(kThreads) void V1FrontierKernel(V1Args a, uint32_t 
       if (sel == kInvalid) {
         res.completion = APGX_V1_DISCONNECTED;
         res.reserved[0] = 1;
+      } else if (own[sel] != q.workspace_owner) {
+        res.workspace_owner = 0;
+        res.completion = APGX_V1_DISCONNECTED;
+        res.reserved[0] = 1;
       } else {
         closed[sel] = 1;
         const uint32_t node = sel / 9u, incoming = sel % 9u;
@@ -293,14 +326,59 @@ __global__ __launch_bounds__(kThreads) void V1FrontierKernel(V1Args a, uint32_t 
             uint64_t w, cand;
             if (!Transition(a, q, node, dir, incoming, &w)) continue;
             if (!CheckedAdd(wg[0], w, &cand)) continue;
-            uint64_t& slot = lab[nd.neighbors[dir] * 9ull + dir];
-            if (cand < slot) slot = cand;
+            const uint64_t t = nd.neighbors[dir] * 9ull + dir;
+            if (cand < lab[t]) {
+              if (own[t] == q.workspace_owner) lab[t] = cand; else res.workspace_owner = 0;
+            }
           }
         }
       }
     }
     __syncthreads();
   }
+}
+
+// Frontier readback: the lowest tight predecessor of every finite state from
+// the final labels, one thread per (query, state) slot
+// (SelectCandidateBatchPredecessors, gpu/cuda/planar_router.cu:1021-1082).
+__global__ void V1SelectPredecessorsKernel(V1Args a, uint64_t total_states) {
+  const uint64_t slot = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (slot >= total_states) return;
+  const uint64_t states = a.node_count * 9;
+  const uint32_t qi = static_cast<uint32_t>(slot / states);
+  const uint64_t base = static_cast<uint64_t>(qi) * states;
+  const uint32_t state = static_cast<uint32_t>(slot - base);
+  const apgx_v1_query q = a.queries[qi];
+  apgx_v1_result& res = a.results[qi];
+  if (a.state_owners[slot] != q.workspace_owner || a.predecessor_owners[slot] != q.workspace_owner) {
+    res.workspace_owner = 0;
+    return;
+  }
+  const uint32_t node = state / 9u, inc = state % 9u;
+  const uint32_t start_state = q.start_node * 9u + 8u;
+  if (state == start_state || inc == 8u || a.labels[slot] == kInf64) {
+    a.predecessors[slot] = kInvalid;
+    return;
+  }
+  const uint32_t pred = a.nodes[node].neighbors[(inc + 4u) % 8u];
+  if (pred == kInvalid) {
+    a.predecessors[slot] = kInvalid;
+    return;
+  }
+  uint32_t best = kInvalid;
+  for (uint32_t ph = 0; ph < 9; ++ph) {
+    if (ph == 8u && pred != q.start_node) continue;
+    const uint32_t ps = pred * 9u + ph;
+    if (a.state_owners[base + ps] != q.workspace_owner) {
+      a.predecessors[slot] = kInvalid;
+      res.workspace_owner = 0;
+      return;
+    }
+    uint64_t w, c;
+    if (!Transition(a, q, pred, inc, ph, &w)) continue;
+    if (CheckedAdd(a.labels[base + ps], w, &c) && c == a.labels[slot] && ps < best) best = ps;
+  }
+  a.predecessors[slot] = best;
 }
 
 // Lowest predecessor state walk from the goal state (BuildQueryMajorCandidateSweepPaths).
@@ -379,9 +457,15 @@ struct apgx_v1_view {
 
 struct apgx_v1_pending {
   uint32_t query_count = 0;
+  uint32_t readback_kind = APGX_V1_READBACK_COMPACT_PATHS;
+  uint64_t states_per_query = 0;
   std::vector<apgx_v1_result> results;
+  std::vector<apgx_v1_query_envelope> envelopes;  // workspace slice + query telemetry beside each header
   std::vector<apgx_v1_compact_path> paths;
   std::vector<uint32_t> states;
+  // Full-workspace readback (bucketed frontier), query-major.
+  std::vector<uint64_t> labels, state_owners, predecessor_owners;
+  std::vector<uint32_t> predecessors;
   apgx_v1_batch_telemetry telemetry{};
 };
 
@@ -542,14 +626,16 @@ extern "C" int apgx_v1_batch_execute(apgx_v1_view* view, const apgx_v1_batch_req
       if (policy[q.policy_offset + k - 1].physical_edge_index >= policy[q.policy_offset + k].physical_edge_index)
         return Fail(APGX_E_INTERNAL_INVARIANT, "query policy slice is not strictly sorted by physical edge");
   }
-  const uint64_t label_bytes = static_cast<uint64_t>(qn) * S * 8, dep_bytes = static_cast<uint64_t>(qn) * nn * 8 * 8;
-  const uint64_t batch_bytes = label_bytes + dep_bytes + static_cast<uint64_t>(qn) * (sizeof(apgx_v1_query) +
-                               sizeof(apgx_v1_result) + sizeof(apgx_v1_compact_path) + 4 * 9 * nn) + pn * 16;
+  const uint64_t total_states = static_cast<uint64_t>(qn) * S;
+  const BatchLayout lay = LayoutBatch(frontier, qn, pn, nn);
+  const uint64_t batch_bytes = lay.total;
   if (batch_bytes + view->persistent_bytes > req->maximum_device_bytes)
     return Fail(APGX_E_RESOURCE_EXHAUSTED, "candidate batch exceeds maximum_device_bytes");
   auto* pend = new (std::nothrow) apgx_v1_pending();
   if (pend == nullptr) return Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_v1_batch_execute: host allocation");
   pend->query_count = qn;
+  pend->states_per_query = S;
+  pend->readback_kind = frontier ? APGX_V1_READBACK_FULL_WORKSPACE : APGX_V1_READBACK_COMPACT_PATHS;
   pend->results.resize(qn);
   pend->paths.resize(qn);
   for (uint32_t i = 0; i < qn; ++i) {
@@ -579,7 +665,9 @@ extern "C" int apgx_v1_batch_execute(apgx_v1_view* view, const apgx_v1_batch_req
   t.batch_device_bytes = batch_bytes;
   t.workspace_capacity_device_bytes = batch_bytes;
   t.peak_device_bytes = batch_bytes + view->persistent_bytes;
-  const uint32_t chunk_rounds = frontier ? kFrontierChunkRounds : kChunkRounds;
+  GeneratorShape shape{};
+  DescribeGenerator(req->generator, &shape);
+  const uint32_t chunk_rounds = shape.chunk_rounds;
   t.chunk_rounds = chunk_rounds;
   if (qn == 0) {
     *out = pend;
@@ -599,23 +687,23 @@ extern "C" int apgx_v1_batch_execute(apgx_v1_view* view, const apgx_v1_batch_req
     if (status == APGX_OK && err != hipSuccess) status = HipFail(err, op);
     return status == APGX_OK;
   };
-  const uint64_t off_q = 0, off_pol = off_q + qn * sizeof(apgx_v1_query);
-  const uint64_t off_res = (off_pol + std::max<uint64_t>(pn, 1) * 16 + 255) & ~255ull;
-  const uint64_t off_paths = (off_res + qn * sizeof(apgx_v1_result) + 255) & ~255ull;
-  const uint64_t off_packed = (off_paths + qn * sizeof(apgx_v1_compact_path) + 255) & ~255ull;
-  const uint64_t off_lab = (off_packed + static_cast<uint64_t>(qn) * S * 4 + 255) & ~255ull;
-  const uint64_t off_dep = (off_lab + label_bytes + 255) & ~255ull;
-  const uint64_t total = off_dep + dep_bytes;
-  check(hipMalloc(&block, total), "candidate batch workspace allocation");
+  check(hipMalloc(&block, lay.total), "candidate batch workspace allocation");
   if (status != APGX_OK) { delete pend; return status; }
   char* base = static_cast<char*>(block);
-  a.queries = reinterpret_cast<apgx_v1_query*>(base + off_q);
-  a.policy = reinterpret_cast<apgx_v1_policy_edge*>(base + off_pol);
-  a.results = reinterpret_cast<apgx_v1_result*>(base + off_res);
-  a.paths = reinterpret_cast<apgx_v1_compact_path*>(base + off_paths);
-  a.packed = reinterpret_cast<uint32_t*>(base + off_packed);
-  a.labels = reinterpret_cast<uint64_t*>(base + off_lab);
-  a.departures = reinterpret_cast<uint64_t*>(base + off_dep);
+  a.queries = reinterpret_cast<apgx_v1_query*>(base + lay.queries);
+  a.policy = reinterpret_cast<apgx_v1_policy_edge*>(base + lay.policy);
+  a.results = reinterpret_cast<apgx_v1_result*>(base + lay.results);
+  a.labels = reinterpret_cast<uint64_t*>(base + lay.labels);
+  if (frontier) {
+    a.predecessors = reinterpret_cast<uint32_t*>(base + lay.predecessors);
+    a.state_owners = reinterpret_cast<uint64_t*>(base + lay.state_owners);
+    a.predecessor_owners = reinterpret_cast<uint64_t*>(base + lay.predecessor_owners);
+    a.closed = reinterpret_cast<uint8_t*>(base + lay.closed);
+  } else {
+    a.paths = reinterpret_cast<apgx_v1_compact_path*>(base + lay.paths);
+    a.packed = reinterpret_cast<uint32_t*>(base + lay.packed);
+    a.departures = reinterpret_cast<uint64_t*>(base + lay.departures);
+  }
   check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
   check(hipEventCreate(&e0), "hipEventCreate");
   check(hipEventCreate(&e1), "hipEventCreate");
@@ -623,15 +711,14 @@ extern "C" int apgx_v1_batch_execute(apgx_v1_view* view, const apgx_v1_batch_req
   if (pn) check(hipMemcpyAsync(const_cast<apgx_v1_policy_edge*>(a.policy), policy, pn * 16, hipMemcpyHostToDevice, stream), "policy upload");
   check(hipMemcpyAsync(a.results, pend->results.data(), qn * sizeof(apgx_v1_result), hipMemcpyHostToDevice, stream), "result init");
   uint64_t launches = 0, syncs = 0;
-  uint32_t dispatched = 0;
-  bool cancelled = false;
+  uint32_t dispatched = 0, finalization = 0;
+  bool cancelled = false, running = true;
   if (status == APGX_OK) {
     check(hipEventRecord(e0, stream), "hipEventRecord");
-    const uint64_t total_states = static_cast<uint64_t>(qn) * S;
-    hipLaunchKernelGGL(V1InitKernel, dim3(static_cast<uint32_t>((total_states + 255) / 256)), dim3(256), 0, stream, a, total_states);
+    const uint32_t state_blocks = static_cast<uint32_t>((total_states + 255) / 256);
+    hipLaunchKernelGGL(V1InitKernel, dim3(state_blocks), dim3(256), 0, stream, a, total_states);
     check(hipGetLastError(), "V1 init launch");
     ++launches;
-    if (frontier) check(hipMemsetAsync(a.departures, 0, dep_bytes, stream), "closed-state clear");
     std::vector<apgx_v1_result> snapshot(qn);
     while (status == APGX_OK && dispatched < req->maximum_rounds) {
       if (cancel != nullptr && *cancel != 0) { cancelled = true; break; }
@@ -646,52 +733,72 @@ extern "C" int apgx_v1_batch_execute(apgx_v1_view* view, const apgx_v1_batch_req
       check(hipMemcpyAsync(snapshot.data(), a.results, qn * sizeof(apgx_v1_result), hipMemcpyDeviceToHost, stream), "status readback");
       check(hipStreamSynchronize(stream), "hipStreamSynchronize");
       ++syncs;
-      bool running = false;
+      running = false;
       for (const apgx_v1_result& r : snapshot) running |= r.reserved[0] == 0;
       if (!running) break;
     }
+    if (status == APGX_OK && running) {
+      const uint32_t completion = cancelled ? APGX_V1_CANCELLED : APGX_V1_BUDGET_EXHAUSTED;
+      hipLaunchKernelGGL(V1FinalizeKernel, dim3((qn + 63) / 64), dim3(64), 0, stream, a, completion);
+      check(hipGetLastError(), "V1 finalization launch");
+      ++launches;
+      finalization = 1;
+    }
     if (status == APGX_OK) {
-      hipLaunchKernelGGL(V1PathKernel, dim3((qn + 63) / 64), dim3(64), 0, stream, a);
-      hipLaunchKernelGGL(V1ScanKernel, dim3(1), dim3(1), 0, stream, a);
-      hipLaunchKernelGGL(V1ScatterKernel, dim3(qn), dim3(256), 0, stream, a);
-      check(hipGetLastError(), "V1 path launch");
-      launches += 3;
+      if (frontier) {
+        hipLaunchKernelGGL(V1SelectPredecessorsKernel, dim3(state_blocks), dim3(256), 0, stream, a, total_states);
+        check(hipGetLastError(), "V1 predecessor launch");
+        launches += 1;
+      } else {
+        hipLaunchKernelGGL(V1PathKernel, dim3((qn + 63) / 64), dim3(64), 0, stream, a);
+        hipLaunchKernelGGL(V1ScanKernel, dim3(1), dim3(1), 0, stream, a);
+        hipLaunchKernelGGL(V1ScatterKernel, dim3(qn), dim3(256), 0, stream, a);
+        check(hipGetLastError(), "V1 path launch");
+        launches += 3;
+      }
       check(hipEventRecord(e1, stream), "hipEventRecord");
       check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     }
   }
   float ms = 0.f;
   if (status == APGX_OK) check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-  if (status == APGX_OK) {
+  if (status == APGX_OK)
     check(hipMemcpy(pend->results.data(), a.results, qn * sizeof(apgx_v1_result), hipMemcpyDeviceToHost), "result readback");
-    check(hipMemcpy(pend->paths.data(), a.paths, qn * sizeof(apgx_v1_compact_path), hipMemcpyDeviceToHost), "path header readback");
-  }
   uint64_t used = 0;
-  for (const auto& p : pend->paths) used += p.state_count;
-  if (status == APGX_OK) {
+  if (status == APGX_OK && frontier) {
+    pend->labels.resize(total_states);
+    pend->predecessors.resize(total_states);
+    pend->state_owners.resize(total_states);
+    pend->predecessor_owners.resize(total_states);
+    check(hipMemcpy(pend->labels.data(), a.labels, total_states * 8, hipMemcpyDeviceToHost), "label readback");
+    check(hipMemcpy(pend->predecessors.data(), a.predecessors, total_states * 4, hipMemcpyDeviceToHost), "predecessor readback");
+    check(hipMemcpy(pend->state_owners.data(), a.state_owners, total_states * 8, hipMemcpyDeviceToHost), "owner readback");
+    check(hipMemcpy(pend->predecessor_owners.data(), a.predecessor_owners, total_states * 8, hipMemcpyDeviceToHost), "owner readback");
+  } else if (status == APGX_OK) {
+    check(hipMemcpy(pend->paths.data(), a.paths, qn * sizeof(apgx_v1_compact_path), hipMemcpyDeviceToHost), "path header readback");
+    for (const auto& p : pend->paths) used += p.state_count;
     pend->states.resize(used);
-    if (used) check(hipMemcpy(pend->states.data(), a.packed, used * 4, hipMemcpyDeviceToHost), "path readback");
+    if (used && status == APGX_OK) check(hipMemcpy(pend->states.data(), a.packed, used * 4, hipMemcpyDeviceToHost), "path readback");
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (stream) (void)hipStreamDestroy(stream);
   (void)hipFree(block);
   if (status != APGX_OK) { delete pend; return status; }
-  uint32_t finalization = 0;
-  for (apgx_v1_result& r : pend->results) {
-    if (r.reserved[0] == 0) {
-      r.completion = cancelled ? APGX_V1_CANCELLED : APGX_V1_BUDGET_EXHAUSTED;
-      finalization = 1;
-    }
-    r.reserved[0] = 0;
+  pend->envelopes.resize(qn);
+  for (uint32_t i = 0; i < qn; ++i) {
+    apgx_v1_result& r = pend->results[i];
+    r.reserved[0] = 0;  // device status word
+    pend->envelopes[i] = apgx_v1_query_envelope{static_cast<uint64_t>(i) * S, S, r.examined_work,
+                                                r.heading_turn_relaxations, r.rounds, 0};
   }
   t.kernel_launch_count = launches;
   t.blocking_status_readback_count = syncs;
   t.dispatched_rounds = dispatched;
   t.finalization_launch_count = finalization;
   t.kernel_milliseconds = ms;
-  t.device_to_host_readback_bytes = qn * (sizeof(apgx_v1_result) + sizeof(apgx_v1_compact_path)) + used * 4;
-  t.batch_host_bytes = qn * (sizeof(apgx_v1_result) + sizeof(apgx_v1_compact_path) + sizeof(apgx_v1_query)) + used * 4;
+  t.device_to_host_readback_bytes = ReadbackBytes(frontier, qn, nn, used);
+  t.batch_host_bytes = HostBytes(frontier, qn, nn, used);
   (void)wall0;
   *out = pend;
   ClearError();
@@ -710,6 +817,130 @@ extern "C" int apgx_v1_batch_readback(apgx_v1_pending* p, apgx_v1_result* result
   if (paths) std::memcpy(paths, p->paths.data(), p->paths.size() * sizeof(apgx_v1_compact_path));
   if (states && !p->states.empty()) std::memcpy(states, p->states.data(), p->states.size() * 4);
   if (tel) *tel = p->telemetry;
+  ClearError();
+  return APGX_OK;
+}
+
+extern "C" int apgx_v1_batch_readback_workspace(apgx_v1_pending* p, uint32_t* readback_kind,
+                                                apgx_v1_query_envelope* per_query, uint64_t* labels,
+                                                uint32_t* predecessors, uint64_t* state_owners,
+                                                uint64_t* predecessor_owners, uint64_t capacity,
+                                                uint64_t* total) {
+  using namespace apgx;
+  if (p == nullptr) return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_v1_batch_readback_workspace: null pending");
+  const uint64_t n = p->labels.size();
+  if (readback_kind) *readback_kind = p->readback_kind;
+  if (total) *total = n;
+  if (per_query && !p->envelopes.empty())
+    std::memcpy(per_query, p->envelopes.data(), p->envelopes.size() * sizeof(apgx_v1_query_envelope));
+  const bool want = labels || predecessors || state_owners || predecessor_owners;
+  if (want && n > capacity)
+    return Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_v1_batch_readback_workspace: state capacity too small");
+  if (n) {
+    if (labels) std::memcpy(labels, p->labels.data(), n * 8);
+    if (predecessors) std::memcpy(predecessors, p->predecessors.data(), n * 4);
+    if (state_owners) std::memcpy(state_owners, p->state_owners.data(), n * 8);
+    if (predecessor_owners) std::memcpy(predecessor_owners, p->predecessor_owners.data(), n * 8);
+  }
+  ClearError();
+  return APGX_OK;
+}
+
+// Replay/test-only readback decorator (FaultInjectingCandidateBatchBackend,
+// src/gpu/fault_injecting_backend.cc:113-224): corrupts the first query of an
+// executed batch's host copy, after the normal readback, so the trust boundary
+// can be shown to catch it.  The execution path itself has no fault controls.
+extern "C" int apgx_v1_pending_inject_fault(apgx_v1_pending* p, uint32_t fault) {
+  using namespace apgx;
+  if (p == nullptr) return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_v1_pending_inject_fault: null pending");
+  if (fault > APGX_V1_FAULT_COMPACT_PATH_EDGE) return Fail(APGX_E_UNSUPPORTED, "unknown readback fault");
+  if (fault == APGX_V1_FAULT_NONE || p->query_count == 0) {
+    ClearError();
+    return APGX_OK;
+  }
+  apgx_v1_result& r = p->results[0];
+  apgx_v1_compact_path& cp = p->paths[0];
+  const bool full = p->readback_kind == APGX_V1_READBACK_FULL_WORKSPACE;
+  const uint64_t S = p->states_per_query;
+  constexpr uint64_t kMax64 = ~0ull;
+  constexpr uint32_t kInvalidState = APGX_V1_INVALID_INDEX;
+  // Compact path slice of query 0 (empty when out of range, as the decorator's).
+  uint32_t* cs = nullptr;
+  uint64_t cn = 0;
+  if (!full && cp.state_offset <= p->states.size() && cp.state_count <= p->states.size() - cp.state_offset) {
+    cs = p->states.data() + cp.state_offset;
+    cn = cp.state_count;
+  }
+  auto contains = [&](uint32_t v) { return std::find(cs, cs + cn, v) != cs + cn; };
+  switch (fault) {
+    case APGX_V1_FAULT_WORKSPACE_BOUNDS: {
+      uint64_t& count = p->envelopes[0].workspace_state_count;
+      count = count == 0 ? 1 : count - 1;
+      break;
+    }
+    case APGX_V1_FAULT_WORKSPACE_OWNER:
+      if (full && S) p->state_owners[0] = r.workspace_owner == kMax64 ? 0 : r.workspace_owner + 1;
+      break;
+    case APGX_V1_FAULT_QUERY_TELEMETRY: {
+      const uint32_t rounds = p->envelopes[0].rounds;
+      r.rounds = rounds == 0xFFFFFFFFu ? rounds - 1 : rounds + 1;
+      break;
+    }
+    case APGX_V1_FAULT_MEMORY_ACCOUNTING:
+      p->telemetry.batch_host_bytes += p->telemetry.batch_host_bytes == kMax64 ? ~0ull : 1;
+      break;
+    case APGX_V1_FAULT_BATCH_TELEMETRY:
+      p->telemetry.kernel_launch_count += p->telemetry.kernel_launch_count == kMax64 ? ~0ull : 1;
+      break;
+    case APGX_V1_FAULT_QUERY_IDENTITY:
+      r.query_id = r.query_id == kMax64 ? 0 : r.query_id + 1;
+      break;
+    case APGX_V1_FAULT_FALSE_DISCONNECTED:
+      r.completion = APGX_V1_DISCONNECTED;
+      r.goal_state = kInvalidState;
+      if (full) {
+        std::fill(p->labels.begin(), p->labels.begin() + S, kMax64);
+        std::fill(p->predecessors.begin(), p->predecessors.begin() + S, kInvalidState);
+        const uint64_t start = static_cast<uint64_t>(r.start_node) * 9 + 8;
+        if (start < S) p->labels[start] = 0;
+      }
+      break;
+    case APGX_V1_FAULT_COMPACT_PATH_BOUNDS:
+      if (!full) cp.schema_version = cp.schema_version == 0xFFFFFFFFu ? cp.schema_version - 1 : cp.schema_version + 1;
+      break;
+    case APGX_V1_FAULT_COMPACT_PATH_ENDPOINT:
+      if (cn) cs[0] = kInvalidState;
+      break;
+    case APGX_V1_FAULT_COMPACT_PATH_CYCLE:
+      if (cn >= 4) cs[1] = cs[2];
+      break;
+    case APGX_V1_FAULT_COMPACT_PATH_HEADING:
+      for (uint64_t i = 1; i + 1 < cn; ++i) {
+        const uint32_t node = cs[i] / 9u, rep = node * 9u + 8u;
+        if (node != r.start_node && !contains(rep)) {
+          cs[i] = rep;
+          break;
+        }
+      }
+      break;
+    case APGX_V1_FAULT_COMPACT_PATH_EDGE: {
+      bool done = false;
+      for (uint64_t i = 1; i + 1 < cn && !done; ++i) {
+        const uint32_t node = cs[i] / 9u, heading = cs[i] % 9u;
+        for (uint32_t h = 0; h < 8; ++h) {
+          const uint32_t rep = node * 9u + h;
+          if (h != heading && !contains(rep)) {
+            cs[i] = rep;
+            done = true;
+            break;
+          }
+        }
+      }
+      break;
+    }
+    default:
+      break;
+  }
   ClearError();
   return APGX_OK;
 }

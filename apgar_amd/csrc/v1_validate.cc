@@ -1,13 +1,15 @@
 // v1_validate.cc — host trust boundary for V1 candidate batches.
 //
-// Re-checks an untrusted compact-path readback against the view, the request
-// and the queries before any result is used, as the reference's host router
-// does (src/gpu/planar_router.cc:969-1131 compact paths, :1808-1955 query and
-// batch telemetry, :629-667 CPU confirmation of "disconnected").  A failure
-// names the same invariant identifiers the reference reports, so callers and
-// replays can assert on them.  Nothing here produces a route: the validator
-// only accepts or rejects what the device returned.
+// Re-checks an untrusted readback (compact paths from the sweep, the full
+// workspace from the frontier) against the view, the request and the queries
+// before any result is used, as the reference's host router does
+// (src/gpu/planar_router.cc:2195-2380, with the checks it calls).  A failure
+// carries the reference's failure class and, where the reference defines one,
+// its invariant identifier, so callers and replays can assert on them.
+// Nothing here produces a route: the validator only accepts or rejects what
+// the device returned.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <queue>
 #include <string>
@@ -15,6 +17,7 @@
 
 #include "apgx.h"
 #include "common.h"
+#include "v1_common.h"
 
 namespace apgx {
 namespace v1check {
@@ -113,6 +116,93 @@ bool GoalReachable(const Context& c, const apgx_v1_query& q) {
   return false;
 }
 
+
+// One adjacent, legal, same-layer compiled edge from `pnode` to `node` in
+// direction `dir`.
+bool AdjacentLegal(const Context& c, uint32_t pnode, uint32_t node, uint32_t dir) {
+  static const int kDx8[8] = {1, 1, 0, -1, -1, -1, 0, 1}, kDy8[8] = {0, 1, 1, 1, 0, -1, -1, -1};
+  const apgx_v1_node& pn = c.nodes[pnode];
+  const int64_t dx = c.nodes[node].lattice_x - pn.lattice_x, dy = c.nodes[node].lattice_y - pn.lattice_y;
+  return pn.neighbors[dir] == node && (pn.legal_edges & (1u << dir)) != 0 && dx == kDx8[dir] && dy == kDy8[dir] &&
+         pn.layer == c.nodes[node].layer;
+}
+
+// ValidateAllPredecessors (src/gpu/planar_router.cc:491-612) over one query's
+// labels and predecessors: exact (label = predecessor label + step) for
+// converged results, a lower bound (and missing predecessors allowed) for
+// unfinished ones.
+bool AllPredecessorsValid(const Context& c, const apgx_v1_query& q, const uint64_t* lab, const uint32_t* pre,
+                          bool exact, std::string* why, const char** id) {
+  const uint64_t S = c.node_count * 9;
+  const uint32_t start_state = q.start_node * 9u + 8u;
+  if (lab[start_state] != 0 || pre[start_state] != kNone) {
+    *why = "GPU start state does not have the canonical zero label and no predecessor";
+    return false;
+  }
+  for (uint64_t s = 0; s < S; ++s) {
+    const uint32_t state = static_cast<uint32_t>(s);
+    if (state == start_state) continue;
+    const uint64_t label = lab[state];
+    const uint32_t p = pre[state];
+    const uint32_t inc = state % 9u;
+    if (inc == 8u) {
+      if (label != kInf || p != kNone) {
+        *why = "GPU created a no-incoming-heading state away from the start";
+        return false;
+      }
+      continue;
+    }
+    if (label == kInf) {
+      if (p != kNone) {
+        *why = "GPU unreachable state carries a predecessor";
+        return false;
+      }
+      continue;
+    }
+    if (p == kNone && !exact) continue;
+    if (p >= S) {
+      *why = "GPU predecessor index is out of bounds";
+      return false;
+    }
+    if (p == state) {
+      *why = "GPU predecessor is self-referential";
+      *id = "gpu.predecessor.self_reference.v1";
+      return false;
+    }
+    const uint32_t node = state / 9u, pnode = p / 9u, pinc = p % 9u;
+    if (!AdjacentLegal(c, pnode, node, inc)) {
+      *why = "GPU predecessor does not follow one adjacent compiled legal edge";
+      return false;
+    }
+    if (pinc == 8u && p != start_state) {
+      *why = "GPU predecessor uses the start-only heading sentinel elsewhere";
+      return false;
+    }
+    if (lab[p] == kInf) {
+      *why = "GPU finite state follows an unreachable predecessor";
+      return false;
+    }
+    uint64_t w, expected;
+    bool banned;
+    if (!Transition(c, q, pnode, inc, pinc, &w, &banned)) {
+      if (banned) {
+        *why = "GPU predecessor traverses a banned or unaddressable resource";
+        *id = "gpu.policy.banned_predecessor.v1";
+        return false;
+      }
+      *why = exact ? "GPU predecessor label does not reconstruct the exact scalar cost"
+                   : "GPU partial predecessor label violates the scalar-cost lower bound";
+      return false;
+    }
+    if (!CheckedAdd(lab[p], w, &expected) || (exact ? expected != label : expected > label)) {
+      *why = exact ? "GPU predecessor label does not reconstruct the exact scalar cost"
+                   : "GPU partial predecessor label violates the scalar-cost lower bound";
+      return false;
+    }
+  }
+  return true;
+}
+
 }  // namespace
 }  // namespace v1check
 }  // namespace apgx
@@ -123,57 +213,105 @@ extern "C" int apgx_v1_validate_batch(const apgx_v1_header* header, const apgx_v
                                       const apgx_v1_policy_edge* policy, uint64_t policy_count,
                                       const apgx_v1_result* results, const apgx_v1_compact_path* paths,
                                       const uint32_t* states, uint64_t state_count,
+                                      const apgx_v1_workspace_view* workspace,
                                       const apgx_v1_batch_telemetry* telemetry, uint64_t* path_costs,
                                       apgx_v1_validation* out) {
   using namespace apgx;
+  using namespace apgx::v1;
   using namespace apgx::v1check;
   if (out == nullptr || header == nullptr || nodes == nullptr || request == nullptr || telemetry == nullptr ||
-      (query_count && (queries == nullptr || results == nullptr || paths == nullptr)) ||
+      workspace == nullptr || (query_count && (queries == nullptr || results == nullptr)) ||
       (policy_count && policy == nullptr) || (state_count && states == nullptr))
     return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_v1_validate_batch: null argument");
   std::memset(out, 0, sizeof(*out));
   out->query_index = kNone;
-  auto reject = [&](uint32_t qi, const char* invariant, const std::string& detail) {
+  for (uint32_t i = 0; i < query_count && path_costs; ++i) path_costs[i] = kInf;
+  auto reject = [&](uint32_t qi, uint32_t failure, const char* invariant, const std::string& detail) {
     out->valid = 0;
     out->query_index = qi;
+    out->failure = failure;
     std::strncpy(out->invariant, invariant, sizeof(out->invariant) - 1);
     std::strncpy(out->detail, detail.c_str(), sizeof(out->detail) - 1);
     ClearError();
     return APGX_OK;
   };
+  auto invariant = [&](uint32_t qi, const char* id, const std::string& detail) {
+    return reject(qi, APGX_V1_CHECK_INTERNAL_INVARIANT, id, detail);
+  };
   const Context c{header, nodes, node_count, policy};
   const uint64_t S = node_count * 9;
-  const bool frontier = request->generator == APGX_V1_GENERATOR_BUCKETED_FRONTIER;
-  if (request->generator != APGX_V1_GENERATOR_BUCKETED_FRONTIER &&
-      request->generator != APGX_V1_GENERATOR_HEADING_AWARE_SWEEP)
-    return reject(kNone, "gpu.generator.unknown.v1", "request names an unknown generator");
-  uint64_t legal_edges = 0;
-  for (uint64_t i = 0; i < node_count; ++i) legal_edges += __builtin_popcount(nodes[i].legal_edges);
+  GeneratorShape shape{};
+  if (!DescribeGenerator(request->generator, &shape))
+    return reject(kNone, APGX_V1_CHECK_VALIDATION_FAILED, "gpu.generator.unknown.v1",
+                  "GPU candidate batch telemetry names an unknown generator");
+  const bool full = shape.full_workspace;
 
-  // Batch telemetry identities.
+  // Every result names exactly one admitted query (:2195-2218).
+  {
+    std::vector<uint64_t> want(query_count), got(query_count);
+    for (uint32_t i = 0; i < query_count; ++i) {
+      want[i] = queries[i].query_id;
+      got[i] = results[i].query_id;
+    }
+    std::sort(want.begin(), want.end());
+    std::sort(got.begin(), got.end());
+    if (want != got)
+      return invariant(kNone, "gpu.batch.query.identity.v1",
+                       "GPU candidate batch returned an extra, foreign, or missing query result");
+  }
+
+  // Batch telemetry (ValidateCandidateBatchTelemetry, :1841-1955).
   const apgx_v1_batch_telemetry& t = *telemetry;
-  const uint32_t chunk = frontier ? 32u : 8u;
-  const uint64_t chunks = (static_cast<uint64_t>(t.dispatched_rounds) + chunk - 1) / chunk;
-  if (t.chunk_rounds != chunk || t.dispatched_rounds > request->maximum_rounds ||
-      t.blocking_status_readback_count != chunks || t.kernel_launch_count != 1 + chunks + 3 ||
-      t.finalization_launch_count > 1)
-    return reject(kNone, "gpu.batch.telemetry.v1", "chunk, readback or launch counts disagree with the generator");
-  uint64_t used = 0;
-  for (uint32_t i = 0; i < query_count; ++i) used += paths[i].state_count;
-  const uint64_t record_bytes = sizeof(apgx_v1_result) + sizeof(apgx_v1_compact_path);
-  if (used != state_count || t.device_to_host_readback_bytes != query_count * record_bytes + used * 4 ||
-      t.batch_host_bytes != query_count * (record_bytes + sizeof(apgx_v1_query)) + used * 4 ||
-      t.peak_device_bytes != t.persistent_device_bytes + t.workspace_capacity_device_bytes ||
-      t.workspace_capacity_device_bytes < t.batch_device_bytes ||
-      t.peak_device_bytes > request->maximum_device_bytes || t.persistent_device_bytes != header->estimated_persistent_device_bytes)
-    return reject(kNone, "gpu.batch.memory.accounting.v1", "byte accounting disagrees with the readback");
+  const uint64_t total_states = static_cast<uint64_t>(query_count) * S;
+  const apgx_v1_workspace_view& ws = *workspace;
+  if (query_count && ws.queries == nullptr)
+    return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_v1_validate_batch: null workspace envelopes");
+  const bool arrays_empty = ws.label_count == 0 && ws.predecessor_count == 0 && ws.state_owner_count == 0 &&
+                            ws.predecessor_owner_count == 0;
+  if ((!full && (ws.readback_kind != APGX_V1_READBACK_COMPACT_PATHS || !arrays_empty || state_count > total_states ||
+                 (query_count && paths == nullptr))) ||
+      (full && (ws.readback_kind != APGX_V1_READBACK_FULL_WORKSPACE || ws.label_count != total_states ||
+                ws.predecessor_count != total_states || ws.state_owner_count != total_states ||
+                ws.predecessor_owner_count != total_states || state_count != 0 ||
+                (total_states && (!ws.labels || !ws.predecessors || !ws.state_owners || !ws.predecessor_owners)))))
+    return invariant(kNone, "gpu.batch.workspace.bounds.v1",
+                     "GPU candidate batch readback kind or flat workspace bounds are invalid");
+  if (!full) {
+    uint64_t expected = 0;
+    for (uint32_t i = 0; i < query_count; ++i) {
+      if (paths[i].state_offset != expected)
+        return invariant(kNone, "gpu.batch.compact_path.bounds.v1",
+                         "GPU compact sweep paths are not one canonical contiguous batch");
+      expected += paths[i].state_count;
+    }
+    if (expected != state_count)
+      return invariant(kNone, "gpu.batch.compact_path.bounds.v1",
+                       "GPU compact sweep path storage has trailing or missing states");
+  }
+  const uint64_t device_bytes = LayoutBatch(full, query_count, policy_count, node_count).total;
+  if (t.persistent_device_bytes != header->estimated_persistent_device_bytes || t.batch_device_bytes != device_bytes ||
+      t.batch_host_bytes != HostBytes(full, query_count, node_count, state_count))
+    return invariant(kNone, "gpu.batch.memory.accounting.v1",
+                     "GPU candidate batch device or host memory telemetry disagrees with deterministic accounting");
+  if (t.peak_device_bytes != t.persistent_device_bytes + t.workspace_capacity_device_bytes ||
+      t.peak_device_bytes < t.persistent_device_bytes || t.workspace_capacity_device_bytes < t.batch_device_bytes ||
+      t.persistent_device_bytes > request->maximum_device_bytes ||
+      t.workspace_capacity_device_bytes > request->maximum_device_bytes - t.persistent_device_bytes ||
+      t.device_to_host_readback_bytes != ReadbackBytes(full, query_count, node_count, state_count) ||
+      !std::isfinite(t.kernel_milliseconds) || t.kernel_milliseconds < 0.0 || t.chunk_rounds != shape.chunk_rounds ||
+      t.dispatched_rounds > request->maximum_rounds || t.finalization_launch_count > 1 ||
+      (t.dispatched_rounds == 0 && t.finalization_launch_count != 1))
+    return invariant(kNone, "gpu.batch.telemetry.v1", "GPU candidate batch peak, timing, or chunk telemetry is invalid");
+  const uint64_t readbacks = (static_cast<uint64_t>(t.dispatched_rounds) + shape.chunk_rounds - 1) / shape.chunk_rounds;
+  if (t.kernel_launch_count != shape.fixed_launches + readbacks + t.finalization_launch_count ||
+      t.blocking_status_readback_count != readbacks)
+    return invariant(kNone, "gpu.batch.telemetry.v1", "GPU candidate batch launch or blocking-readback telemetry is invalid");
 
   std::vector<uint64_t> visited((S + 63) / 64, 0);
   for (uint32_t qi = 0; qi < query_count; ++qi) {
     const apgx_v1_query& q = queries[qi];
     const apgx_v1_result& r = results[qi];
-    const apgx_v1_compact_path& p = paths[qi];
-    if (path_costs) path_costs[qi] = kInf;
+    // Envelope (ValidateCandidateBatchEnvelope, :1732-1839).
     if (r.schema_version != APGX_V1_SCHEMA_VERSION || r.batch_id != q.batch_id || r.query_id != q.query_id ||
         r.workspace_owner != q.workspace_owner || r.input_ordinal != q.input_ordinal ||
         r.start_node != q.start_node || r.goal_node != q.goal_node || r.generator != q.generator ||
@@ -182,64 +320,135 @@ extern "C" int apgx_v1_validate_batch(const apgx_v1_header* header, const apgx_v
         r.source_board_content_hash != header->source_board_content_hash ||
         r.compiler_profile_fingerprint != header->compiler_profile_fingerprint ||
         r.rule_bucket_identity != header->rule_bucket_identity ||
-        r.device_view_fingerprint != header->device_view_fingerprint)
-      return reject(qi, "gpu.batch.query.identity.v1", "result header does not echo its query and view");
+        r.device_view_fingerprint != header->device_view_fingerprint || r.reserved[0] != 0 || r.reserved[1] != 0)
+      return reject(qi, APGX_V1_CHECK_VALIDATION_FAILED, "",
+                    "GPU candidate batch result associations do not match its immutable query workspace");
+    const apgx_v1_query_envelope& env = ws.queries[qi];
+    if (env.workspace_offset != q.workspace_offset || env.workspace_state_count != S)
+      return invariant(qi, "gpu.batch.workspace.bounds.v1",
+                       "GPU candidate batch query-major workspace has invalid array bounds");
+    const uint64_t off = env.workspace_offset;
+    if (full) {
+      if (off > total_states || env.workspace_state_count > total_states - off)
+        return invariant(qi, "gpu.batch.workspace.bounds.v1",
+                         "GPU candidate batch query-major workspace has invalid array bounds");
+      for (uint64_t k = 0; k < S; ++k)
+        if (ws.state_owners[off + k] != q.workspace_owner || ws.predecessor_owners[off + k] != q.workspace_owner)
+          return invariant(qi, "gpu.batch.workspace.owner.v1",
+                           "GPU candidate batch workspace contains foreign query ownership");
+    } else {
+      const apgx_v1_compact_path& p = paths[qi];
+      const bool reached = r.completion == APGX_V1_REACHED;
+      const uint64_t end = p.state_offset + static_cast<uint64_t>(p.state_count);
+      if (p.schema_version != APGX_V1_SCHEMA_VERSION || p.query_id != q.query_id ||
+          p.workspace_owner != q.workspace_owner || p.state_count > S || end < p.state_offset || end > state_count ||
+          (reached ? p.state_count == 0 : p.state_count != 0))
+        return invariant(qi, "gpu.batch.compact_path.bounds.v1",
+                         "GPU compact sweep path has invalid associations or array bounds");
+    }
+    if (env.rounds != r.rounds || r.rounds > t.dispatched_rounds)
+      return invariant(qi, "gpu.batch.query.telemetry.v1",
+                       "GPU candidate query round telemetry disagrees with its result header or shared dispatch envelope");
+    if ((r.completion == APGX_V1_BUDGET_EXHAUSTED || r.completion == APGX_V1_CANCELLED) &&
+        t.finalization_launch_count != 1)
+      return invariant(qi, "gpu.batch.query.telemetry.v1",
+                       "GPU unfinished query completion lacks its shared finalization launch");
+    const unsigned __int128 max_examined =
+        static_cast<unsigned __int128>(env.rounds) *
+        (shape.fixed_examined_work_per_round + static_cast<unsigned __int128>(node_count) *
+                                                   shape.examined_work_per_node_per_round);
+    if (env.examined_work > max_examined ||
+        env.heading_turn_relaxations > max_examined * shape.turn_relaxations_per_examined_work)
+      return invariant(qi, "gpu.batch.query.telemetry.v1", "GPU candidate query work telemetry exceeds deterministic bounds");
+    // ValidateCompletionTelemetry (:456-484).
     if (r.completion > APGX_V1_CANCELLED)
-      return reject(qi, "gpu.completion.unknown.v1", "unknown completion value");
-    const bool examined_ok = frontier ? r.examined_work <= 8ull * r.rounds
-                                      : r.examined_work == static_cast<uint64_t>(r.rounds) * legal_edges;
-    if (r.rounds > t.dispatched_rounds || !examined_ok || r.heading_turn_relaxations > 8 * r.examined_work ||
-        (r.completion == APGX_V1_REACHED && r.rounds == 0) ||
-        (r.completion == APGX_V1_BUDGET_EXHAUSTED && r.rounds < std::min(request->maximum_rounds, q.maximum_rounds)))
-      return reject(qi, "gpu.batch.query.telemetry.v1", "query rounds or work disagree with the batch");
-    if (p.schema_version != APGX_V1_SCHEMA_VERSION || p.query_id != q.query_id ||
-        p.workspace_owner != q.workspace_owner ||
-        (r.completion != APGX_V1_REACHED && p.state_count != 0) ||
-        (r.completion == APGX_V1_REACHED && (p.state_count < 2 || p.state_count > S)) ||
-        p.state_offset > state_count || p.state_count > state_count - p.state_offset)
-      return reject(qi, "gpu.batch.compact_path.bounds.v1", "compact path header out of bounds");
+      return invariant(qi, "gpu.completion.unknown.v1", "GPU result contains an unknown completion code");
+    if (r.rounds > request->maximum_rounds)
+      return invariant(qi, "", "GPU round telemetry exceeds the configured execution budget");
+    if (r.completion == APGX_V1_BUDGET_EXHAUSTED && r.rounds != request->maximum_rounds)
+      return invariant(qi, "", "GPU reported budget exhaustion before consuming its round budget");
+    if ((r.completion == APGX_V1_REACHED || r.completion == APGX_V1_DISCONNECTED) && r.rounds == 0)
+      return invariant(qi, "", "GPU reported a converged completion without an execution round");
+
+    const uint32_t start_state = q.start_node * 9u + 8u;
+    if (full) {
+      const uint64_t* lab = ws.labels + off;
+      const uint32_t* pre = ws.predecessors + off;
+      const bool exact = r.completion == APGX_V1_REACHED || r.completion == APGX_V1_DISCONNECTED;
+      std::string why;
+      const char* id = "";
+      if (!AllPredecessorsValid(c, q, lab, pre, exact, &why, &id)) return invariant(qi, id, why);
+      if (r.completion == APGX_V1_REACHED) {
+        uint64_t best = kInf;
+        uint32_t best_state = kNone;
+        for (uint32_t h = 0; h < 8; ++h) {
+          const uint32_t st = q.goal_node * 9u + h;
+          if (lab[st] < best || (lab[st] == best && st < best_state)) {
+            best = lab[st];
+            best_state = st;
+          }
+        }
+        if (best == kInf || r.goal_state != best_state)
+          return invariant(qi, "", "GPU goal state is not the stable minimum finite heading state");
+        uint32_t cursor = best_state;
+        for (uint64_t steps = 0;; ++steps) {
+          if (cursor >= S) return invariant(qi, "", "GPU reconstruction escaped the allocated state array");
+          if (cursor == start_state) break;
+          if (steps >= S) return invariant(qi, "", "GPU predecessor reconstruction exceeded its acyclic state bound");
+          cursor = pre[cursor];
+        }
+        if (path_costs) path_costs[qi] = best;
+      } else if (r.completion == APGX_V1_DISCONNECTED) {
+        for (uint32_t h = 0; h < 8; ++h)
+          if (lab[q.goal_node * 9u + h] != kInf)
+            return invariant(qi, "", "GPU reported disconnected with a finite goal label");
+        if (GoalReachable(c, q))
+          return invariant(qi, "gpu.disconnected.cpu_reachable.v1",
+                           "GPU reported disconnected for a query the CPU oracle reached under the identical policy");
+      }
+      continue;
+    }
+    // Compact path (ValidateAndReconstructCompactGpuPath, :969-1131).
+    const apgx_v1_compact_path& p = paths[qi];
     if (r.completion == APGX_V1_DISCONNECTED && GoalReachable(c, q))
-      return reject(qi, "gpu.disconnected.cpu_reachable.v1", "the CPU search reaches a goal reported disconnected");
+      return invariant(qi, "gpu.disconnected.cpu_reachable.v1",
+                       "GPU reported disconnected for a query the CPU oracle reached under the identical policy");
     if (r.completion != APGX_V1_REACHED) continue;
     const uint32_t* rev = states + p.state_offset;
     const uint32_t n = p.state_count;
-    const uint32_t start_state = q.start_node * 9u + 8u;
     if (rev[0] != r.goal_state || rev[n - 1] != start_state || r.goal_state >= S ||
         r.goal_state / 9u != q.goal_node || r.goal_state % 9u >= 8u)
-      return reject(qi, "gpu.batch.compact_path.endpoint.v1", "path does not bind the goal to the canonical start");
+      return invariant(qi, "gpu.batch.compact_path.endpoint.v1", "path does not bind the goal to the canonical start");
     for (uint32_t k = 0; k < n; ++k) {
-      if (rev[k] >= S) return reject(qi, "gpu.batch.compact_path.bounds.v1", "path state out of range");
+      if (rev[k] >= S) return invariant(qi, "gpu.batch.compact_path.bounds.v1", "path state out of range");
       visited[rev[k] / 64] = 0;
     }
     uint64_t total = 0;
     for (uint32_t k = 0; k < n; ++k) {
       const uint32_t st = rev[n - 1 - k];
       const uint64_t mask = 1ull << (st % 64);
-      if (visited[st / 64] & mask) return reject(qi, "gpu.batch.compact_path.cycle.v1", "repeated path state");
+      if (visited[st / 64] & mask) return invariant(qi, "gpu.batch.compact_path.cycle.v1", "repeated path state");
       visited[st / 64] |= mask;
       if (k == 0) continue;
       const uint32_t pred = rev[n - k];
       const uint32_t pnode = pred / 9u, node = st / 9u, inc = st % 9u, pinc = pred % 9u;
       if (inc >= 8u || (pinc == 8u && pred != start_state))
-        return reject(qi, "gpu.batch.compact_path.heading.v1", "noncanonical incoming heading");
-      const apgx_v1_node& pn = nodes[pnode];
-      const int64_t dx = nodes[node].lattice_x - pn.lattice_x, dy = nodes[node].lattice_y - pn.lattice_y;
-      static const int kDx[8] = {1, 1, 0, -1, -1, -1, 0, 1}, kDy[8] = {0, 1, 1, 1, 0, -1, -1, -1};
-      if (pn.neighbors[inc] != node || (pn.legal_edges & (1u << inc)) == 0 || dx != kDx[inc] || dy != kDy[inc] ||
-          pn.layer != nodes[node].layer)
-        return reject(qi, "gpu.batch.compact_path.edge.v1", "path step is not one adjacent legal edge");
+        return invariant(qi, "gpu.batch.compact_path.heading.v1", "noncanonical incoming heading");
+      if (!AdjacentLegal(c, pnode, node, inc))
+        return invariant(qi, "gpu.batch.compact_path.edge.v1", "path step is not one adjacent legal edge");
       uint64_t w;
       bool banned;
       if (!Transition(c, q, pnode, inc, pinc, &w, &banned)) {
-        if (banned) return reject(qi, "gpu.policy.banned_predecessor.v1", "path uses a banned resource");
-        return reject(qi, "gpu.batch.compact_path.edge.v1", "path step cost is invalid or overflows");
+        if (banned) return invariant(qi, "gpu.policy.banned_predecessor.v1", "path uses a banned resource");
+        return invariant(qi, "gpu.batch.compact_path.edge.v1", "path step cost is invalid or overflows");
       }
       if (!CheckedAdd(total, w, &total))
-        return reject(qi, "gpu.batch.compact_path.edge.v1", "path cost overflows");
+        return invariant(qi, "gpu.batch.compact_path.edge.v1", "path cost overflows");
     }
     if (path_costs) path_costs[qi] = total;
   }
   out->valid = 1;
+  out->failure = APGX_V1_CHECK_OK;
   ClearError();
   return APGX_OK;
 }

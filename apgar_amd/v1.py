@@ -8,6 +8,9 @@ from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass
+from typing import NamedTuple, Optional
+
+import numpy as np
 
 from . import check, lib
 
@@ -15,6 +18,14 @@ INVALID = 0xFFFFFFFF
 BANNED = 0xFFFFFFFFFFFFFFFF
 GENERATOR_FRONTIER, GENERATOR_SWEEP = 0, 1
 REACHED, DISCONNECTED, BUDGET_EXHAUSTED, CANCELLED = 0, 1, 2, 3
+READBACK_COMPACT_PATHS, READBACK_FULL_WORKSPACE = 0, 1
+# apgx_v1_pending_inject_fault ids, in the reference decorator's order
+FAULT_IDS = {"none": 0, "workspace_bounds": 1, "workspace_owner": 2, "query_telemetry": 3, "memory_accounting": 4,
+             "batch_telemetry": 5, "query_identity": 6, "false_disconnected": 7, "compact_path_bounds": 8,
+             "compact_path_endpoint": 9, "compact_path_cycle": 10, "compact_path_heading": 11,
+             "compact_path_edge": 12}
+FAULT_NONE = 0
+CHECK_OK, CHECK_INTERNAL_INVARIANT, CHECK_VALIDATION_FAILED = 0, 1, 2
 U8, U32, U64, I64, P = C.c_uint8, C.c_uint32, C.c_uint64, C.c_int64, C.POINTER
 
 
@@ -105,7 +116,47 @@ class CompiledNode(C.Structure):
 
 
 class Validation(C.Structure):
-    _fields_ = [("valid", U32), ("query_index", U32), ("invariant", C.c_char * 64), ("detail", C.c_char * 128)]
+    _fields_ = [("valid", U32), ("query_index", U32), ("failure", U32), ("reserved", U32),
+                ("invariant", C.c_char * 64), ("detail", C.c_char * 128)]
+
+
+class QueryEnvelope(C.Structure):
+    _fields_ = [("workspace_offset", U64), ("workspace_state_count", U64), ("examined_work", U64),
+                ("heading_turn_relaxations", U64), ("rounds", U32), ("reserved", U32)]
+
+
+class WorkspaceView(C.Structure):
+    _fields_ = [("readback_kind", U32), ("reserved", U32), ("queries", P(QueryEnvelope)),
+                ("labels", P(U64)), ("label_count", U64), ("predecessors", P(U32)), ("predecessor_count", U64),
+                ("state_owners", P(U64)), ("state_owner_count", U64), ("predecessor_owners", P(U64)),
+                ("predecessor_owner_count", U64)]
+
+
+@dataclass
+class Workspace:
+    """The workspace half of a readback: readback kind, per-query envelopes
+    and, for the frontier, the flat query-major arrays (numpy)."""
+    readback_kind: int
+    envelopes: list
+    labels: np.ndarray
+    predecessors: np.ndarray
+    state_owners: np.ndarray
+    predecessor_owners: np.ndarray
+
+    def copy(self):
+        return Workspace(self.readback_kind, [QueryEnvelope(e.workspace_offset, e.workspace_state_count,
+                                                            e.examined_work, e.heading_turn_relaxations, e.rounds, 0)
+                                              for e in self.envelopes],
+                         self.labels.copy(), self.predecessors.copy(), self.state_owners.copy(),
+                         self.predecessor_owners.copy())
+
+
+class Readback(NamedTuple):
+    results: list
+    paths: list
+    states: list
+    telemetry: "BatchTelemetry"
+    workspace: Optional[Workspace]
 
 
 _SIGS = {
@@ -122,10 +173,13 @@ _SIGS = {
                                         U64, P(U32), P(C.c_void_p)]),
     "apgx_v1_batch_readback": (C.c_int, [C.c_void_p, P(Result), P(CompactPath), P(U32), U64,
                                          P(U64), P(BatchTelemetry)]),
+    "apgx_v1_batch_readback_workspace": (C.c_int, [C.c_void_p, P(U32), P(QueryEnvelope), P(U64), P(U32),
+                                                   P(U64), P(U64), U64, P(U64)]),
+    "apgx_v1_pending_inject_fault": (C.c_int, [C.c_void_p, U32]),
     "apgx_v1_pending_release": (None, [C.c_void_p]),
     "apgx_v1_validate_batch": (C.c_int, [P(Header), P(Node), U64, P(BatchRequest), P(Query), U32, P(PolicyEdge),
-                                         U64, P(Result), P(CompactPath), P(U32), U64, P(BatchTelemetry), P(U64),
-                                         P(Validation)]),
+                                         U64, P(Result), P(CompactPath), P(U32), U64, P(WorkspaceView),
+                                         P(BatchTelemetry), P(U64), P(Validation)]),
 }
 
 
@@ -214,8 +268,12 @@ class View:
             pass
 
     def execute(self, queries, policy_edges=(), *, batch_id=1, maximum_rounds=100_000,
-                generator=GENERATOR_SWEEP, maximum_device_bytes=(1 << 63), cancel=None):
-        """Runs a batch; returns (results, compact paths, packed states, telemetry)."""
+                generator=GENERATOR_SWEEP, maximum_device_bytes=(1 << 63), cancel=None, fault=FAULT_NONE):
+        """Runs a batch and returns its Readback: results, compact paths and
+        packed states (sweep; empty for the frontier), telemetry, and the
+        workspace (envelopes for both; flat arrays for the frontier).
+        `fault` (replay/test only) corrupts query 0 of the host copy with the
+        library's readback decorator before it is read back."""
         S = len(self.board.nodes) * 9
         req = BatchRequest(1, generator, batch_id, maximum_rounds, 0, S, S, maximum_device_bytes,
                            1 << 63)
@@ -227,6 +285,7 @@ class View:
                                            len(policy_edges), flag if cancel is not None else None,
                                            C.byref(pending)))
         try:
+            check(_lib().apgx_v1_pending_inject_fault(pending, fault))
             used = U64()
             results = (Result * max(len(queries), 1))()
             paths = (CompactPath * max(len(queries), 1))()
@@ -235,10 +294,42 @@ class View:
             tel = BatchTelemetry()
             check(_lib().apgx_v1_batch_readback(pending, results, paths, states, used.value,
                                                 C.byref(used), C.byref(tel)))
+            ws = _read_workspace(pending, len(queries))
         finally:
             _lib().apgx_v1_pending_release(pending)
         n = len(queries)
-        return list(results)[:n], list(paths)[:n], list(states)[: used.value], tel
+        return Readback(list(results)[:n], list(paths)[:n], list(states)[: used.value], tel, ws)
+
+
+def _read_workspace(pending, n):
+    kind, total = U32(), U64()
+    env = (QueryEnvelope * max(n, 1))()
+    check(_lib().apgx_v1_batch_readback_workspace(pending, C.byref(kind), env, None, None, None, None, 0,
+                                                  C.byref(total)))
+    t = total.value
+    lab = np.zeros(t, np.uint64)
+    pre = np.zeros(t, np.uint32)
+    sown = np.zeros(t, np.uint64)
+    pown = np.zeros(t, np.uint64)
+    if t:
+        check(_lib().apgx_v1_batch_readback_workspace(
+            pending, None, None, lab.ctypes.data_as(P(U64)), pre.ctypes.data_as(P(U32)),
+            sown.ctypes.data_as(P(U64)), pown.ctypes.data_as(P(U64)), t, C.byref(total)))
+    return Workspace(kind.value, list(env)[:n], lab, pre, sown, pown)
+
+
+def reconstruct(workspace: Workspace, qi, result):
+    """Reverse state path (goal .. start) of a reached frontier query, walked
+    along the readback's predecessors from its goal state (no validation)."""
+    off = workspace.envelopes[qi].workspace_offset
+    out, cur = [], result.goal_state
+    start = result.start_node * 9 + 8
+    while cur != INVALID and len(out) <= workspace.envelopes[qi].workspace_state_count:
+        out.append(cur)
+        if cur == start:
+            return out
+        cur = int(workspace.predecessors[off + cur])
+    raise ValueError("predecessor walk does not reach the start")
 
 
 def make_query(i, start, goal, board: Board, *, batch_id=1, policy_offset=0, policy_count=0,
@@ -273,10 +364,24 @@ def request_for(board: Board, *, batch_id=1, maximum_rounds=100_000, generator=G
     return BatchRequest(1, generator, batch_id, maximum_rounds, 0, S, S, maximum_device_bytes, 1 << 63)
 
 
-def validate(board: Board, request, queries, policy_edges, results, paths, states, telemetry):
+def validate(board: Board, request, queries, policy_edges, results, paths, states, telemetry, workspace=None):
     """Host trust-boundary check of a readback (apgx_v1_validate_batch).
-    Returns (valid, query_index, invariant id, detail, path costs)."""
+    `workspace` None means a compact readback with canonical envelopes.
+    Returns (valid, query_index, invariant id, detail, path costs, failure class)."""
     n = len(queries)
+    S = len(board.nodes) * 9
+    if workspace is None:
+        workspace = Workspace(READBACK_COMPACT_PATHS,
+                              [QueryEnvelope(i * S, S, r.examined_work, r.heading_turn_relaxations, r.rounds, 0)
+                               for i, r in enumerate(results)],
+                              *(np.zeros(0, d) for d in (np.uint64, np.uint32, np.uint64, np.uint64)))
+    env = (QueryEnvelope * max(n, 1))(*workspace.envelopes)
+    arrays = [np.ascontiguousarray(a) for a in (workspace.labels, workspace.predecessors, workspace.state_owners,
+                                                  workspace.predecessor_owners)]
+    wv = WorkspaceView(workspace.readback_kind, 0, env,
+                       arrays[0].ctypes.data_as(P(U64)), len(arrays[0]), arrays[1].ctypes.data_as(P(U32)),
+                       len(arrays[1]), arrays[2].ctypes.data_as(P(U64)), len(arrays[2]),
+                       arrays[3].ctypes.data_as(P(U64)), len(arrays[3]))
     qarr = (Query * max(n, 1))(*queries)
     parr = (PolicyEdge * max(len(policy_edges), 1))(*policy_edges)
     rarr = (Result * max(n, 1))(*results)
@@ -286,5 +391,6 @@ def validate(board: Board, request, queries, policy_edges, results, paths, state
     out = Validation()
     check(_lib().apgx_v1_validate_batch(C.byref(board.header), board.nodes, len(board.nodes), C.byref(request),
                                         qarr, n, parr, len(policy_edges), rarr, carr, sarr, len(states),
-                                        C.byref(telemetry), costs, C.byref(out)))
-    return (bool(out.valid), out.query_index, out.invariant.decode(), out.detail.decode(), list(costs)[:n])
+                                        C.byref(wv), C.byref(telemetry), costs, C.byref(out)))
+    return (bool(out.valid), out.query_index, out.invariant.decode(), out.detail.decode(), list(costs)[:n],
+            out.failure)

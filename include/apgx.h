@@ -485,6 +485,10 @@ int apgx_prior_seed_history(apgx_prior* prior, apgx_lattice* lattice, const uint
 #define APGX_V1_BUDGET_EXHAUSTED 2u
 #define APGX_V1_CANCELLED 3u
 #define APGX_V1_BANNED_ADJUSTMENT 0xFFFFFFFFFFFFFFFFull
+/* CandidateBatchReadbackKind (include/apgar/gpu/planar_router.h): the sweep
+ * returns compact paths, the bucketed frontier its full query-major workspace. */
+#define APGX_V1_READBACK_COMPACT_PATHS 0u
+#define APGX_V1_READBACK_FULL_WORKSPACE 1u
 
 typedef struct apgx_v1_header {            /* DeviceCompiledHeaderV1, 112 bytes */
   uint32_t schema_version;
@@ -684,7 +688,11 @@ void apgx_v1_view_release(apgx_v1_view* view);
  * bucketed frontier (CandidateFrontierChunk semantics, 32 rounds per chunk;
  * queries carry frontier_state_capacity = represented states).  `cancel`
  * (may be NULL) is sampled between round chunks; a nonzero value cancels the
- * unfinished queries.  Results are read back in compact-path form for both. */
+ * unfinished queries.  Launches: init, one per chunk, one finalization when
+ * queries are still running after the last chunk, then 3 compact-path kernels
+ * (sweep) or 1 predecessor-selection kernel (frontier).  The sweep is read
+ * back in compact-path form, the frontier as its full workspace
+ * (apgx_v1_batch_readback_workspace), as the reference reads each back. */
 int apgx_v1_batch_execute(apgx_v1_view* view, const apgx_v1_batch_request* request,
                           const apgx_v1_query* queries, uint32_t query_count,
                           const apgx_v1_policy_edge* policy_edges, uint64_t policy_edge_count,
@@ -695,34 +703,108 @@ int apgx_v1_batch_execute(apgx_v1_view* view, const apgx_v1_batch_request* reque
 int apgx_v1_batch_readback(apgx_v1_pending* pending, apgx_v1_result* results,
                            apgx_v1_compact_path* paths, uint32_t* states, uint64_t state_capacity,
                            uint64_t* states_used, apgx_v1_batch_telemetry* telemetry);
+/* Per-query envelope of a readback: UntrustedCandidateBatchQueryResult
+ * (include/apgar/gpu/planar_router.h:502-508) less the result header and the
+ * compact path, i.e. the workspace slice and the query telemetry
+ * (CandidateQueryTelemetry) the backend reports beside the header. */
+typedef struct apgx_v1_query_envelope {
+  uint64_t workspace_offset;
+  uint64_t workspace_state_count;
+  uint64_t examined_work;
+  uint64_t heading_turn_relaxations;
+  uint32_t rounds;
+  uint32_t reserved;
+} apgx_v1_query_envelope;
+
+/* Full-workspace half of ReadbackCandidateBatch (gpu/cuda/planar_router.cu:
+ * 2700-2760): the readback kind, the per-query envelopes (both kinds), and for
+ * the frontier every state's label, lowest tight predecessor, state owner and
+ * predecessor owner, query-major (`total` = queries x represented states; 0 for
+ * the sweep).  Any array pointer may be NULL; non-NULL arrays need `capacity`
+ * >= total. */
+int apgx_v1_batch_readback_workspace(apgx_v1_pending* pending, uint32_t* readback_kind,
+                                     apgx_v1_query_envelope* per_query, uint64_t* labels,
+                                     uint32_t* predecessors, uint64_t* state_owners,
+                                     uint64_t* predecessor_owners, uint64_t capacity, uint64_t* total);
+/* Replay/test-only readback decorator (FaultInjectingCandidateBatchBackend,
+ * src/gpu/fault_injecting_backend.cc:113-224): corrupts query 0 of an executed
+ * batch's host copy the way the reference's decorator does, so the next
+ * readback returns the corrupted records.  Production execution has no fault
+ * controls; this exists for replays and trust-boundary tests. */
+#define APGX_V1_FAULT_NONE 0u
+#define APGX_V1_FAULT_WORKSPACE_BOUNDS 1u
+#define APGX_V1_FAULT_WORKSPACE_OWNER 2u
+#define APGX_V1_FAULT_QUERY_TELEMETRY 3u
+#define APGX_V1_FAULT_MEMORY_ACCOUNTING 4u
+#define APGX_V1_FAULT_BATCH_TELEMETRY 5u
+#define APGX_V1_FAULT_QUERY_IDENTITY 6u
+#define APGX_V1_FAULT_FALSE_DISCONNECTED 7u
+#define APGX_V1_FAULT_COMPACT_PATH_BOUNDS 8u
+#define APGX_V1_FAULT_COMPACT_PATH_ENDPOINT 9u
+#define APGX_V1_FAULT_COMPACT_PATH_CYCLE 10u
+#define APGX_V1_FAULT_COMPACT_PATH_HEADING 11u
+#define APGX_V1_FAULT_COMPACT_PATH_EDGE 12u
+int apgx_v1_pending_inject_fault(apgx_v1_pending* pending, uint32_t fault);
 void apgx_v1_pending_release(apgx_v1_pending* pending);
 
-/* Host trust boundary: re-checks an untrusted compact readback before use, as
- * RouteCandidateBatchWithPreparedPreflight does (src/gpu/planar_router.cc:
- * 969-1131 ValidateAndReconstructCompactGpuPath, 1808-1955 query and batch
- * telemetry, 629-667 CPU confirmation of a reported disconnection).  Checks, in
- * order: generator; batch telemetry identities (chunk size, one status readback
- * per chunk, launches = init + chunks + 3 path kernels); byte accounting; per
- * query the header echo, completion, rounds / examined work, path bounds, a
- * disconnection the CPU search contradicts, endpoints, cycles, headings,
- * edges and banned resources.  `valid` = 0 names the first failing query and
- * the reference's invariant identifier (e.g. "gpu.batch.compact_path.cycle.v1");
- * path_costs (may be NULL) receives each reached path's recomputed cost
+/* Host trust boundary: re-checks an untrusted readback before any result is
+ * used, as RouteCandidateBatchWithPreparedPreflight does (src/gpu/
+ * planar_router.cc:2195-2380).  Order: generator; batch query identity (every
+ * result names one admitted query, :2195-2218); batch telemetry (readback kind
+ * and flat bounds, compact contiguity, byte accounting, peak / chunk / launch /
+ * readback counts: ValidateCandidateBatchTelemetry, :1841-1955); then per query
+ * the envelope (header echo, workspace bounds, ownership stamps or compact
+ * header, round / work telemetry: ValidateCandidateBatchEnvelope, :1732-1839)
+ * and the result itself.  Compact paths (sweep): endpoints, cycles, headings,
+ * edges, banned resources and cost (:969-1131).  Full workspace (frontier):
+ * every predecessor exact for reached / disconnected and a lower bound
+ * otherwise (ValidateAllPredecessors, :491-612), the goal state as the least
+ * finite goal label and the predecessor walk to the start (:829-930), finite
+ * goal labels under "disconnected" (:614-627).  Every "disconnected" is
+ * confirmed by an exact CPU search (:629-667).
+ * `valid` = 0 names the first failing query (APGX_V1_INVALID_INDEX: the whole
+ * batch), the failure class (APGX_V1_CHECK_*) and the reference's invariant
+ * identifier where it defines one (e.g. "gpu.batch.compact_path.cycle.v1";
+ * empty otherwise; `detail` then carries the reference's message).
+ * path_costs (may be NULL) receives each reached query's validated cost
  * (UINT64_MAX otherwise).  The call returns APGX_OK unless an argument is null. */
+#define APGX_V1_CHECK_OK 0u
+#define APGX_V1_CHECK_INTERNAL_INVARIANT 1u  /* PlanarGpuFailureCode::kInternalInvariant */
+#define APGX_V1_CHECK_VALIDATION_FAILED 2u   /* PlanarGpuFailureCode::kValidationFailed */
 typedef struct apgx_v1_validation {
   uint32_t valid;
   uint32_t query_index;                    /* APGX_V1_INVALID_INDEX: batch-level */
+  uint32_t failure;                        /* APGX_V1_CHECK_* */
+  uint32_t reserved;
   char invariant[64];
   char detail[128];
 } apgx_v1_validation;
+
+/* The workspace half of a readback, as apgx_v1_batch_readback_workspace
+ * returns it; counts are the lengths of the flat arrays (0 and NULL for the
+ * compact form). */
+typedef struct apgx_v1_workspace_view {
+  uint32_t readback_kind;                  /* APGX_V1_READBACK_* */
+  uint32_t reserved;
+  const apgx_v1_query_envelope* queries;  /* query_count envelopes */
+  const uint64_t* labels;
+  uint64_t label_count;
+  const uint32_t* predecessors;
+  uint64_t predecessor_count;
+  const uint64_t* state_owners;
+  uint64_t state_owner_count;
+  const uint64_t* predecessor_owners;
+  uint64_t predecessor_owner_count;
+} apgx_v1_workspace_view;
 
 int apgx_v1_validate_batch(const apgx_v1_header* header, const apgx_v1_node* nodes, uint64_t node_count,
                            const apgx_v1_batch_request* request, const apgx_v1_query* queries,
                            uint32_t query_count, const apgx_v1_policy_edge* policy_edges,
                            uint64_t policy_edge_count, const apgx_v1_result* results,
                            const apgx_v1_compact_path* paths, const uint32_t* states,
-                           uint64_t state_count, const apgx_v1_batch_telemetry* telemetry,
-                           uint64_t* path_costs, apgx_v1_validation* out);
+                           uint64_t state_count, const apgx_v1_workspace_view* workspace,
+                           const apgx_v1_batch_telemetry* telemetry, uint64_t* path_costs,
+                           apgx_v1_validation* out);
 
 /* ------------------------------------------------------------------------ */
 /* Synthetic lattice generators (deterministic xoshiro256** streams)         */

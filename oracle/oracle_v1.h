@@ -70,6 +70,14 @@ int orc_v1_frontier(const orc_v1_node* nodes, uint64_t n, const orc_v1_profile* 
                     uint32_t* rounds_out, uint64_t* examined_out, uint32_t* states_out, uint64_t cap,
                     uint64_t* len_out);
 
+/* The frontier's full workspace after it stops (reached, disconnected or out
+ * of rounds): every state's label and lowest tight predecessor
+ * (UINT32_MAX: none), flattened numbering, n * 9 entries each. */
+int orc_v1_frontier_workspace(const orc_v1_node* nodes, uint64_t n, const orc_v1_profile* pr,
+                              const orc_v1_policy* po, uint32_t layer, int64_t sx, int64_t sy, int64_t gx,
+                              int64_t gy, uint32_t max_rounds, uint32_t* goal_state_out, uint32_t* rounds_out,
+                              uint64_t* labels_out, uint32_t* preds_out);
+
 #ifdef __cplusplus
 }
 #endif

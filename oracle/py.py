@@ -227,6 +227,10 @@ def _v1_setup():
                                   C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_uint32, P(C.c_uint64),
                                   P(C.c_uint32), P(C.c_uint32), P(C.c_uint64), P(C.c_uint32), C.c_uint64,
                                   P(C.c_uint64)]
+    h.orc_v1_frontier_workspace.restype = C.c_int
+    h.orc_v1_frontier_workspace.argtypes = [P(V1Node), C.c_uint64, P(V1Profile), P(V1Policy), C.c_uint32,
+                                            C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_uint32,
+                                            P(C.c_uint32), P(C.c_uint32), P(C.c_uint64), P(C.c_uint32)]
     h._v1_ready = True
     return h
 
@@ -300,3 +304,23 @@ def v1_frontier(o: "V1", start, goal, policy=None, layer=0, max_rounds=100_000):
     return {"status": rc, "cost": cost.value if rc == 0 else None, "goal_state": goal_state.value,
             "rounds": rounds.value, "examined_work": examined.value,
             "states": list(states)[: plen.value] if rc == 0 else []}
+
+
+def v1_frontier_workspace(o: "V1", start, goal, policy=None, layer=0, max_rounds=100_000):
+    """The frontier restatement's workspace when it stops: dict(status, goal_state,
+    rounds, labels, predecessors) with per-state arrays in flattened numbering
+    (predecessor 0xFFFFFFFF = none)."""
+    import numpy as np
+    h = _v1_setup()
+    goal_state, rounds = C.c_uint32(0), C.c_uint32(0)
+    labels = np.zeros(9 * o.n, np.uint64)
+    preds = np.zeros(9 * o.n, np.uint32)
+    rc = h.orc_v1_frontier_workspace(o.nodes, o.n, C.byref(o.profile),
+                                     C.byref(policy) if policy is not None else None, layer, start[0], start[1],
+                                     goal[0], goal[1], max_rounds, C.byref(goal_state), C.byref(rounds),
+                                     labels.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                     preds.ctypes.data_as(C.POINTER(C.c_uint32)))
+    if rc < 0:
+        raise RuntimeError("oracle v1 frontier workspace error")
+    return {"status": rc, "goal_state": goal_state.value, "rounds": rounds.value, "labels": labels,
+            "predecessors": preds}

@@ -429,11 +429,11 @@ done:
  * labels drop only on strict improvement).  Then the lowest-predecessor walk
  * of :1084-1165 on the resulting labels.  Returns 0 reached, 1 disconnected,
  * 2 round budget exhausted, -1 error. */
-int orc_v1_frontier(const orc_v1_node* nodes, uint64_t n, const orc_v1_profile* pr,
-                    const orc_v1_policy* po, uint32_t layer, int64_t sx, int64_t sy, int64_t gx,
-                    int64_t gy, uint32_t max_rounds, uint64_t* cost_out, uint32_t* goal_state_out,
-                    uint32_t* rounds_out, uint64_t* examined_out, uint32_t* states_out, uint64_t cap,
-                    uint64_t* len_out) {
+static int frontier_impl(const orc_v1_node* nodes, uint64_t n, const orc_v1_profile* pr,
+                         const orc_v1_policy* po, uint32_t layer, int64_t sx, int64_t sy, int64_t gx,
+                         int64_t gy, uint32_t max_rounds, uint64_t* cost_out, uint32_t* goal_state_out,
+                         uint32_t* rounds_out, uint64_t* examined_out, uint32_t* states_out, uint64_t cap,
+                         uint64_t* len_out, uint64_t* labels_out, uint32_t* preds_out) {
   uint32_t* order = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
   orc_v1_node* flat = (orc_v1_node*)malloc(sizeof(orc_v1_node) * (n ? n : 1));
   if (!order || !flat || orc_v1_flatten_order(nodes, n, pr->tile_w, pr->tile_h, order)) {
@@ -517,6 +517,31 @@ int orc_v1_frontier(const orc_v1_node* nodes, uint64_t n, const orc_v1_profile* 
     }
     if (len_out) *len_out = k;
   }
+  if (rc >= 0 && labels_out && preds_out) {
+    /* Final labels and the lowest tight predecessor of every state
+     * (SelectCandidateBatchPredecessors, gpu/cuda/planar_router.cu:1021-1082). */
+    const uint64_t start_state = (uint64_t)s * 9 + 8;
+    for (uint64_t st = 0; st < S; ++st) {
+      labels_out[st] = L[st];
+      preds_out[st] = UINT32_MAX;
+      const int in = (int)(st % 9);
+      if (st == start_state || in == 8 || L[st] == INF64) continue;
+      const uint64_t v = st / 9;
+      const int opp = (in + 4) % 8;
+      const int64_t p = lookup(&lk, layer, flat[v].x + DX[opp], flat[v].y + DY[opp]);
+      if (p < 0 || !(flat[p].mask & (1u << in))) continue;
+      uint64_t chosen = INF64;
+      for (int ph = 0; ph < 9; ++ph) {
+        if (ph == 8 && p != s) continue;
+        const uint64_t ps = (uint64_t)p * 9 + ph;
+        uint64_t w, c;
+        if (L[ps] == INF64 || transition(pr, po, flat[p].x, flat[p].y, in, ph, &w)) continue;
+        if (checked_add(L[ps], w, &c)) continue;
+        if (c == L[st] && ps < chosen) chosen = ps;
+      }
+      if (chosen != INF64) preds_out[st] = (uint32_t)chosen;
+    }
+  }
 done:
   if (rounds_out) *rounds_out = rounds;
   if (examined_out) *examined_out = examined;
@@ -526,5 +551,27 @@ done:
   free(lk.keys);
   free(order);
   free(flat);
+  return rc;
+}
+
+int orc_v1_frontier(const orc_v1_node* nodes, uint64_t n, const orc_v1_profile* pr,
+                    const orc_v1_policy* po, uint32_t layer, int64_t sx, int64_t sy, int64_t gx,
+                    int64_t gy, uint32_t max_rounds, uint64_t* cost_out, uint32_t* goal_state_out,
+                    uint32_t* rounds_out, uint64_t* examined_out, uint32_t* states_out, uint64_t cap,
+                    uint64_t* len_out) {
+  return frontier_impl(nodes, n, pr, po, layer, sx, sy, gx, gy, max_rounds, cost_out, goal_state_out,
+                       rounds_out, examined_out, states_out, cap, len_out, NULL, NULL);
+}
+
+int orc_v1_frontier_workspace(const orc_v1_node* nodes, uint64_t n, const orc_v1_profile* pr,
+                              const orc_v1_policy* po, uint32_t layer, int64_t sx, int64_t sy, int64_t gx,
+                              int64_t gy, uint32_t max_rounds, uint32_t* goal_state_out, uint32_t* rounds_out,
+                              uint64_t* labels_out, uint32_t* preds_out) {
+  uint64_t len = 0;
+  uint32_t* states = (uint32_t*)malloc(sizeof(uint32_t) * (n * 9 + 2));
+  if (!states) return -1;
+  const int rc = frontier_impl(nodes, n, pr, po, layer, sx, sy, gx, gy, max_rounds, NULL, goal_state_out,
+                               rounds_out, NULL, states, n * 9 + 2, &len, labels_out, preds_out);
+  free(states);
   return rc;
 }

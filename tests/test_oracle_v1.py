@@ -103,6 +103,66 @@ def test_frontier_restatement_hand_worked(oracle):
     assert (f["status"], f["rounds"], f["examined_work"]) == (2, 4, 7)
 
 
+def test_frontier_workspace_hand_worked(oracle):
+    """Workspace of the corridor frontier above when it closes the goal: each
+    closed (k, E) state relaxed (k+1, E) to 10(k+1) and (k-1, W) to
+    10k + 10 + 3 (a turn); (9, W) would come from the goal, which is never
+    expanded.  Predecessors are the relaxing states; the start's is none."""
+    p, s, g = vc.known_answer_boards()["horizontal_corridor"]
+    _, o = _oracle_for(oracle, p)
+    w = oracle.v1_frontier_workspace(o, vc.lattice(p, s), vc.lattice(p, g))
+    lab, pre = w["labels"], w["predecessors"]
+    assert (w["status"], w["goal_state"], w["rounds"]) == (0, 90, 11)
+    inf, none = (1 << 64) - 1, 0xFFFFFFFF
+    want_lab = {8: 0}
+    want_pre = {}
+    for k in range(1, 11):
+        want_lab[9 * k] = 10 * k
+        want_pre[9 * k] = 9 * (k - 1) if k > 1 else 8
+    for k in range(0, 9):
+        want_lab[9 * k + 4] = 10 * k + 23
+        want_pre[9 * k + 4] = 9 * (k + 1)
+    for st in range(len(lab)):
+        assert int(lab[st]) == want_lab.get(st, inf), st
+        assert int(pre[st]) == want_pre.get(st, none), st
+
+
+@pytest.mark.parametrize("name", sorted(vc.corpus()))
+def test_frontier_workspace_is_consistent(oracle, name):
+    """On every corpus case: the walk along the workspace predecessors from the
+    goal state is the frontier's path, the goal label is the published cost,
+    and every finite label is its predecessor's label plus the step cost
+    (orthogonal / diagonal step, plus the bend on a heading change)."""
+    profile, start, goal = vc.corpus()[name]
+    nodes, o = _oracle_for(oracle, profile)
+    s, t = vc.lattice(profile, start), vc.lattice(profile, goal)
+    w = oracle.v1_frontier_workspace(o, s, t)
+    f = oracle.v1_frontier(o, s, t)
+    lab, pre = [int(v) for v in w["labels"]], [int(v) for v in w["predecessors"]]
+    assert (w["status"], w["rounds"]) == (f["status"], f["rounds"])
+    orth, diag, bend = profile.costs
+    start_state = next(st for st in range(len(lab)) if lab[st] == 0)
+    assert start_state % 9 == 8 and pre[start_state] == 0xFFFFFFFF
+    for st, (l, pr) in enumerate(zip(lab, pre)):
+        if st == start_state or l == (1 << 64) - 1:
+            assert pr == 0xFFFFFFFF
+            continue
+        h, ph = st % 9, pr % 9
+        assert h != 8 and pr != 0xFFFFFFFF
+        step = (diag if h % 2 else orth) + (bend if ph != 8 and ph != h else 0)
+        assert lab[pr] + step == l, (st, pr)
+    if f["status"] == 0:
+        cur, walk = f["goal_state"], []
+        while True:
+            walk.append(cur)
+            if cur == start_state:
+                break
+            cur = pre[cur]
+        assert walk == f["states"]
+        gold = GOLDEN[name]
+        assert lab[f["goal_state"]] == gold.get("cuda_frontier_path_cost", gold.get("base_policy_scalar_cost"))
+
+
 @pytest.mark.parametrize("name", sorted(vc.corpus()))
 def test_frontier_restatement_costs_and_paths(oracle, name):
     """The frontier's costs are the reference's published optimal costs and its

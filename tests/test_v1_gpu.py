@@ -1,9 +1,11 @@
-"""GPU parity of the V1 heading-aware sweep on the reference corpus.
+"""GPU parity of the V1 generators on the reference corpus.
 
-Checks every query's reverse compact path against the oracle's
-lowest-predecessor path, recomputes the path cost and compares it with the
-reference's published optimal cost, and compares the round count and examined
-work with the reference's published sweep telemetry.
+Sweep: every query's reverse compact path against the oracle's
+lowest-predecessor path, the recomputed path cost against the reference's
+published optimal cost, and the round count and examined work against the
+reference's published sweep telemetry.  Frontier: the full-workspace readback
+(every state's label and predecessor) bit-exact against the frontier
+restatement, plus rounds, examined work, goal state and the predecessor walk.
 """
 import json
 from pathlib import Path
@@ -50,7 +52,7 @@ def test_corpus_case(apgx, oracle, gpu, name):
     s_xy, g_xy = vc.lattice(profile, start), vc.lattice(profile, goal)
     s, g = b.node_index(*s_xy), b.node_index(*g_xy)
     view = v1.View(b)
-    results, paths, states, tel = view.execute([v1.make_query(0, s, g, b)])
+    results, paths, states, tel, _ = view.execute([v1.make_query(0, s, g, b)])
     r, p = results[0], paths[0]
     gold = GOLDEN[name]
     o = oracle.V1(nodes, profile.costs, profile.heading, profile.tile_w, profile.tile_h)
@@ -83,7 +85,7 @@ def test_policy_known_answers(apgx, gpu):
     qs = [v1.make_query(0, s, g, b),
           v1.make_query(1, s, g, b, policy_offset=0, policy_count=1),
           v1.make_query(2, s, g, b, policy_offset=1, policy_count=1)]
-    results, paths, states, _ = view.execute(qs, edges)
+    results, paths, states, _, _ = view.execute(qs, edges)
     costs = []
     for i, (r, p) in enumerate(zip(results, paths)):
         assert r.completion == v1.REACHED
@@ -102,7 +104,7 @@ def test_policy_known_answers(apgx, gpu):
     qs = [v1.make_query(0, s, g, b, policy_count=1, surcharges=(2, 3, 5)),
           v1.make_query(1, g, s, b, policy_count=1, surcharges=(2, 3, 5)),
           v1.make_query(2, s, g, b, policy_offset=1, policy_count=1)]
-    results, paths, states, _ = view.execute(qs, edges)
+    results, paths, states, _, _ = view.execute(qs, edges)
     for i in (0, 1):
         p = paths[i]
         got = states[p.state_offset:p.state_offset + p.state_count]
@@ -116,7 +118,7 @@ def test_cheap_diagonal_cost(apgx, oracle, gpu):
     nodes, b = board_for(profile)
     s, g = b.node_index(*vc.lattice(profile, start)), b.node_index(*vc.lattice(profile, goal))
     view = v1.View(b)
-    results, paths, states, _ = view.execute([v1.make_query(0, s, g, b)])
+    results, paths, states, _, _ = view.execute([v1.make_query(0, s, g, b)])
     p = paths[0]
     got = states[p.state_offset:p.state_offset + p.state_count]
     assert path_cost(b, got, profile.costs) == 37
@@ -134,7 +136,7 @@ def test_batch_of_random_queries(apgx, oracle, gpu):
     pairs = [tuple(rng.choice(n, size=2, replace=False)) for _ in range(512)]
     qs = [v1.make_query(i, int(s), int(g), b) for i, (s, g) in enumerate(pairs)]
     view = v1.View(b)
-    results, paths, states, tel = view.execute(qs)
+    results, paths, states, tel, _ = view.execute(qs)
     o = oracle.V1(nodes, profile.costs, profile.heading, profile.tile_w, profile.tile_h)
     for i in range(0, 512, 37):
         s, g = pairs[i]
@@ -153,31 +155,43 @@ def test_cancel_and_budget(apgx, gpu):
     _, b = board_for(profile)
     s, g = b.node_index(*vc.lattice(profile, start)), b.node_index(*vc.lattice(profile, goal))
     view = v1.View(b)
-    r, p, _, _ = view.execute([v1.make_query(0, s, g, b)], cancel=True)
+    r, p, _, _, _ = view.execute([v1.make_query(0, s, g, b)], cancel=True)
     assert r[0].completion == v1.CANCELLED and p[0].state_count == 0
-    r, p, _, tel = view.execute([v1.make_query(0, s, g, b, maximum_rounds=2)], maximum_rounds=2)
+    r, p, _, tel, _ = view.execute([v1.make_query(0, s, g, b, maximum_rounds=2)], maximum_rounds=2)
     assert r[0].completion == v1.BUDGET_EXHAUSTED and r[0].rounds == 2
     assert tel.finalization_launch_count == 1
     view.close()
 
 
-def _frontier_check(b, o, s_xy, g_xy, r, p, states, policy=None):
-    f = vc_frontier(o, s_xy, g_xy, policy)
-    assert r.rounds == f["rounds"] and r.examined_work == f["examined_work"]
-    assert r.heading_turn_relaxations == 0
-    if f["status"] == 1:
-        assert r.completion == v1.DISCONNECTED and p.state_count == 0
-        return None
-    assert r.completion == v1.REACHED and r.goal_state == f["goal_state"]
-    got = states[p.state_offset:p.state_offset + p.state_count]
-    assert got == f["states"]
-    return got
-
-
-def vc_frontier(o, s_xy, g_xy, policy=None):
+def _frontier_check(b, o, s_xy, g_xy, rb, qi, policy=None, max_rounds=100_000):
+    """One frontier query of a full-workspace readback against the
+    restatement: rounds, examined work, goal state, every state's label and
+    predecessor (bit-exact), ownership stamps, and the predecessor walk."""
     from oracle import py as oracle_py
 
-    return oracle_py.v1_frontier(o, s_xy, g_xy, policy)
+    f = oracle_py.v1_frontier(o, s_xy, g_xy, policy, max_rounds=max_rounds)
+    w = oracle_py.v1_frontier_workspace(o, s_xy, g_xy, policy, max_rounds=max_rounds)
+    r, ws = rb.results[qi], rb.workspace
+    assert ws.readback_kind == v1.READBACK_FULL_WORKSPACE and rb.states == []
+    env = ws.envelopes[qi]
+    lo, n = env.workspace_offset, env.workspace_state_count
+    assert (env.rounds, env.examined_work) == (r.rounds, r.examined_work)
+    assert r.rounds == f["rounds"] and r.examined_work == f["examined_work"]
+    assert r.heading_turn_relaxations == 0
+    assert np.array_equal(ws.labels[lo:lo + n], w["labels"])
+    assert np.array_equal(ws.predecessors[lo:lo + n], w["predecessors"])
+    assert (ws.state_owners[lo:lo + n] == r.workspace_owner).all()
+    assert (ws.predecessor_owners[lo:lo + n] == r.workspace_owner).all()
+    if f["status"] == 2:
+        assert r.completion == v1.BUDGET_EXHAUSTED
+        return None
+    if f["status"] == 1:
+        assert r.completion == v1.DISCONNECTED
+        return None
+    assert r.completion == v1.REACHED and r.goal_state == f["goal_state"]
+    got = v1.reconstruct(ws, qi, r)
+    assert got == f["states"]
+    return got
 
 
 @pytest.mark.parametrize("name", sorted(vc.corpus()))
@@ -192,10 +206,11 @@ def test_frontier_corpus_case(apgx, oracle, gpu, name):
     s, g = b.node_index(*s_xy), b.node_index(*g_xy)
     view = v1.View(b)
     q = v1.make_query(0, s, g, b, generator=v1.GENERATOR_FRONTIER)
-    results, paths, states, tel = view.execute([q], generator=v1.GENERATOR_FRONTIER)
+    rb = view.execute([q], generator=v1.GENERATOR_FRONTIER)
+    results, tel = rb.results, rb.telemetry
     o = oracle.V1(nodes, profile.costs, profile.heading, profile.tile_w, profile.tile_h)
-    got = _frontier_check(b, o, s_xy, g_xy, results[0], paths[0], states)
-    sres, spaths, sstates, _ = view.execute([v1.make_query(0, s, g, b)])
+    got = _frontier_check(b, o, s_xy, g_xy, rb, 0)
+    sres, spaths, sstates, _, _ = view.execute([v1.make_query(0, s, g, b)])
     if got is not None:
         gold = GOLDEN[name]
         want = gold.get("cuda_frontier_path_cost", gold.get("base_policy_scalar_cost"))
@@ -228,8 +243,9 @@ def test_frontier_batch_with_policies(apgx, oracle, gpu):
         qs_f.append(v1.make_query(i, s, g, b, policy_count=pc, surcharges=sur, generator=v1.GENERATOR_FRONTIER))
         qs_s.append(v1.make_query(i, s, g, b, policy_count=pc, surcharges=sur))
     view = v1.View(b)
-    fr, fp, fs, _ = view.execute(qs_f, edges, generator=v1.GENERATOR_FRONTIER)
-    sr, sp, ss, _ = view.execute(qs_s, edges)
+    frb = view.execute(qs_f, edges, generator=v1.GENERATOR_FRONTIER)
+    fr = frb.results
+    sr, sp, ss, _, _ = view.execute(qs_s, edges)
     o = oracle.V1(nodes, profile.costs, profile.heading, profile.tile_w, profile.tile_h)
     for i, (s, g, sur, pc) in enumerate(pairs):
         s_xy = (b.nodes[s].lattice_x, b.nodes[s].lattice_y)
@@ -240,7 +256,7 @@ def test_frontier_batch_with_policies(apgx, oracle, gpu):
             key = (b.nodes[node].lattice_x, b.nodes[node].lattice_y, d)
             (bans.append(key) if e.adjustment == v1.BANNED else pens.append((key, e.adjustment)))
         pol = oracle.V1.policy(*sur, bans=bans, penalties=pens)
-        got = _frontier_check(b, o, s_xy, g_xy, fr[i], fp[i], fs, pol)
+        got = _frontier_check(b, o, s_xy, g_xy, frb, i, pol)
         assert fr[i].completion == sr[i].completion
         if got is not None:
             sw = ss[sp[i].state_offset:sp[i].state_offset + sp[i].state_count]
@@ -249,13 +265,19 @@ def test_frontier_batch_with_policies(apgx, oracle, gpu):
     view.close()
 
 
-def test_frontier_budget(apgx, gpu):
+def test_frontier_budget(apgx, oracle, gpu):
+    """A 40-round budget: the partial workspace (labels of open states are
+    upper bounds) equals the restatement's after the same 40 rounds."""
     profile, start, goal = vc.corpus()["high_turn_maze"]
-    _, b = board_for(profile)
+    nodes, b = board_for(profile)
     s, g = b.node_index(*vc.lattice(profile, start)), b.node_index(*vc.lattice(profile, goal))
     view = v1.View(b)
     q = v1.make_query(0, s, g, b, maximum_rounds=40, generator=v1.GENERATOR_FRONTIER)
-    r, p, _, tel = view.execute([q], maximum_rounds=40, generator=v1.GENERATOR_FRONTIER)
-    assert r[0].completion == v1.BUDGET_EXHAUSTED and r[0].rounds == 40 and p[0].state_count == 0
+    rb = view.execute([q], maximum_rounds=40, generator=v1.GENERATOR_FRONTIER)
+    r, tel = rb.results, rb.telemetry
+    assert r[0].completion == v1.BUDGET_EXHAUSTED and r[0].rounds == 40
     assert tel.finalization_launch_count == 1 and tel.dispatched_rounds == 40
+    assert tel.kernel_launch_count == 2 + 2 + 1  # init + select, two chunks, finalization
+    o = oracle.V1(nodes, profile.costs, profile.heading, profile.tile_w, profile.tile_h)
+    _frontier_check(b, o, vc.lattice(profile, start), vc.lattice(profile, goal), rb, 0, max_rounds=40)
     view.close()

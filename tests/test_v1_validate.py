@@ -1,16 +1,33 @@
 """Host trust boundary (apgx_v1_validate_batch) against corrupted readbacks.
 
-CPU: a clean readback assembled from the oracle's canonical paths must pass
-with the published path costs, and each fault of the reference's decorator
-must be reported under the reference's invariant id.  GPU: the same on real
-readbacks of both generators."""
+CPU: clean readbacks assembled from the oracle -- compact canonical paths for
+the sweep, the frontier restatement's full workspace for the frontier -- must
+pass with the published path costs, and each fault of the reference's
+decorator must be reported under the reference's invariant id.  GPU: the same
+on real readbacks of both generators."""
 import numpy as np
 import pytest
 
 from apgar_amd import v1
 from tests import v1_corpus as vc
 from tests.test_v1_board import board_for
-from tests.v1_faults import FAULTS, inject
+from tests.v1_faults import BATCH_LEVEL, CONTRACT, FAULTS, inject
+
+INF = (1 << 64) - 1
+
+
+def layout_bytes(frontier, qn, pn, nn):
+    """Device bytes of one batch block (the section list of LayoutBatch):
+    256-byte aligned queries, policy, results, labels, then predecessors and
+    owners and closed flags (frontier) or path headers, packed states and
+    departures (sweep)."""
+    al = lambda v: (v + 255) & ~255
+    S = qn * nn * 9
+    off = al(al(al(qn * 120) + max(pn, 1) * 16) + qn * 120)  # labels
+    tail = al(off + S * 8)
+    if frontier:
+        return al(al(al(tail + S * 4) + S * 8) + S * 8) + S
+    return al(al(tail + qn * 32) + S * 4) + qn * nn * 64
 
 
 def _pairs(b, rng, k):
@@ -18,78 +35,189 @@ def _pairs(b, rng, k):
     return [tuple(int(v) for v in rng.choice(n, size=2, replace=False)) for _ in range(k)]
 
 
-def oracle_readback(oracle, name, rounds=5):
+def _telemetry(b, frontier, n, dispatched, compact_states, policy_count=0):
+    t = v1.BatchTelemetry()
+    nn = len(b.nodes)
+    S = n * nn * 9
+    t.persistent_device_bytes = b.header.estimated_persistent_device_bytes
+    t.batch_device_bytes = t.workspace_capacity_device_bytes = layout_bytes(frontier, n, policy_count, nn)
+    t.peak_device_bytes = t.persistent_device_bytes + t.workspace_capacity_device_bytes
+    readback = n * 120 + (S * 28 if frontier else n * 32 + 4 * compact_states)
+    t.device_to_host_readback_bytes = readback
+    t.batch_host_bytes = readback + n * 120
+    chunk = 32 if frontier else 8
+    t.chunk_rounds = chunk
+    t.dispatched_rounds = dispatched
+    t.blocking_status_readback_count = -(-dispatched // chunk)
+    t.kernel_launch_count = (2 if frontier else 4) + t.blocking_status_readback_count
+    return t
+
+
+def oracle_readback(oracle, name, generator, k=4):
+    """A clean readback of (the case's own query + k random pairs) made from the
+    oracle; returns (board, request, queries, readback, costs)."""
+    from oracle import py as oracle_py
     profile, start, goal = vc.corpus()[name]
     nodes, b = board_for(profile)
     o = oracle.V1(nodes, profile.costs, profile.heading, profile.tile_w, profile.tile_h)
     s0, g0 = b.node_index(*vc.lattice(profile, start)), b.node_index(*vc.lattice(profile, goal))
-    pairs = [(s0, g0)] + _pairs(b, np.random.default_rng(5), 4)
+    pairs = [(s0, g0)] + _pairs(b, np.random.default_rng(5), k)
+    nn = len(b.nodes)
+    S = nn * 9
     legal = sum(bin(n.legal_edges).count("1") for n in b.nodes)
-    queries, results, paths, states, costs = [], [], [], [], []
+    frontier = generator == v1.GENERATOR_FRONTIER
+    queries, results, paths, states, costs, envs = [], [], [], [], [], []
+    labels = np.zeros(0, np.uint64)
+    preds = np.zeros(0, np.uint32)
+    max_rounds = 0
+    xy = lambda i: (b.nodes[i].lattice_x, b.nodes[i].lattice_y)
     for i, (s, g) in enumerate(pairs):
-        q = v1.make_query(i, s, g, b)
+        q = v1.make_query(i, s, g, b, generator=generator)
         queries.append(q)
-        cost, st = o.canonical((b.nodes[s].lattice_x, b.nodes[s].lattice_y), (b.nodes[g].lattice_x, b.nodes[g].lattice_y))
         r = v1.Result()
         r.schema_version, r.compiler_version, r.input_ordinal = 1, b.header.compiler_version, i
-        r.start_node, r.goal_node, r.generator = s, g, v1.GENERATOR_SWEEP
-        r.rounds, r.examined_work = rounds, rounds * legal
+        r.start_node, r.goal_node, r.generator = s, g, generator
         r.batch_id, r.query_id, r.workspace_owner = q.batch_id, q.query_id, q.workspace_owner
         r.source_board_content_hash = b.header.source_board_content_hash
         r.compiler_profile_fingerprint = b.header.compiler_profile_fingerprint
         r.rule_bucket_identity = b.header.rule_bucket_identity
         r.device_view_fingerprint = b.header.device_view_fingerprint
         p = v1.CompactPath()
-        p.schema_version, p.query_id, p.workspace_owner = 1, q.query_id, q.workspace_owner
-        if cost is None:
-            r.completion, r.goal_state = v1.DISCONNECTED, 0xFFFFFFFF
+        if frontier:
+            f = oracle_py.v1_frontier(o, xy(s), xy(g))
+            w = oracle_py.v1_frontier_workspace(o, xy(s), xy(g))
+            assert f["status"] == w["status"] and f["rounds"] == w["rounds"]
+            r.rounds, r.examined_work = f["rounds"], f["examined_work"]
+            r.completion = v1.REACHED if f["status"] == 0 else v1.DISCONNECTED
+            r.goal_state = f["goal_state"] if f["status"] == 0 else 0xFFFFFFFF
+            labels = np.concatenate([labels, w["labels"]])
+            preds = np.concatenate([preds, w["predecessors"]])
+            costs.append(f["cost"])
         else:
-            r.completion, r.goal_state = v1.REACHED, st[0]
-            p.state_offset, p.state_count = len(states), len(st)
-            states.extend(st)
+            cost, st = o.canonical(xy(s), xy(g))
+            r.rounds = 5
+            r.examined_work = r.rounds * legal
+            p.schema_version, p.query_id, p.workspace_owner = 1, q.query_id, q.workspace_owner
+            if cost is None:
+                r.completion, r.goal_state = v1.DISCONNECTED, 0xFFFFFFFF
+            else:
+                r.completion, r.goal_state = v1.REACHED, st[0]
+                p.state_offset, p.state_count = len(states), len(st)
+                states.extend(st)
+            costs.append(cost)
+        max_rounds = max(max_rounds, r.rounds)
+        envs.append(v1.QueryEnvelope(i * S, S, r.examined_work, r.heading_turn_relaxations, r.rounds, 0))
         results.append(r)
         paths.append(p)
-        costs.append(cost)
-    t = v1.BatchTelemetry()
-    t.persistent_device_bytes = b.header.estimated_persistent_device_bytes
-    t.batch_device_bytes = t.workspace_capacity_device_bytes = 1 << 20
-    t.peak_device_bytes = t.persistent_device_bytes + t.workspace_capacity_device_bytes
     n = len(queries)
-    t.batch_host_bytes = n * (120 + 32 + 120) + 4 * len(states)
-    t.device_to_host_readback_bytes = n * (120 + 32) + 4 * len(states)
-    t.kernel_launch_count, t.blocking_status_readback_count = 5, 1
-    t.dispatched_rounds, t.chunk_rounds = 8, 8
-    return b, queries, results, paths, states, t, costs
+    if frontier:
+        ws = v1.Workspace(v1.READBACK_FULL_WORKSPACE, envs, labels, preds,
+                          np.array([q.workspace_owner for q in queries for _ in range(S)], np.uint64),
+                          np.array([q.workspace_owner for q in queries for _ in range(S)], np.uint64))
+        dispatched = -(-max_rounds // 32) * 32
+    else:
+        ws = v1.Workspace(v1.READBACK_COMPACT_PATHS, envs, *(np.zeros(0, d) for d in
+                                                            (np.uint64, np.uint32, np.uint64, np.uint64)))
+        dispatched = 8
+    tel = _telemetry(b, frontier, n, dispatched, len(states))
+    req = v1.request_for(b, generator=generator)
+    return b, req, queries, v1.Readback(results, paths, states, tel, ws), costs
 
 
-def test_clean_oracle_readback_passes(apgx, oracle):
-    b, queries, results, paths, states, t, costs = oracle_readback(oracle, "negative_coordinates")
-    ok, qi, inv, detail, got = v1.validate(b, v1.request_for(b), queries, [], results, paths, states, t)
-    assert ok, (qi, inv, detail)
+def _validate(b, req, queries, rb, policy=()):
+    return v1.validate(b, req, queries, list(policy), rb.results, rb.paths, rb.states, rb.telemetry, rb.workspace)
+
+
+@pytest.mark.parametrize("generator", [v1.GENERATOR_SWEEP, v1.GENERATOR_FRONTIER])
+def test_clean_oracle_readback_passes(apgx, oracle, generator):
+    b, req, queries, rb, costs = oracle_readback(oracle, "negative_coordinates", generator)
+    ok, qi, inv, detail, got, failure = _validate(b, req, queries, rb)
+    assert ok and failure == v1.CHECK_OK, (qi, inv, detail)
     assert got[0] == 370  # the published optimal cost of the case's own query
-    assert got == [c if c is not None else (1 << 64) - 1 for c in costs]
+    assert got == [c if c is not None else INF for c in costs]
 
 
 @pytest.mark.parametrize("fault", FAULTS)
 def test_each_fault_is_reported(apgx, oracle, fault):
-    b, queries, results, paths, states, t, _ = oracle_readback(oracle, "high_turn_maze")
-    r2, p2, s2, t2, want = inject(fault, results, paths, states, t)
-    ok, qi, inv, detail, _ = v1.validate(b, v1.request_for(b), queries, [], r2, p2, s2, t2)
-    assert not ok and inv == want, (inv, detail)
-    assert qi == (0xFFFFFFFF if want in ("gpu.batch.telemetry.v1", "gpu.batch.memory.accounting.v1") else 0)
+    generator, want = CONTRACT[fault]
+    b, req, queries, rb, _ = oracle_readback(oracle, "high_turn_maze", generator)
+    ok, qi, inv, detail, _, failure = _validate(b, req, queries, inject(fault, rb))
+    assert not ok and inv == want and failure == v1.CHECK_INTERNAL_INVARIANT, (inv, detail)
+    assert qi == (0xFFFFFFFF if want in BATCH_LEVEL else 0)
 
 
-def test_banned_resource_on_path_is_reported(apgx, oracle):
-    b, queries, results, paths, states, t, _ = oracle_readback(oracle, "high_turn_maze")
-    p = paths[0]
-    rev = states[p.state_offset:p.state_offset + p.state_count]
+def test_fault_leaves_original_intact(apgx, oracle):
+    b, req, queries, rb, _ = oracle_readback(oracle, "high_turn_maze", v1.GENERATOR_FRONTIER)
+    inject("false_disconnected", rb)
+    inject("workspace_owner", rb)
+    assert _validate(b, req, queries, rb)[0]
+
+
+def test_banned_resource_on_compact_path_is_reported(apgx, oracle):
+    b, req, queries, rb, _ = oracle_readback(oracle, "high_turn_maze", v1.GENERATOR_SWEEP)
+    p = rb.paths[0]
+    rev = rb.states[p.state_offset:p.state_offset + p.state_count]
     pred, nxt = rev[-1], rev[-2]  # first step from the start
     node, d = pred // 9, nxt % 9
     edge = node * 8 + d if d < 4 else b.nodes[node].neighbors[d] * 8 + d - 4
     queries[0].policy_offset, queries[0].policy_count = 0, 1
-    ok, qi, inv, _, _ = v1.validate(b, v1.request_for(b), queries, [v1.PolicyEdge(edge, 0, v1.BANNED)], results,
-                                    paths, states, t)
+    rb.telemetry.batch_device_bytes = rb.telemetry.workspace_capacity_device_bytes = \
+        layout_bytes(False, len(queries), 1, len(b.nodes))
+    rb.telemetry.peak_device_bytes = rb.telemetry.persistent_device_bytes + rb.telemetry.workspace_capacity_device_bytes
+    ok, qi, inv, _, _, _ = _validate(b, req, queries, rb, [v1.PolicyEdge(edge, 0, v1.BANNED)])
     assert not ok and qi == 0 and inv == "gpu.policy.banned_predecessor.v1"
+
+
+def test_banned_resource_under_a_workspace_predecessor_is_reported(apgx, oracle):
+    """A full-workspace predecessor that crosses a banned edge
+    (ValidateAllPredecessors' policy branch)."""
+    b, req, queries, rb, _ = oracle_readback(oracle, "high_turn_maze", v1.GENERATOR_FRONTIER)
+    ws = rb.workspace
+    S = len(b.nodes) * 9
+    goal = rb.results[0].goal_state
+    pred = int(ws.predecessors[goal])
+    node, d = pred // 9, goal % 9
+    edge = node * 8 + d if d < 4 else b.nodes[node].neighbors[d] * 8 + d - 4
+    queries[0].policy_offset, queries[0].policy_count = 0, 1
+    t = rb.telemetry
+    t.batch_device_bytes = t.workspace_capacity_device_bytes = layout_bytes(True, len(queries), 1, len(b.nodes))
+    t.peak_device_bytes = t.persistent_device_bytes + t.workspace_capacity_device_bytes
+    assert goal < S
+    ok, qi, inv, _, _, _ = _validate(b, req, queries, rb, [v1.PolicyEdge(edge, 0, v1.BANNED)])
+    assert not ok and qi == 0 and inv == "gpu.policy.banned_predecessor.v1"
+
+
+def test_goal_predecessor_self_cycle_is_reported(apgx, oracle):
+    """The single-route replay's fault (goal_predecessor_self_cycle_v1.replay:
+    predecessors[goal] = goal) under the full-workspace checks."""
+    b, req, queries, rb, _ = oracle_readback(oracle, "high_turn_maze", v1.GENERATOR_FRONTIER)
+    rb.workspace.predecessors[rb.results[0].goal_state] = rb.results[0].goal_state
+    ok, qi, inv, _, _, failure = _validate(b, req, queries, rb)
+    assert not ok and qi == 0 and inv == "gpu.predecessor.self_reference.v1"
+    assert failure == v1.CHECK_INTERNAL_INVARIANT
+
+
+def test_wrong_goal_state_and_finite_disconnected_goal_are_reported(apgx, oracle):
+    b, req, queries, rb, _ = oracle_readback(oracle, "high_turn_maze", v1.GENERATOR_FRONTIER)
+    r2 = inject("query_identity", rb)  # only to get deep copies
+    r2.results[0].query_id -= 1
+    r2.results[0].goal_state = r2.results[0].goal_node * 9 + (r2.results[0].goal_state % 9 + 1) % 8
+    ok, qi, inv, detail, _, _ = _validate(b, req, queries, r2)
+    assert not ok and qi == 0 and "goal state" in detail
+    r3 = inject("query_identity", rb)
+    r3.results[0].query_id -= 1
+    r3.results[0].completion = v1.DISCONNECTED
+    ok, qi, inv, detail, _, _ = _validate(b, req, queries, r3)
+    assert not ok and qi == 0 and "finite goal label" in detail
+
+
+def test_association_drift_is_a_validation_failure(apgx, oracle):
+    b, req, queries, rb, _ = oracle_readback(oracle, "high_turn_maze", v1.GENERATOR_SWEEP)
+    r2 = inject("query_identity", rb)
+    r2.results[0].query_id -= 1
+    r2.results[0].policy_identity ^= 1
+    ok, qi, inv, detail, _, failure = _validate(b, req, queries, r2)
+    assert not ok and qi == 0 and failure == v1.CHECK_VALIDATION_FAILED and inv == ""
 
 
 @pytest.mark.gpu
@@ -102,13 +230,53 @@ def test_gpu_readbacks_validate_and_faults_are_caught(apgx, gpu, generator):
     pairs += _pairs(b, np.random.default_rng(11), 31)
     queries = [v1.make_query(i, s, g, b, generator=generator) for i, (s, g) in enumerate(pairs)]
     view = v1.View(b)
-    results, paths, states, tel = view.execute(queries, generator=generator)
+    rb = view.execute(queries, generator=generator)
+    view.close()
     req = v1.request_for(b, generator=generator)
-    ok, qi, inv, detail, costs = v1.validate(b, req, queries, [], results, paths, states, tel)
+    ok, qi, inv, detail, costs, _ = _validate(b, req, queries, rb)
     assert ok, (qi, inv, detail)
     assert costs[0] == 140  # published optimal cost of dense_corridors
     for fault in FAULTS:
-        r2, p2, s2, t2, want = inject(fault, results, paths, states, tel)
-        ok, qi, inv, detail, _ = v1.validate(b, req, queries, [], r2, p2, s2, t2)
-        assert not ok and inv == want, (fault, inv, detail)
+        if CONTRACT[fault][0] != generator:
+            continue
+        ok, qi, inv, detail, _, _ = _validate(b, req, queries, inject(fault, rb))
+        assert not ok and inv == CONTRACT[fault][1], (fault, inv, detail)
+
+
+def _same_readback(a, b):
+    raw = lambda x: bytes(memoryview(x))
+    assert [raw(r) for r in a.results] == [raw(r) for r in b.results]
+    assert [raw(p) for p in a.paths] == [raw(p) for p in b.paths]
+    assert list(a.states) == list(b.states)
+    ta, tb = raw(a.telemetry)[:-8], raw(b.telemetry)[:-8]  # kernel_milliseconds is timing
+    assert ta == tb
+    wa, wb = a.workspace, b.workspace
+    assert wa.readback_kind == wb.readback_kind
+    assert [raw(e) for e in wa.envelopes] == [raw(e) for e in wb.envelopes]
+    for x, y in ((wa.labels, wb.labels), (wa.predecessors, wb.predecessors), (wa.state_owners, wb.state_owners),
+                 (wa.predecessor_owners, wb.predecessor_owners)):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.gpu
+def test_library_decorator_matches_restatement(apgx, gpu):
+    """apgx_v1_pending_inject_fault corrupts a real readback exactly as the
+    decorator restatement above does, for every fault with its generator."""
+    print("[case] v1 library fault decorator", flush=True)
+    profile, start, goal = vc.corpus()["high_turn_maze"]
+    _, b = board_for(profile)
+    pairs = [(b.node_index(*vc.lattice(profile, start)), b.node_index(*vc.lattice(profile, goal)))]
+    pairs += _pairs(b, np.random.default_rng(13), 7)
+    view = v1.View(b)
+    for generator in (v1.GENERATOR_SWEEP, v1.GENERATOR_FRONTIER):
+        queries = [v1.make_query(i, s, g, b, generator=generator) for i, (s, g) in enumerate(pairs)]
+        clean = view.execute(queries, generator=generator)
+        req = v1.request_for(b, generator=generator)
+        for fault in FAULTS:
+            if CONTRACT[fault][0] != generator:
+                continue
+            lib = view.execute(queries, generator=generator, fault=v1.FAULT_IDS[fault])
+            _same_readback(lib, inject(fault, clean))
+            ok, _, inv, detail, _, _ = _validate(b, req, queries, lib)
+            assert not ok and inv == CONTRACT[fault][1], (fault, inv, detail)
     view.close()

@@ -1,13 +1,35 @@
-"""Readback corruptions for the host trust boundary, one per invariant class of
-the reference's fault decorator (src/gpu/fault_injecting_backend.cc:124-224),
-applied to the first query of a compact-path readback.  The workspace faults
-(kWorkspaceBounds, kWorkspaceOwner) act on the full-record readback and have
-no compact-form counterpart."""
+"""Readback corruptions for the host trust boundary, one per fault of the
+reference's candidate-batch decorator (src/gpu/fault_injecting_backend.cc:
+113-224), applied to the first query of a readback.  The generator each fault
+is replayed with follows the replay contract
+(schemas/gpu_batch_replay/v1.md): the compact_path_* faults corrupt the
+sweep's compact readback, every other fault the frontier's full workspace."""
 import ctypes as C
+
+import numpy as np
 
 from apgar_amd import v1
 
 INVALID = 0xFFFFFFFF
+INF = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+# fault -> (generator, expected invariant)
+CONTRACT = {
+    "workspace_bounds": (v1.GENERATOR_FRONTIER, "gpu.batch.workspace.bounds.v1"),
+    "workspace_owner": (v1.GENERATOR_FRONTIER, "gpu.batch.workspace.owner.v1"),
+    "query_telemetry": (v1.GENERATOR_FRONTIER, "gpu.batch.query.telemetry.v1"),
+    "memory_accounting": (v1.GENERATOR_FRONTIER, "gpu.batch.memory.accounting.v1"),
+    "batch_telemetry": (v1.GENERATOR_FRONTIER, "gpu.batch.telemetry.v1"),
+    "query_identity": (v1.GENERATOR_FRONTIER, "gpu.batch.query.identity.v1"),
+    "false_disconnected": (v1.GENERATOR_FRONTIER, "gpu.disconnected.cpu_reachable.v1"),
+    "compact_path_bounds": (v1.GENERATOR_SWEEP, "gpu.batch.compact_path.bounds.v1"),
+    "compact_path_endpoint": (v1.GENERATOR_SWEEP, "gpu.batch.compact_path.endpoint.v1"),
+    "compact_path_cycle": (v1.GENERATOR_SWEEP, "gpu.batch.compact_path.cycle.v1"),
+    "compact_path_heading": (v1.GENERATOR_SWEEP, "gpu.batch.compact_path.heading.v1"),
+    "compact_path_edge": (v1.GENERATOR_SWEEP, "gpu.batch.compact_path.edge.v1"),
+}
+FAULTS = list(CONTRACT)
+BATCH_LEVEL = {"gpu.batch.telemetry.v1", "gpu.batch.memory.accounting.v1", "gpu.batch.query.identity.v1"}
 
 
 def _copy(struct):
@@ -16,81 +38,68 @@ def _copy(struct):
     return out
 
 
-def _path_slice(paths, states):
-    p = paths[0]
-    return p.state_offset, p.state_count
-
-
-def inject(fault, results, paths, states, telemetry):
-    """Returns corrupted copies (results, paths, states, telemetry) and the
-    invariant id the host validator must report."""
-    results = [_copy(r) for r in results]
-    paths = [_copy(p) for p in paths]
-    states = list(states)
-    telemetry = _copy(telemetry)
+def inject(fault, rb: v1.Readback) -> v1.Readback:
+    """A corrupted copy of the readback; the original is left intact."""
+    results = [_copy(r) for r in rb.results]
+    paths = [_copy(p) for p in rb.paths]
+    states = list(rb.states)
+    tel = _copy(rb.telemetry)
+    ws = rb.workspace.copy() if rb.workspace is not None else None
     r = results[0]
-    off, n = _path_slice(paths, states)
-    if fault == "query_telemetry":
-        r.rounds = telemetry.dispatched_rounds + 1
-        return results, paths, states, telemetry, "gpu.batch.query.telemetry.v1"
-    if fault == "memory_accounting":
-        telemetry.batch_host_bytes += 1
-        return results, paths, states, telemetry, "gpu.batch.memory.accounting.v1"
-    if fault == "batch_telemetry":
-        telemetry.kernel_launch_count += 1
-        return results, paths, states, telemetry, "gpu.batch.telemetry.v1"
-    if fault == "query_identity":
-        r.query_id += 1
-        return results, paths, states, telemetry, "gpu.batch.query.identity.v1"
-    if fault == "false_disconnected":
+    env = ws.envelopes[0] if ws is not None else None
+    if fault == "workspace_bounds":
+        env.workspace_state_count = 1 if env.workspace_state_count == 0 else env.workspace_state_count - 1
+    elif fault == "workspace_owner":
+        if env.workspace_state_count:
+            ws.state_owners[env.workspace_offset] = np.uint64((r.workspace_owner + 1) & (2**64 - 1))
+    elif fault == "query_telemetry":
+        r.rounds = r.rounds + 1 if r.rounds != 0xFFFFFFFF else r.rounds - 1
+    elif fault == "memory_accounting":
+        tel.batch_host_bytes += 1
+    elif fault == "batch_telemetry":
+        tel.kernel_launch_count += 1
+    elif fault == "query_identity":
+        r.query_id = (r.query_id + 1) & (2**64 - 1)
+    elif fault == "false_disconnected":
         r.completion = v1.DISCONNECTED
         r.goal_state = INVALID
-        # drop the path's states and fix the byte accounting, as a real
-        # producer of the claim would
-        del states[off:off + n]
-        for p in paths[1:]:
-            if p.state_offset > off:
-                p.state_offset -= n
-        paths[0].state_count = 0
-        paths[0].state_offset = 0
-        telemetry.batch_host_bytes -= 4 * n
-        telemetry.device_to_host_readback_bytes -= 4 * n
-        return results, paths, states, telemetry, "gpu.disconnected.cpu_reachable.v1"
-    if fault == "compact_path_bounds":
-        paths[0].schema_version += 1
-        return results, paths, states, telemetry, "gpu.batch.compact_path.bounds.v1"
-    if fault == "compact_path_endpoint":
-        states[off] = INVALID
-        return results, paths, states, telemetry, "gpu.batch.compact_path.endpoint.v1"
-    if fault == "compact_path_cycle":
-        assert n >= 4
-        states[off + 1] = states[off + 2]
-        return results, paths, states, telemetry, "gpu.batch.compact_path.cycle.v1"
-    if fault == "compact_path_heading":
+        lo, n = env.workspace_offset, env.workspace_state_count
+        ws.labels[lo:lo + n] = INF
+        ws.predecessors[lo:lo + n] = INVALID
         start = r.start_node * 9 + 8
+        if start < n:
+            ws.labels[lo + start] = 0
+    else:
+        p = paths[0]
+        off, n = p.state_offset, p.state_count
         cur = states[off:off + n]
-        for i in range(1, n - 1):
-            node = cur[i] // 9
-            rep = node * 9 + 8
-            if node != r.start_node and rep not in cur and rep != start:
-                states[off + i] = rep
-                break
-        else:
-            raise AssertionError("no interior state to corrupt")
-        return results, paths, states, telemetry, "gpu.batch.compact_path.heading.v1"
-    if fault == "compact_path_edge":
-        cur = states[off:off + n]
-        for i in range(1, n - 1):
-            node, h = divmod(cur[i], 9)
-            for nh in range(8):
-                rep = node * 9 + nh
-                if nh != h and rep not in cur:
+        if fault == "compact_path_bounds":
+            p.schema_version += 1
+        elif fault == "compact_path_endpoint":
+            if n:
+                states[off] = INVALID
+        elif fault == "compact_path_cycle":
+            if n >= 4:
+                states[off + 1] = states[off + 2]
+        elif fault == "compact_path_heading":
+            for i in range(1, n - 1):
+                node = cur[i] // 9
+                rep = node * 9 + 8
+                if node != r.start_node and rep not in cur:
                     states[off + i] = rep
-                    return results, paths, states, telemetry, "gpu.batch.compact_path.edge.v1"
-        raise AssertionError("no interior state to corrupt")
-    raise KeyError(fault)
-
-
-FAULTS = ["query_telemetry", "memory_accounting", "batch_telemetry", "query_identity", "false_disconnected",
-          "compact_path_bounds", "compact_path_endpoint", "compact_path_cycle", "compact_path_heading",
-          "compact_path_edge"]
+                    break
+        elif fault == "compact_path_edge":
+            done = False
+            for i in range(1, n - 1):
+                node, h = divmod(cur[i], 9)
+                for nh in range(8):
+                    rep = node * 9 + nh
+                    if nh != h and rep not in cur:
+                        states[off + i] = rep
+                        done = True
+                        break
+                if done:
+                    break
+        else:
+            raise KeyError(fault)
+    return v1.Readback(results, paths, states, tel, ws)
