@@ -807,6 +807,104 @@ int apgx_v1_validate_batch(const apgx_v1_header* header, const apgx_v1_node* nod
                            apgx_v1_validation* out);
 
 /* ------------------------------------------------------------------------ */
+/* Candidate-batch invariant replays (gpu_batch_replay v1)                   */
+/* ------------------------------------------------------------------------ */
+
+/* A parsed replay artifact (schemas/gpu_batch_replay/v1.md; field meaning and
+ * order of tools/gpu_batch_replay.cc:129-226).  The policy is the v1 replay
+ * policy: base_scalar_cost with literal empty ban and penalty sets. */
+#define APGX_REPLAY_FAULT_UNAUTHENTICATED_PRODUCER 13u  /* beside APGX_V1_FAULT_* */
+#define APGX_REPLAY_EXPECT_INTERNAL_INVARIANT 1u
+#define APGX_REPLAY_EXPECT_UNSUPPORTED 2u
+typedef struct apgx_batch_replay {
+  char fixture[256];                       /* repository-relative fixture path */
+  uint64_t fixture_checksum;               /* FNV-1a of the fixture bytes */
+  uint32_t generator;                      /* APGX_V1_GENERATOR_* */
+  uint32_t fault;                          /* APGX_V1_FAULT_* or APGX_REPLAY_FAULT_* */
+  uint32_t layer;
+  uint32_t maximum_rounds;
+  uint64_t maximum_workspace_states_per_query;
+  uint64_t maximum_frontier_states_per_query;
+  uint64_t maximum_device_bytes;
+  uint64_t maximum_host_bytes;
+  uint64_t batch_id;
+  uint64_t query_id;
+  uint32_t input_ordinal;
+  uint32_t policy_schema_version;
+  uint64_t deterministic_seed;
+  uint32_t candidate_ordinal;
+  uint32_t expected_compiler_version;
+  uint64_t orthogonal_step_surcharge;
+  uint64_t diagonal_step_surcharge;
+  uint64_t bend_surcharge;
+  uint64_t expected_policy_identity;
+  uint64_t expected_board_content_hash;
+  uint64_t expected_compiler_profile_fingerprint;
+  uint64_t expected_rule_bucket_identity;
+  uint64_t expected_routing_profile_fingerprint;
+  uint64_t expected_device_view_fingerprint;
+  uint32_t expected_failure;               /* APGX_REPLAY_EXPECT_* */
+  uint32_t reserved;
+  char expected_invariant[64];
+} apgx_batch_replay;
+
+/* Parses one canonical artifact: UTF-8, no NUL / CR, LF line ends, the FNV-1a
+ * trailer, every field once in canonical order, canonical unsigned decimals
+ * within their widths, and the generator / fault / expected-outcome contract.
+ * Returns APGX_OK, or APGX_E_UNSUPPORTED with the reference's message
+ * ("replay payload checksum mismatch", "replay format, schema, generator,
+ * policy, fault, or expected outcome is unsupported", ...) in `error`. */
+int apgx_batch_replay_parse(const char* text, uint64_t length, apgx_batch_replay* out, char* error,
+                            uint64_t error_capacity);
+
+/* FingerprintCandidateGenerationPolicy (src/routing/candidate_policy.cc:139-160)
+ * of the artifact's policy. */
+uint64_t apgx_batch_replay_policy_identity(const apgx_batch_replay* artifact);
+
+/* Replay fixture: the output of the reference's board compiler for one corpus
+ * case -- compiled profile, request endpoints and represented nodes -- which
+ * apgx_v1_build_board turns into the DeviceCompiledBoardV1.  Layout: this
+ * header, then node_count apgx_v1_compiled_node records, little endian. */
+#define APGX_REPLAY_FIXTURE_MAGIC "APGX-V1-FIXTURE"
+typedef struct apgx_replay_fixture_header {
+  char magic[16];                          /* APGX_REPLAY_FIXTURE_MAGIC, NUL padded */
+  uint32_t schema_version;                 /* 1 */
+  uint32_t layer;                          /* request layer */
+  int64_t start_x, start_y;                /* request endpoints, lattice coordinates */
+  int64_t goal_x, goal_y;
+  uint64_t routing_profile_fingerprint;    /* FingerprintRoutingProfile of the routed net */
+  apgx_v1_compiled_profile profile;
+  uint64_t node_count;
+} apgx_replay_fixture_header;
+
+#define APGX_REPLAY_STAGE_PARSE 1u
+#define APGX_REPLAY_STAGE_FIXTURE 2u
+#define APGX_REPLAY_STAGE_ASSOCIATION 3u
+#define APGX_REPLAY_STAGE_EXECUTE 4u
+#define APGX_REPLAY_STAGE_REPRODUCE 5u
+#define APGX_REPLAY_DRY_RUN 1u                 /* stop after the association gate (no GPU work) */
+typedef struct apgx_replay_outcome {
+  int32_t exit_code;     /* the tool's: 0 reproduced (or dry run passed), 1 not reproduced, 2 artifact,
+                            fixture or association error */
+  uint32_t stage;        /* APGX_REPLAY_STAGE_* reached */
+  char invariant[64];    /* invariant the trust boundary reported */
+  char message[256];
+} apgx_replay_outcome;
+
+/* Runs one artifact against its fixture bytes (Replay, tools/gpu_batch_replay.cc:
+ * 228-391): parse, fixture checksum, device-view compilation, the association
+ * gate (board hash, compiler profile and version, rule bucket, routing profile,
+ * device view, policy identity) before any GPU work, then one query executed
+ * with the forced generator and budgets, the readback corrupted by
+ * apgx_v1_pending_inject_fault, validated by apgx_v1_validate_batch, and the
+ * reported failure class and invariant compared with the expected ones.  The
+ * unauthenticated-producer contract needs candidate construction, which this
+ * library does not contain: it stops at the gate with exit code 2.  Returns
+ * APGX_OK unless an argument is null; the verdict is in `out`. */
+int apgx_batch_replay_run(const char* text, uint64_t text_length, const uint8_t* fixture, uint64_t fixture_length,
+                          uint32_t flags, apgx_replay_outcome* out);
+
+/* ------------------------------------------------------------------------ */
 /* Synthetic lattice generators (deterministic xoshiro256** streams)         */
 /* ------------------------------------------------------------------------ */
 

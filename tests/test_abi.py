@@ -28,9 +28,10 @@ def test_library_exports_every_declared_symbol(apgx):
 
 
 def test_python_binding_covers_the_header(apgx):
-    from apgar_amd import v1
+    from apgar_amd import replay, v1
 
-    assert declared_functions() <= set(apgx.exported_symbols()) | set(v1.exported_symbols())
+    bound = set(apgx.exported_symbols()) | set(v1.exported_symbols()) | set(replay.exported_symbols())
+    assert declared_functions() <= bound
 
 
 def test_last_error_reports_invalid_arguments(apgx):

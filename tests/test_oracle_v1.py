@@ -82,7 +82,7 @@ def test_surcharges_and_merged_penalty_both_directions(oracle):
 
 def test_stable_hash_string_matches_replay_checksum():
     # A published replay trailer: FNV-1a of every byte before the checksum line.
-    text = (Path(__file__).parent / "golden" / "replay_batch_workspace_owner_v1.txt").read_bytes()
+    text = (Path(__file__).parent / "golden" / "reference_replays" / "batch_workspace_owner_v1.replay").read_bytes()
     payload, _, last = text.rstrip(b"\n").rpartition(b"\n")
     assert last.startswith(b"checksum_fnv1a64=")
     assert vc.stable_hash_string(payload + b"\n") == int(last.split(b"=")[1])
