@@ -167,9 +167,10 @@ typedef struct apgx_route_options {
   uint64_t max_device_bytes;  /* 0: unlimited (workspace budget) */
   uint32_t use_graph;         /* 1: launch round chunks as a captured hipGraph */
   uint32_t profile_rounds;    /* 1: time every round kernel with hipEvents (round_kernel_ms) */
-  uint32_t frontier_delta;    /* distance bucket width of tile scheduling; 0: 64 x mean cell
+  uint32_t frontier_delta;    /* distance bucket width of tile scheduling; 0: 48 x mean cell
                                  cost; 0xFFFFFFFF: unordered (every queued tile every round) */
-  uint32_t max_sweeps;        /* LDS/register sweeps per tile visit before re-queueing; 0: 64 */
+  uint32_t max_sweeps;        /* sweeps per tile visit before re-queueing; 0: 10 on register
+                                 tiles, 64 on LDS tiles */
   uint32_t no_goal_bound;     /* 1: order tiles by distance only; 0: distance plus the
                                  consistent lower bound to the nearest unconnected pin */
   uint32_t sweep_schedule;    /* register tiles: k = one full sweep (via exchange and
@@ -218,8 +219,8 @@ void apgx_search_release(apgx_search* search);
  * (frontier_delta, max_sweeps) on an evenly strided sample of the nets and
  * writes the fastest candidate into *tuned; every other field is copied from
  * *base (NULL: defaults).  The set: tiles 64x64 (register-resident, when the
- * lattice allows), 64x32 and 32x32 (LDS); delta 16, 64 and 256 x mean cell
- * cost and unordered; max_sweeps 64 and 16 on 64x64 tiles.  Results do not
+ * lattice allows), 64x32 and 32x32 (LDS); delta 16, 48 and 256 x mean cell
+ * cost and unordered; max_sweeps 64, 16 and 10 on 64x64 tiles.  Results do not
  * depend on the choice (labels are the unique fixed point); only speed does.
  * No reference counterpart: the reference has one fixed launch shape. */
 #define APGX_TUNE_MAX_CANDIDATES 24u
