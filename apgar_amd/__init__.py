@@ -156,6 +156,18 @@ class PriorDecode(C.Structure):
     _fields_ = [("tile", C.c_uint32), ("max_value", C.c_uint32), ("unit", C.c_float), ("gain", C.c_float)]
 
 
+class V2BoardHeader(C.Structure):
+    """DeviceDenseBoardV2 header (128 bytes), schemas/device_dense_board/v2.md."""
+    _fields_ = [("schema_version", C.c_uint32), ("compiler_version", C.c_uint32),
+                ("source_board_content_hash", C.c_uint64), ("compiler_profile_fingerprint", C.c_uint64),
+                ("rule_bucket_identity", C.c_uint64), ("device_view_fingerprint", C.c_uint64),
+                ("estimated_persistent_device_bytes", C.c_uint64), ("lattice_origin_x", C.c_int64),
+                ("lattice_origin_y", C.c_int64), ("lattice_step", C.c_int64), ("width", C.c_uint32),
+                ("height", C.c_uint32), ("layers", C.c_uint32), ("cost_bytes", C.c_uint32),
+                ("via_cost", C.c_uint32), ("min_cell_cost", C.c_uint32), ("max_cell_cost", C.c_uint32),
+                ("reserved0", C.c_uint32), ("free_cells", C.c_uint64), ("reserved", C.c_uint8 * 16)]
+
+
 _P = C.POINTER
 _SIGNATURES = {
     "apgx_device_info_query": (C.c_int, [_P(DeviceInfo)]),
@@ -164,6 +176,10 @@ _SIGNATURES = {
     "apgx_device_synchronize": (C.c_int, []),
     "apgx_lattice_create": (C.c_int, [_P(LatticeDesc), C.c_void_p, _P(C.c_void_p)]),
     "apgx_lattice_create_from_device": (C.c_int, [_P(LatticeDesc), C.c_void_p, _P(C.c_void_p)]),
+    "apgx_v2_board_fingerprint": (C.c_uint64, [_P(V2BoardHeader), C.c_void_p]),
+    "apgx_v2_board_seal": (C.c_int, [_P(V2BoardHeader), C.c_void_p, C.c_uint64]),
+    "apgx_v2_board_validate": (C.c_int, [_P(V2BoardHeader), C.c_void_p, C.c_uint64]),
+    "apgx_lattice_create_v2": (C.c_int, [_P(V2BoardHeader), C.c_void_p, C.c_uint64, _P(C.c_void_p)]),
     "apgx_lattice_release": (None, [C.c_void_p]),
     "apgx_lattice_device_costs": (C.c_void_p, [C.c_void_p]),
     "apgx_route_nets": (C.c_int, [C.c_void_p, _P(NetQuery), C.c_uint32, _P(C.c_uint32), C.c_uint64,
@@ -299,6 +315,50 @@ def generate_board(**params):
     return cost.reshape(p.layers, p.height, p.width), nets, pins[:total.value].copy()
 
 
+# ------------------------------------------------------------ sealed V2 boards
+def seal_board(cost: np.ndarray, via_cost: int = 0, *, cost_bytes: int | None = None, compiler_version: int = 1,
+               source_board_content_hash: int = 0, compiler_profile_fingerprint: int = 0,
+               rule_bucket_identity: int = 0, origin=(0, 0), step: int = 1):
+    """Seals a (layers, height, width) cost plane as DeviceDenseBoardV2:
+    returns (header, plane) with the derived fields and fingerprint filled."""
+    L, H, W = cost.shape
+    if cost_bytes is None:
+        cost_bytes = 2 if int(cost.max(initial=0)) <= 0xFFFF else 4
+    plane = np.ascontiguousarray(cost, dtype=np.uint16 if cost_bytes == 2 else np.uint32)
+    h = V2BoardHeader()
+    h.schema_version, h.compiler_version = LATTICE_SCHEMA_VERSION, compiler_version
+    h.source_board_content_hash = source_board_content_hash
+    h.compiler_profile_fingerprint = compiler_profile_fingerprint
+    h.rule_bucket_identity = rule_bucket_identity
+    h.lattice_origin_x, h.lattice_origin_y = origin
+    h.lattice_step = step
+    h.width, h.height, h.layers, h.cost_bytes, h.via_cost = W, H, L, cost_bytes, via_cost
+    check(lib().apgx_v2_board_seal(C.byref(h), plane.ctypes.data, plane.nbytes))
+    return h, plane
+
+
+def validate_board(header: V2BoardHeader, plane: np.ndarray) -> None:
+    """Raises ApgxError unless the sealed board is intact."""
+    plane = np.ascontiguousarray(plane)
+    check(lib().apgx_v2_board_validate(C.byref(header), plane.ctypes.data, plane.nbytes))
+
+
+def board_bytes(header: V2BoardHeader, plane: np.ndarray) -> bytes:
+    """Wire form: the 128-byte header followed by the plane."""
+    return bytes(header) + np.ascontiguousarray(plane).tobytes()
+
+
+def parse_board_bytes(data: bytes):
+    """(header, plane) of a wire-form board; the plane's dtype and shape come
+    from the header (nothing is validated here)."""
+    h = V2BoardHeader.from_buffer_copy(data[:C.sizeof(V2BoardHeader)])
+    dt = np.uint16 if h.cost_bytes == 2 else np.uint32
+    plane = np.frombuffer(data, dtype=dt, offset=C.sizeof(V2BoardHeader))
+    if plane.size == h.layers * h.height * h.width:
+        plane = plane.reshape(h.layers, h.height, h.width)
+    return h, plane
+
+
 # --------------------------------------------------------------------- search
 @dataclass
 class NetRoute:
@@ -324,6 +384,22 @@ class Lattice:
         handle = C.c_void_p()
         check(lib().apgx_lattice_create(C.byref(self.desc), plane.ctypes.data, C.byref(handle)))
         self.handle = handle
+
+    @classmethod
+    def from_sealed(cls, header: V2BoardHeader, plane: np.ndarray) -> "Lattice":
+        """Uploads a sealed DeviceDenseBoardV2 (validated first)."""
+        self = cls.__new__(cls)
+        plane = np.ascontiguousarray(plane)
+        self.shape = (header.layers, header.height, header.width)
+        self.via_cost = header.via_cost
+        self.cost_bytes = header.cost_bytes
+        self.desc = LatticeDesc(LATTICE_SCHEMA_VERSION, header.width, header.height, header.layers,
+                                header.cost_bytes, header.via_cost, (C.c_uint32 * 2)())
+        handle = C.c_void_p()
+        self.handle = None
+        check(lib().apgx_lattice_create_v2(C.byref(header), plane.ctypes.data, plane.nbytes, C.byref(handle)))
+        self.handle = handle
+        return self
 
     def close(self):
         if self.handle:

@@ -111,6 +111,44 @@ typedef struct apgx_lattice_desc {
 
 typedef struct apgx_lattice apgx_lattice;
 
+/* DeviceDenseBoardV2 (schemas/device_dense_board/v2.md): the sealed wire form
+ * of a dense board, a 128-byte header followed by the cost plane
+ * (cost_bytes per cell, cell order above).  The producer fills the
+ * association and shape fields and calls apgx_v2_board_seal, which derives the
+ * cost summary, the persistent bytes and the FNV-1a device view fingerprint;
+ * the consumer's apgx_v2_board_validate / apgx_lattice_create_v2 re-derive all
+ * of them and reject any difference, as DeviceCompiledBoardV1 uploads are
+ * re-checked (gpu/cuda/planar_router.cu:1256-1451).  V1 records are unchanged. */
+typedef struct apgx_v2_board_header {
+  uint32_t schema_version;                 /* APGX_LATTICE_SCHEMA_VERSION */
+  uint32_t compiler_version;
+  uint64_t source_board_content_hash;
+  uint64_t compiler_profile_fingerprint;
+  uint64_t rule_bucket_identity;
+  uint64_t device_view_fingerprint;        /* derived */
+  uint64_t estimated_persistent_device_bytes;  /* derived: 128 + plane bytes */
+  int64_t lattice_origin_x;                /* exact coordinate of cell (0, 0) */
+  int64_t lattice_origin_y;
+  int64_t lattice_step;                    /* > 0 */
+  uint32_t width;
+  uint32_t height;
+  uint32_t layers;
+  uint32_t cost_bytes;                     /* 2 or 4 */
+  uint32_t via_cost;
+  uint32_t min_cell_cost;                  /* derived: smallest free cost, 0 without free cells */
+  uint32_t max_cell_cost;                  /* derived */
+  uint32_t reserved0;                      /* 0 */
+  uint64_t free_cells;                     /* derived */
+  uint8_t reserved[16];                    /* 0 */
+} apgx_v2_board_header;
+
+uint64_t apgx_v2_board_fingerprint(const apgx_v2_board_header* header, const void* plane);
+int apgx_v2_board_seal(apgx_v2_board_header* header, const void* plane, uint64_t plane_bytes);
+int apgx_v2_board_validate(const apgx_v2_board_header* header, const void* plane, uint64_t plane_bytes);
+/* Validates a sealed board, then uploads it as apgx_lattice_create does. */
+int apgx_lattice_create_v2(const apgx_v2_board_header* header, const void* plane, uint64_t plane_bytes,
+                           apgx_lattice** out);
+
 /* Validates and uploads one immutable cost plane (host memory, cost_bytes per
  * cell).  The handle owns the device copy until apgx_lattice_release. */
 int apgx_lattice_create(const apgx_lattice_desc* desc, const void* host_cost_plane,
