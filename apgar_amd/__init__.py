@@ -156,6 +156,28 @@ class PriorDecode(C.Structure):
     _fields_ = [("tile", C.c_uint32), ("max_value", C.c_uint32), ("unit", C.c_float), ("gain", C.c_float)]
 
 
+class PriceAssociations(C.Structure):
+    _fields_ = [("board_content_hash", C.c_uint64), ("compiler_profile_fingerprint", C.c_uint64),
+                ("geometry_compiler_version", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("layers", C.c_uint32)]
+
+
+class PriceSnapshot(C.Structure):
+    _fields_ = [("associations", PriceAssociations), ("capacity_units", C.c_uint32), ("reserved", C.c_uint32),
+                ("policy_identity", C.c_uint64), ("prior_snapshot_identity", C.c_uint64),
+                ("epoch_index", C.c_uint64), ("present_factor", C.c_uint64), ("price_count", C.c_uint64),
+                ("snapshot_identity", C.c_uint64)]
+
+
+class PriceCheck(C.Structure):
+    _fields_ = [("valid", C.c_uint32), ("reserved", C.c_uint32), ("entry_index", C.c_uint64),
+                ("invariant", C.c_char * 64)]
+
+
+# apgx_price_entry as a numpy record
+PRICE_ENTRY = np.dtype([("cell", "<u8"), ("present", "<u8"), ("history", "<u8"), ("total", "<u8")])
+
+
 class V2BoardHeader(C.Structure):
     """DeviceDenseBoardV2 header (128 bytes), schemas/device_dense_board/v2.md."""
     _fields_ = [("schema_version", C.c_uint32), ("compiler_version", C.c_uint32),
@@ -177,6 +199,16 @@ _SIGNATURES = {
     "apgx_lattice_create": (C.c_int, [_P(LatticeDesc), C.c_void_p, _P(C.c_void_p)]),
     "apgx_lattice_create_from_device": (C.c_int, [_P(LatticeDesc), C.c_void_p, _P(C.c_void_p)]),
     "apgx_v2_board_fingerprint": (C.c_uint64, [_P(V2BoardHeader), C.c_void_p]),
+    "apgx_device_class": (C.c_int, [C.c_char_p, C.c_uint64]),
+    "apgx_price_policy_identity": (C.c_uint64, [_P(PricePolicy)]),
+    "apgx_price_snapshot_identity": (C.c_uint64, [_P(PriceSnapshot), C.c_void_p]),
+    "apgx_price_snapshot_capture": (C.c_int, [_P(PriceAssociations), C.c_uint32, _P(PricePolicy), C.c_uint64,
+                                              C.c_uint64, _P(C.c_uint32), _P(C.c_uint32), C.c_void_p, C.c_uint64,
+                                              _P(PriceSnapshot)]),
+    "apgx_price_snapshot_validate": (C.c_int, [_P(PriceSnapshot), C.c_void_p, _P(PriceAssociations), C.c_uint32,
+                                               _P(PricePolicy), _P(PriceLimits), C.c_uint64, C.c_uint64,
+                                               _P(PriceCheck)]),
+    "apgx_price_snapshot_restore": (C.c_int, [_P(PriceSnapshot), C.c_void_p, _P(C.c_uint32), _P(C.c_uint32)]),
     "apgx_v2_board_seal": (C.c_int, [_P(V2BoardHeader), C.c_void_p, C.c_uint64]),
     "apgx_v2_board_validate": (C.c_int, [_P(V2BoardHeader), C.c_void_p, C.c_uint64]),
     "apgx_lattice_create_v2": (C.c_int, [_P(V2BoardHeader), C.c_void_p, C.c_uint64, _P(C.c_void_p)]),
@@ -261,6 +293,13 @@ def check(code: int) -> None:
 def u32p(a: np.ndarray):
     assert a.dtype == np.uint32 and a.flags.c_contiguous
     return a.ctypes.data_as(_P(C.c_uint32))
+
+
+def device_class() -> str:
+    """Candidate device class of the current device, e.g. "hip-gfx950"."""
+    buf = C.create_string_buffer(64)
+    check(lib().apgx_device_class(buf, len(buf)))
+    return buf.value.decode()
 
 
 def synchronize() -> None:
@@ -676,3 +715,62 @@ def price_update(epoch: int, usage: np.ndarray, history: np.ndarray, capacity=No
                                   u32p(h), u32p(present), u32p(total), C.byref(out)))
     summary = {k: getattr(out, k) for k, _ in PriceEpoch._fields_ if k != "reserved"}
     return summary, present, h, total
+
+
+def _price_policy(**settings):
+    pol, lim = PricePolicy(), PriceLimits()
+    lib().apgx_price_defaults(C.byref(pol), C.byref(lim))
+    for k, v in settings.items():
+        setattr(pol if hasattr(pol, k) else lim, k, v)
+    return pol, lim
+
+
+def price_policy_identity(**policy) -> int:
+    return lib().apgx_price_policy_identity(C.byref(_price_policy(**policy)[0]))
+
+
+def price_snapshot_capture(associations: PriceAssociations, capacity_units: int, epoch: int, prior_identity: int,
+                           present: np.ndarray, history: np.ndarray, **policy):
+    """Captures a price snapshot of host u32 planes; returns (snapshot,
+    entries as a PRICE_ENTRY array)."""
+    pol, _ = _price_policy(**policy)
+    p = np.ascontiguousarray(present, dtype=np.uint32).reshape(-1)
+    h = np.ascontiguousarray(history, dtype=np.uint32).reshape(-1)
+    snap = PriceSnapshot()
+    check(lib().apgx_price_snapshot_capture(C.byref(associations), capacity_units, C.byref(pol), epoch,
+                                            prior_identity, u32p(p), u32p(h), None, 0, C.byref(snap)))
+    entries = np.zeros(max(snap.price_count, 1), PRICE_ENTRY)
+    check(lib().apgx_price_snapshot_capture(C.byref(associations), capacity_units, C.byref(pol), epoch,
+                                            prior_identity, u32p(p), u32p(h), entries.ctypes.data, entries.size,
+                                            C.byref(snap)))
+    return snap, entries[: snap.price_count]
+
+
+def price_snapshot_validate(snapshot: PriceSnapshot, entries: np.ndarray, expected: PriceAssociations,
+                            capacity_units: int, *, maximum_epoch_index=1024, maximum_overuse_per_epoch=1 << 32,
+                            **settings):
+    """ValidatePriorSnapshot on a snapshot; returns (valid, invariant id, entry index)."""
+    pol, lim = _price_policy(**settings)
+    e = np.ascontiguousarray(entries, dtype=PRICE_ENTRY)
+    out = PriceCheck()
+    check(lib().apgx_price_snapshot_validate(C.byref(snapshot), e.ctypes.data if e.size else None, C.byref(expected),
+                                             capacity_units, C.byref(pol), C.byref(lim), maximum_epoch_index,
+                                             maximum_overuse_per_epoch, C.byref(out)))
+    return bool(out.valid), out.invariant.decode(), out.entry_index
+
+
+def price_snapshot_identity(snapshot: PriceSnapshot, entries: np.ndarray) -> int:
+    e = np.ascontiguousarray(entries, dtype=PRICE_ENTRY)
+    return lib().apgx_price_snapshot_identity(C.byref(snapshot), e.ctypes.data if e.size else None)
+
+
+def price_snapshot_restore(snapshot: PriceSnapshot, entries: np.ndarray):
+    """Dense (present, history) u32 planes of a snapshot."""
+    a = snapshot.associations
+    cells = a.width * a.height * a.layers
+    present = np.zeros(cells, np.uint32)
+    history = np.zeros(cells, np.uint32)
+    e = np.ascontiguousarray(entries, dtype=PRICE_ENTRY)
+    check(lib().apgx_price_snapshot_restore(C.byref(snapshot), e.ctypes.data if e.size else None, u32p(present),
+                                            u32p(history)))
+    return present, history

@@ -59,3 +59,26 @@ extern "C" int apgx_device_info_query(apgx_device_info* out) {
   apgx::ClearError();
   return APGX_OK;
 }
+
+// CudaCandidateDeviceClass (src/candidates/gpu_candidate_adapter.cc:27-35) for
+// this backend: "hip-" + the device's gfx architecture name (the part before
+// any ':' feature suffix), e.g. "hip-gfx950".  Incomplete metadata has no
+// class, as there.  The marketing name is not required: some ROCm installs
+// leave it empty, and the architecture is what identifies the class.
+extern "C" int apgx_device_class(char* out, uint64_t capacity) {
+  using namespace apgx;
+  if (out == nullptr || capacity == 0) return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_device_class: null output");
+  out[0] = '\0';
+  apgx_device_info info;
+  if (int s = apgx_device_info_query(&info)) return s;
+  std::string arch(info.gcn_arch_name);
+  arch = arch.substr(0, arch.find(':'));
+  if (std::string(info.backend) != "hip" || arch.empty() || info.runtime_version == 0 || info.driver_version == 0 ||
+      info.global_memory_bytes == 0)
+    return Fail(APGX_E_UNSUPPORTED, "apgx_device_class: device metadata is incomplete");
+  const std::string cls = "hip-" + arch;
+  if (cls.size() + 1 > capacity) return Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_device_class: buffer too small");
+  std::memcpy(out, cls.c_str(), cls.size() + 1);
+  ClearError();
+  return APGX_OK;
+}

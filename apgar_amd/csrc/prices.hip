@@ -30,6 +30,7 @@
 
 #include "apgx.h"
 #include "common.h"
+#include "v1_common.h"
 
 namespace apgx {
 namespace {
@@ -342,6 +343,349 @@ extern "C" int apgx_price_update(uint64_t epoch_index, const apgx_price_policy* 
   if (present) APGX_HIP(hipMemcpy(present, b.present, 4 * cells, hipMemcpyDeviceToHost), "present readback");
   if (total) APGX_HIP(hipMemcpy(total, b.total, 4 * cells, hipMemcpyDeviceToHost), "total readback");
   out->outcome = APGX_PRICE_OK;
+  ClearError();
+  return APGX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Price snapshots (NegotiatedPriceSnapshot, include/apgar/allocator/
+// negotiated_regeneration_plan.h:49-60) on cell resources.  Capture compacts
+// the nonzero price cells of the present and history planes, in cell order, on
+// the device: a counting pass per 4096-cell chunk, an exclusive scan of the
+// chunk counts, and a scatter pass whose wave ballots and prefix counts give
+// every entry its slot.  The identity is the reference's SnapshotIdentity
+// field order (negotiated_regeneration_plan.cc:108-127) with a cell resource
+// (layer, x, y) in place of an edge resource.  Validation restates
+// ValidatePriorSnapshot (:211-396) check for check; restore scatters a
+// validated snapshot back into dense planes so the next epoch can resume.
+// ---------------------------------------------------------------------------
+namespace apgx {
+namespace {
+
+constexpr int kSnapPerThread = 16;
+constexpr int kSnapChunk = kThreads * kSnapPerThread;  // cells per block
+
+__global__ __launch_bounds__(kThreads) void SnapshotCount(const uint32_t* __restrict__ present,
+                                                          const uint32_t* __restrict__ history, uint64_t cells,
+                                                          uint32_t* __restrict__ counts) {
+  __shared__ uint32_t s_wave[kThreads / 64];
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kSnapChunk;
+  uint32_t n = 0;
+#pragma unroll
+  for (int k = 0; k < kSnapPerThread; ++k) {
+    const uint64_t c = base + static_cast<uint64_t>(k) * kThreads + threadIdx.x;
+    n += (c < cells && (present[c] | history[c]) != 0) ? 1u : 0u;
+  }
+  for (int off = 32; off > 0; off >>= 1) n += __shfl_down(n, off, 64);
+  if ((threadIdx.x & 63) == 0) s_wave[threadIdx.x >> 6] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < kThreads / 64; ++w) t += s_wave[w];
+    counts[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void SnapshotScatter(const uint32_t* __restrict__ present,
+                                                            const uint32_t* __restrict__ history, uint64_t cells,
+                                                            const uint64_t* __restrict__ offsets,
+                                                            apgx_price_entry* __restrict__ out) {
+  __shared__ uint32_t s_wave[kThreads / 64];
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kSnapChunk;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint64_t slot = offsets[blockIdx.x];
+  for (int k = 0; k < kSnapPerThread; ++k) {
+    const uint64_t c = base + static_cast<uint64_t>(k) * kThreads + threadIdx.x;
+    uint32_t p = 0, h = 0;
+    if (c < cells) {
+      p = present[c];
+      h = history[c];
+    }
+    const bool keep = (p | h) != 0;
+    const uint64_t mask = __ballot(keep);
+    const uint32_t below = static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)));
+    if (lane == 0) s_wave[wave] = static_cast<uint32_t>(__popcll(mask));
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+    for (int w = 0; w < kThreads / 64; ++w) {
+      before += w < wave ? s_wave[w] : 0u;
+      total += s_wave[w];
+    }
+    if (keep) {
+      apgx_price_entry e;
+      e.cell = c;
+      e.present_price = p;
+      e.historical_price = h;
+      e.total_price = static_cast<uint64_t>(p) + h;
+      out[slot + before + below] = e;
+    }
+    slot += total;
+    __syncthreads();  // s_wave is rewritten by the next sub-chunk
+  }
+}
+
+__global__ void Fill32(uint32_t* __restrict__ p, uint64_t n) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+__global__ void SnapshotRestore(const apgx_price_entry* __restrict__ entries, uint64_t count,
+                                uint32_t* __restrict__ present, uint32_t* __restrict__ history) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const apgx_price_entry e = entries[i];
+  present[e.cell] = static_cast<uint32_t>(e.present_price);
+  history[e.cell] = static_cast<uint32_t>(e.historical_price);
+}
+
+uint64_t NonzeroFinish(const StableHash& h) {
+  const uint64_t v = h.Finish();
+  return v == 0 ? 1 : v;
+}
+
+}  // namespace
+}  // namespace apgx
+
+extern "C" uint64_t apgx_price_policy_identity(const apgx_price_policy* policy) {
+  // PricePolicyIdentity (src/allocator/negotiated_regeneration_plan.cc:98-106).
+  if (policy == nullptr) return 0;
+  apgx::StableHash h;
+  h.String("APGAR-P4R06-NEGOTIATED-PRICE-POLICY-V1");
+  h.U64(policy->initial_present_factor);
+  h.U64(policy->present_factor_increment);
+  h.U64(policy->historical_price_increment);
+  return apgx::NonzeroFinish(h);
+}
+
+extern "C" uint64_t apgx_price_snapshot_identity(const apgx_price_snapshot* s, const apgx_price_entry* entries) {
+  if (s == nullptr || (s->price_count && entries == nullptr)) return 0;
+  apgx::StableHash h;
+  h.String("APGX-CELL-PRICE-SNAPSHOT-V1");
+  h.U64(s->associations.board_content_hash);
+  h.U64(s->associations.compiler_profile_fingerprint);
+  h.U32(s->associations.geometry_compiler_version);
+  h.U32(s->associations.width);
+  h.U32(s->associations.height);
+  h.U32(s->associations.layers);
+  h.U32(s->capacity_units);
+  h.U64(s->policy_identity);
+  h.U64(s->prior_snapshot_identity);
+  h.U64(s->epoch_index);
+  h.U64(s->present_factor);
+  h.U64(s->price_count);
+  const uint64_t plane = static_cast<uint64_t>(s->associations.width) * s->associations.height;
+  for (uint64_t i = 0; i < s->price_count; ++i) {
+    const apgx_price_entry& e = entries[i];
+    const uint64_t w = s->associations.width ? s->associations.width : 1;
+    h.U32(plane ? static_cast<uint32_t>(e.cell / plane) : 0u);          // layer
+    h.I64(static_cast<int64_t>(e.cell % w));                            // x
+    h.I64(static_cast<int64_t>(plane ? (e.cell % plane) / w : 0u));     // y
+    h.U64(e.present_price);
+    h.U64(e.historical_price);
+    h.U64(e.total_price);
+  }
+  return apgx::NonzeroFinish(h);
+}
+
+extern "C" int apgx_price_snapshot_capture(const apgx_price_associations* associations, uint32_t capacity_units,
+                                           const apgx_price_policy* policy, uint64_t epoch_index,
+                                           uint64_t prior_snapshot_identity, const uint32_t* present,
+                                           const uint32_t* history, apgx_price_entry* entries,
+                                           uint64_t entry_capacity, apgx_price_snapshot* out) {
+  using namespace apgx;
+  if (associations == nullptr || policy == nullptr || out == nullptr || present == nullptr || history == nullptr)
+    return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_price_snapshot_capture: null argument");
+  const uint64_t cells = static_cast<uint64_t>(associations->width) * associations->height * associations->layers;
+  uint64_t factor = 0;
+  if (__builtin_mul_overflow(epoch_index, policy->present_factor_increment, &factor) ||
+      __builtin_add_overflow(factor, policy->initial_present_factor, &factor))
+    return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_price_snapshot_capture: present factor overflows");
+  std::memset(out, 0, sizeof(*out));
+  out->associations = *associations;
+  out->capacity_units = capacity_units;
+  out->policy_identity = apgx_price_policy_identity(policy);
+  out->prior_snapshot_identity = prior_snapshot_identity;
+  out->epoch_index = epoch_index;
+  out->present_factor = factor;
+  uint64_t total = 0;
+  if (cells) {
+    const uint64_t blocks = (cells + kSnapChunk - 1) / kSnapChunk;
+    uint32_t *d_present = nullptr, *d_history = nullptr, *d_counts = nullptr;
+    uint64_t* d_offsets = nullptr;
+    apgx_price_entry* d_entries = nullptr;
+    struct Guard {
+      std::vector<void*> p;
+      ~Guard() {
+        for (void* q : p)
+          if (q) (void)hipFree(q);
+      }
+    } guard;
+    APGX_HIP(hipMalloc(&d_present, 4 * cells), "hipMalloc");
+    guard.p.push_back(d_present);
+    APGX_HIP(hipMalloc(&d_history, 4 * cells), "hipMalloc");
+    guard.p.push_back(d_history);
+    APGX_HIP(hipMalloc(&d_counts, 4 * blocks), "hipMalloc");
+    guard.p.push_back(d_counts);
+    APGX_HIP(hipMalloc(&d_offsets, 8 * blocks), "hipMalloc");
+    guard.p.push_back(d_offsets);
+    APGX_HIP(hipMemcpy(d_present, present, 4 * cells, hipMemcpyHostToDevice), "present upload");
+    APGX_HIP(hipMemcpy(d_history, history, 4 * cells, hipMemcpyHostToDevice), "history upload");
+    hipLaunchKernelGGL(SnapshotCount, dim3(static_cast<unsigned>(blocks)), dim3(kThreads), 0, nullptr, d_present,
+                       d_history, cells, d_counts);
+    APGX_HIP(hipGetLastError(), "snapshot count launch");
+    std::vector<uint32_t> counts(blocks);
+    APGX_HIP(hipMemcpy(counts.data(), d_counts, 4 * blocks, hipMemcpyDeviceToHost), "count readback");
+    std::vector<uint64_t> offsets(blocks);
+    for (uint64_t b = 0; b < blocks; ++b) {
+      offsets[b] = total;
+      total += counts[b];
+    }
+    out->price_count = total;
+    if (entries != nullptr && total > entry_capacity)
+      return Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_price_snapshot_capture: entry capacity too small");
+    if (entries != nullptr && total) {
+      APGX_HIP(hipMalloc(&d_entries, sizeof(apgx_price_entry) * total), "hipMalloc");
+      guard.p.push_back(d_entries);
+      APGX_HIP(hipMemcpy(d_offsets, offsets.data(), 8 * blocks, hipMemcpyHostToDevice), "offset upload");
+      hipLaunchKernelGGL(SnapshotScatter, dim3(static_cast<unsigned>(blocks)), dim3(kThreads), 0, nullptr, d_present,
+                         d_history, cells, d_offsets, d_entries);
+      APGX_HIP(hipGetLastError(), "snapshot scatter launch");
+      APGX_HIP(hipMemcpy(entries, d_entries, sizeof(apgx_price_entry) * total, hipMemcpyDeviceToHost),
+               "entry readback");
+    }
+  }
+  if (entries == nullptr && total) {  // size query only
+    ClearError();
+    return APGX_OK;
+  }
+  out->snapshot_identity = apgx_price_snapshot_identity(out, entries);
+  ClearError();
+  return APGX_OK;
+}
+
+extern "C" int apgx_price_snapshot_validate(const apgx_price_snapshot* prior, const apgx_price_entry* entries,
+                                            const apgx_price_associations* expected, uint32_t capacity_units,
+                                            const apgx_price_policy* policy, const apgx_price_limits* limits,
+                                            uint64_t maximum_epoch_index, uint64_t maximum_overuse_per_epoch,
+                                            apgx_price_check* out) {
+  using namespace apgx;
+  if (prior == nullptr || expected == nullptr || policy == nullptr || limits == nullptr || out == nullptr ||
+      (prior->price_count && entries == nullptr))
+    return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_price_snapshot_validate: null argument");
+  std::memset(out, 0, sizeof(*out));
+  out->entry_index = kNone;
+  auto reject = [&](const char* id, uint64_t index = kNone) {
+    out->valid = 0;
+    out->entry_index = index;
+    std::strncpy(out->invariant, id, sizeof(out->invariant) - 1);
+    ClearError();
+    return APGX_OK;
+  };
+  const apgx_price_associations& a = prior->associations;
+  if (a.board_content_hash != expected->board_content_hash ||
+      a.compiler_profile_fingerprint != expected->compiler_profile_fingerprint ||
+      a.geometry_compiler_version != expected->geometry_compiler_version || a.width != expected->width ||
+      a.height != expected->height || a.layers != expected->layers || prior->capacity_units != capacity_units)
+    return reject("allocator.negotiated_plan.prior_association.v1");
+  if (prior->policy_identity != apgx_price_policy_identity(policy))
+    return reject("allocator.negotiated_plan.prior_policy.v1");
+  if ((prior->epoch_index == 0) != (prior->prior_snapshot_identity == 0))
+    return reject("allocator.negotiated_plan.prior_chain.v1");
+  uint64_t factor = 0;
+  if (__builtin_mul_overflow(prior->epoch_index, policy->present_factor_increment, &factor) ||
+      __builtin_add_overflow(factor, policy->initial_present_factor, &factor))
+    return reject("allocator.negotiated_plan.prior_schedule_overflow.v1");
+  if (prior->present_factor != factor) return reject("allocator.negotiated_plan.prior_present_factor.v1");
+  if (prior->price_count > limits->maximum_price_entries)
+    return reject("allocator.negotiated_plan.prior_price_count.v1");
+  const uint64_t cells = static_cast<uint64_t>(a.width) * a.height * a.layers;
+  uint64_t aggregate = 0;
+  for (uint64_t i = 0; i < prior->price_count; ++i) {
+    const apgx_price_entry& e = entries[i];
+    if (e.cell >= cells || (i > 0 && entries[i - 1].cell >= e.cell) ||
+        (e.present_price == 0 && e.historical_price == 0))
+      return reject("allocator.negotiated_plan.prior_price_order.v1", i);
+    uint64_t total = 0;
+    if (__builtin_add_overflow(e.present_price, e.historical_price, &total))
+      return reject("allocator.negotiated_plan.prior_price_sum_overflow.v1", i);
+    if (total != e.total_price) return reject("allocator.negotiated_plan.prior_price_total.v1", i);
+    if (policy->historical_price_increment == 0 || e.historical_price % policy->historical_price_increment != 0)
+      return reject("allocator.negotiated_plan.prior_history_schedule.v1", i);
+    const uint64_t cumulative = e.historical_price / policy->historical_price_increment;
+    uint64_t current = cumulative;
+    if (prior->present_factor == 0) {
+      if (e.present_price != 0) return reject("allocator.negotiated_plan.prior_present_schedule.v1", i);
+    } else {
+      if (e.present_price % prior->present_factor != 0)
+        return reject("allocator.negotiated_plan.prior_present_schedule.v1", i);
+      current = e.present_price / prior->present_factor;
+    }
+    if (current > maximum_overuse_per_epoch)
+      return reject("allocator.negotiated_plan.prior_current_overuse_bound.v1", i);
+    if (cumulative < current) return reject("allocator.negotiated_plan.prior_history_contribution.v1", i);
+    uint64_t max_prior = 0;
+    if (__builtin_mul_overflow(prior->epoch_index, maximum_overuse_per_epoch, &max_prior))
+      return reject("allocator.negotiated_plan.prior_history_bound_overflow.v1", i);
+    if (cumulative - current > max_prior)
+      return reject("allocator.negotiated_plan.prior_cumulative_overuse_bound.v1", i);
+    if (e.present_price > limits->maximum_price_value || e.historical_price > limits->maximum_price_value ||
+        e.total_price > limits->maximum_price_value)
+      return reject("allocator.negotiated_plan.prior_price_bound.v1", i);
+    if (__builtin_add_overflow(aggregate, e.total_price, &aggregate))
+      return reject("allocator.negotiated_plan.prior_aggregate_price_overflow.v1", i);
+    if (aggregate > limits->maximum_aggregate_price)
+      return reject("allocator.negotiated_plan.prior_aggregate_price_bound.v1", i);
+  }
+  if (prior->snapshot_identity == 0 || prior->snapshot_identity != apgx_price_snapshot_identity(prior, entries))
+    return reject("allocator.negotiated_plan.prior_identity.v1");
+  if (prior->epoch_index >= maximum_epoch_index) return reject("allocator.negotiated_plan.epoch_bound.v1");
+  out->valid = 1;
+  ClearError();
+  return APGX_OK;
+}
+
+extern "C" int apgx_price_snapshot_restore(const apgx_price_snapshot* snapshot, const apgx_price_entry* entries,
+                                           uint32_t* present, uint32_t* history) {
+  using namespace apgx;
+  if (snapshot == nullptr || present == nullptr || history == nullptr || (snapshot->price_count && !entries))
+    return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_price_snapshot_restore: null argument");
+  const apgx_price_associations& a = snapshot->associations;
+  const uint64_t cells = static_cast<uint64_t>(a.width) * a.height * a.layers;
+  const uint64_t n = snapshot->price_count;
+  for (uint64_t i = 0; i < n; ++i)
+    if (entries[i].cell >= cells || entries[i].present_price > 0xFFFFFFFFull ||
+        entries[i].historical_price > 0xFFFFFFFFull)
+      return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_price_snapshot_restore: entry outside the u32 planes");
+  if (cells == 0) {
+    ClearError();
+    return APGX_OK;
+  }
+  uint32_t *d_present = nullptr, *d_history = nullptr;
+  apgx_price_entry* d_entries = nullptr;
+  struct Guard {
+    std::vector<void*> p;
+    ~Guard() {
+      for (void* q : p)
+        if (q) (void)hipFree(q);
+    }
+  } guard;
+  APGX_HIP(hipMalloc(&d_present, 4 * cells), "hipMalloc");
+  guard.p.push_back(d_present);
+  APGX_HIP(hipMalloc(&d_history, 4 * cells), "hipMalloc");
+  guard.p.push_back(d_history);
+  hipLaunchKernelGGL(Fill32, dim3(static_cast<unsigned>((cells + 255) / 256)), dim3(256), 0, nullptr, d_present, cells);
+  hipLaunchKernelGGL(Fill32, dim3(static_cast<unsigned>((cells + 255) / 256)), dim3(256), 0, nullptr, d_history, cells);
+  APGX_HIP(hipGetLastError(), "plane clear launch");
+  if (n) {
+    APGX_HIP(hipMalloc(&d_entries, sizeof(apgx_price_entry) * n), "hipMalloc");
+    guard.p.push_back(d_entries);
+    APGX_HIP(hipMemcpy(d_entries, entries, sizeof(apgx_price_entry) * n, hipMemcpyHostToDevice), "entry upload");
+    hipLaunchKernelGGL(SnapshotRestore, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, nullptr, d_entries,
+                       n, d_present, d_history);
+    APGX_HIP(hipGetLastError(), "snapshot restore launch");
+  }
+  APGX_HIP(hipMemcpy(present, d_present, 4 * cells, hipMemcpyDeviceToHost), "present readback");
+  APGX_HIP(hipMemcpy(history, d_history, 4 * cells, hipMemcpyDeviceToHost), "history readback");
   ClearError();
   return APGX_OK;
 }

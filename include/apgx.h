@@ -61,6 +61,12 @@ typedef struct apgx_device_info {
 /* Replaces IPlanarRouteBackend::QueryMetadata (include/apgar/gpu/planar_router.h:550). */
 int apgx_device_info_query(apgx_device_info* out);
 
+/* Candidate device class of the current device, the HIP counterpart of
+ * CudaCandidateDeviceClass (src/candidates/gpu_candidate_adapter.cc:27-35):
+ * "hip-<gfx arch>", e.g. "hip-gfx950".  Fails (APGX_E_UNSUPPORTED) on
+ * incomplete metadata. */
+int apgx_device_class(char* out, uint64_t capacity);
+
 /* Thread-local detail of the last failing call on this thread ("" if none). */
 const char* apgx_last_error(void);
 
@@ -413,6 +419,72 @@ typedef struct apgx_price_epoch {
 } apgx_price_epoch;
 
 void apgx_price_defaults(apgx_price_policy* policy, apgx_price_limits* limits);
+
+/* Price snapshots (NegotiatedPriceSnapshot, include/apgar/allocator/
+ * negotiated_regeneration_plan.h:49-60) on cell resources: the nonzero cells of
+ * the present and history planes in cell order, with the board associations,
+ * the price policy identity, the snapshot chain, the epoch and its present
+ * factor.  Capture compacts the planes on the device (ballot and prefix-scan
+ * compaction); the identity is SnapshotIdentity's field order
+ * (src/allocator/negotiated_regeneration_plan.cc:108-127) with a cell resource
+ * (layer, x, y) in place of an edge resource and the lattice shape among the
+ * associations; validation restates ValidatePriorSnapshot (:211-396) check for
+ * check, with the reference's invariant identifiers; restore writes a
+ * snapshot back into dense planes so the next epoch resumes from it. */
+typedef struct apgx_price_associations {   /* ResourceLatticeAssociations + lattice shape */
+  uint64_t board_content_hash;
+  uint64_t compiler_profile_fingerprint;
+  uint32_t geometry_compiler_version;
+  uint32_t width;
+  uint32_t height;
+  uint32_t layers;
+} apgx_price_associations;
+
+typedef struct apgx_price_entry {          /* NegotiatedResourcePrice on one cell */
+  uint64_t cell;
+  uint64_t present_price;
+  uint64_t historical_price;
+  uint64_t total_price;
+} apgx_price_entry;
+
+typedef struct apgx_price_snapshot {
+  apgx_price_associations associations;
+  uint32_t capacity_units;
+  uint32_t reserved;
+  uint64_t policy_identity;
+  uint64_t prior_snapshot_identity;        /* 0 exactly at epoch 0 */
+  uint64_t epoch_index;
+  uint64_t present_factor;
+  uint64_t price_count;
+  uint64_t snapshot_identity;
+} apgx_price_snapshot;
+
+typedef struct apgx_price_check {
+  uint32_t valid;
+  uint32_t reserved;
+  uint64_t entry_index;                    /* failing entry, or UINT64_MAX */
+  char invariant[64];                      /* e.g. "allocator.negotiated_plan.prior_price_order.v1" */
+} apgx_price_check;
+
+/* PricePolicyIdentity (src/allocator/negotiated_regeneration_plan.cc:98-106). */
+uint64_t apgx_price_policy_identity(const apgx_price_policy* policy);
+uint64_t apgx_price_snapshot_identity(const apgx_price_snapshot* snapshot, const apgx_price_entry* entries);
+/* present / history: host u32 planes of width*height*layers cells.  entries
+ * NULL: only out->price_count is set (size query). */
+int apgx_price_snapshot_capture(const apgx_price_associations* associations, uint32_t capacity_units,
+                                const apgx_price_policy* policy, uint64_t epoch_index,
+                                uint64_t prior_snapshot_identity, const uint32_t* present, const uint32_t* history,
+                                apgx_price_entry* entries, uint64_t entry_capacity, apgx_price_snapshot* out);
+/* maximum_overuse_per_epoch: the most overuse one epoch can put on a cell
+ * (the reference derives it from its candidate-selection bounds; for cells:
+ * nets routed in the epoch minus the cell capacity). */
+int apgx_price_snapshot_validate(const apgx_price_snapshot* prior, const apgx_price_entry* entries,
+                                 const apgx_price_associations* expected, uint32_t capacity_units,
+                                 const apgx_price_policy* policy, const apgx_price_limits* limits,
+                                 uint64_t maximum_epoch_index, uint64_t maximum_overuse_per_epoch,
+                                 apgx_price_check* out);
+int apgx_price_snapshot_restore(const apgx_price_snapshot* snapshot, const apgx_price_entry* entries,
+                                uint32_t* present, uint32_t* history);
 /* history: in/out; present and total (may be NULL): out. */
 int apgx_price_update(uint64_t epoch_index, const apgx_price_policy* policy,
                       const apgx_price_limits* limits, uint64_t cells, const uint32_t* usage,

@@ -82,3 +82,80 @@ def price_epoch(epoch, usage, capacity, history, policy=None, limits=None):
     return OK, None, {"present_factor": factor, "hot_resources": hot, "price_entries": len(prices),
                       "aggregate_price": aggregate, "present": out_present, "history": out_hist,
                       "total": out_total}
+
+
+# --------------------------------------------------------------------------
+# Price snapshots on cell resources.  The entry map is the nonzero cells of
+# the present and history planes in cell order; the identity follows
+# SnapshotIdentity (src/allocator/negotiated_regeneration_plan.cc:108-127)
+# field for field with the cell resource (layer, x, y) in place of the edge
+# resource (layer, x, y, direction), under its own domain string, and the
+# lattice shape after the compiler version.  PricePolicyIdentity (:98-106) is
+# the reference's exactly.
+# --------------------------------------------------------------------------
+class _Fnv:
+    def __init__(self):
+        self.v = 14695981039346656037
+
+    def byte(self, b):
+        self.v = ((self.v ^ b) * 1099511628211) & U64
+
+    def u32(self, x):
+        for i in range(4):
+            self.byte((x >> (8 * i)) & 0xFF)
+
+    def u64(self, x):
+        x &= U64
+        for i in range(8):
+            self.byte((x >> (8 * i)) & 0xFF)
+
+    def string(self, s):
+        self.u64(len(s))
+        for c in s.encode():
+            self.byte(c)
+
+    def nonzero(self):
+        return self.v or 1
+
+
+def policy_identity(policy=None):
+    pol = dict(DEFAULT_POLICY, **(policy or {}))
+    h = _Fnv()
+    h.string("APGAR-P4R06-NEGOTIATED-PRICE-POLICY-V1")
+    h.u64(pol["initial_present_factor"])
+    h.u64(pol["present_factor_increment"])
+    h.u64(pol["historical_price_increment"])
+    return h.nonzero()
+
+
+def snapshot_entries(present, history):
+    """[(cell, present, history, total)] of the nonzero cells, in cell order."""
+    import numpy as np
+    p = np.asarray(present, dtype=np.uint64).reshape(-1)
+    hh = np.asarray(history, dtype=np.uint64).reshape(-1)
+    idx = np.flatnonzero((p | hh) != 0)
+    return [(int(c), int(p[c]), int(hh[c]), int(p[c]) + int(hh[c])) for c in idx]
+
+
+def snapshot_identity(assoc, capacity_units, policy_ident, prior_identity, epoch, present_factor, entries):
+    """assoc: dict(board_content_hash, compiler_profile_fingerprint,
+    geometry_compiler_version, width, height, layers)."""
+    h = _Fnv()
+    h.string("APGX-CELL-PRICE-SNAPSHOT-V1")
+    h.u64(assoc["board_content_hash"])
+    h.u64(assoc["compiler_profile_fingerprint"])
+    h.u32(assoc["geometry_compiler_version"])
+    h.u32(assoc["width"]); h.u32(assoc["height"]); h.u32(assoc["layers"])
+    h.u32(capacity_units)
+    h.u64(policy_ident)
+    h.u64(prior_identity)
+    h.u64(epoch)
+    h.u64(present_factor)
+    h.u64(len(entries))
+    plane = assoc["width"] * assoc["height"]
+    for cell, pres, hist, tot in entries:
+        h.u32(cell // plane)
+        h.u64(cell % assoc["width"])          # x as i64 (non-negative)
+        h.u64((cell % plane) // assoc["width"])
+        h.u64(pres); h.u64(hist); h.u64(tot)
+    return h.nonzero()

@@ -1,9 +1,12 @@
-"""CPU checks of the C-ABI library: it loads and exports every declared symbol.
+"""Checks of the C-ABI library: it loads and exports every declared symbol.
 
-No compute call is made here (there may be no GPU)."""
+The CPU tests make no compute call (there may be no GPU); the one GPU test
+reads the device class."""
 import ctypes
 import re
 from pathlib import Path
+
+import pytest
 
 ROOT = Path(__file__).resolve().parent.parent
 
@@ -40,3 +43,13 @@ def test_last_error_reports_invalid_arguments(apgx):
     rc = lib.apgx_generate_lattice(None, None)
     assert rc == 4
     assert b"invalid" in lib.apgx_last_error()
+
+
+@pytest.mark.gpu
+def test_device_class_names_the_gfx_architecture(apgx, gpu):
+    """apgx_device_class: "hip-" + the gfx name of the device (the HIP
+    counterpart of the reference's "cuda-cc-<major>.<minor>")."""
+    info = apgx.device_info()
+    arch = info.gcn_arch_name.decode().split(":")[0]
+    assert arch.startswith("gfx")
+    assert apgx.device_class() == "hip-" + arch

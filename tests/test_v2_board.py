@@ -134,7 +134,7 @@ def test_net_batch_record_layouts(apgx):
 def test_sealed_upload_routes_like_direct_upload(apgx, gpu):
     print("[case] v2 sealed board upload", flush=True)
     h, plane = _board(apgx, seed=9)
-    nets, pins = apgx.generate_nets(plane, 24, 2, 5, 8, 64, seed=4)
+    nets, pins = apgx.generate_nets(plane, 24, 2, 5, 8, 32, seed=4)
     a = apgx.Lattice(plane, via_cost=20)
     b = apgx.Lattice.from_sealed(h, plane)
     ra, _ = a.route(nets, pins)
