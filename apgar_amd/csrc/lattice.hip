@@ -879,19 +879,20 @@ __device__ __forceinline__ uint32_t Neighbour(uint32_t cell, uint32_t code, uint
 
 template <int TX, int TY>
 __device__ void WalkConnection(const SearchArgs& a, uint32_t r, const NetStatic& ns, NetState& st,
-                               DevConnection& c, uint32_t start, bool reseed, uint32_t n);
+                               DevConnection& c, uint32_t distance, uint32_t start, bool reseed, uint32_t n);
 
 // Runs after every round.  Nets without queued work hold exact distances from
 // their current tree: connect the nearest unconnected pin (ties: lowest pin
 // index) along the predecessor chain, make that path part of the tree and
-// queue its tiles for the next round.  One thread per net.
+// queue its tiles for the next round.  One wave per net: every lane reads the
+// net's state, lane 0 writes it, and the walk is wave-cooperative.
 template <int TX, int TY>
 __device__ void PhaseNet(const SearchArgs& a, uint32_t r, uint32_t n);
 
 template <int TX, int TY>
 __global__ void PhaseKernel(SearchArgs a) {
   const uint32_t r = __builtin_amdgcn_readfirstlane(*a.round_ptr) + 1;  // the round new work goes to
-  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n = blockIdx.x;  // one 64-lane workgroup per net
   if (n < a.net_count) PhaseNet<TX, TY>(a, r, n);
   // The last block to finish advances the device round: every block read it
   // above, before its completion count.
@@ -907,9 +908,10 @@ __global__ void PhaseKernel(SearchArgs a) {
 
 template <int TX, int TY>
 __device__ void PhaseNet(const SearchArgs& a, uint32_t r, uint32_t n) {
+  const bool lead = (threadIdx.x & 63u) == 0;
   // The key buffer read by the round just executed collects pushes of the
   // round after the next one.
-  a.net_minkey[((r - 1) & 1) * a.net_count + n] = kInfD;
+  if (lead) a.net_minkey[((r - 1) & 1) * a.net_count + n] = kInfD;
   NetState& st = a.state[n];
   if (st.status != kSearching) return;
   const uint32_t pending = a.net_minkey[(r & 1) * a.net_count + n];
@@ -931,27 +933,30 @@ __device__ void PhaseNet(const SearchArgs& a, uint32_t r, uint32_t n) {
   // settled (its distance and every label on its predecessor chain exact) once
   // its distance is at most the smallest pending key.
   if (best_pin == kNever) {
-    if (pending >= kInfD) st.status = APGX_NET_DISCONNECTED;
+    if (lead && pending >= kInfD) st.status = APGX_NET_DISCONNECTED;
     return;
   }
   if (best > pending) return;
-  const bool last = st.n_connected + 2 == ns.pin_count;
-  DevConnection& c = a.conns[ns.conn_offset + st.n_connected];
-  c.pin_index = best_pin;
-  c.distance = best;
-  c.cell_offset = 0;
-  c.cell_count = 0;
-  c.reserved = 0;
-  st.cost += best;
-  st.connected |= 1u << best_pin;
-  st.n_connected += 1;
-  if (last) {
+  const uint32_t k = st.n_connected;
+  const uint32_t cost = st.cost, connected = st.connected;
+  const bool last = k + 2 == ns.pin_count;
+  DevConnection& c = a.conns[ns.conn_offset + k];
+  WaveSync();  // every lane has read the state before lane 0 updates it
+  if (lead) {
+    c.pin_index = best_pin;
+    c.distance = best;
+    c.cell_offset = 0;
+    c.cell_count = 0;
+    c.reserved = 0;
+    st.cost = cost + best;
+    st.connected = connected | (1u << best_pin);
+    st.n_connected = k + 1;
     // Nothing depends on the last path: it is written after the round loop so
     // this (per-round) kernel stays short.
-    st.status = kFinalWalk;
-    return;
+    if (last) st.status = kFinalWalk;
   }
-  WalkConnection<TX, TY>(a, r, ns, st, c, a.pins[ns.pin_offset + best_pin], true, n);
+  if (last) return;
+  WalkConnection<TX, TY>(a, r, ns, st, c, best, a.pins[ns.pin_offset + best_pin], true, n);
 }
 
 // Writes the predecessor chain of a settled pin back to the tree (distance 0)
@@ -960,65 +965,140 @@ __device__ void PhaseNet(const SearchArgs& a, uint32_t r, uint32_t n) {
 // `reseed`, the path cells become sources of the next connection step and
 // their tiles are queued.  The step bound and window checks guard against a
 // corrupted word only.
+//
+// One wave walks a path, segment by segment: the wave loads the TX x TY plane
+// of the tile and layer holding the current cell into LDS with coalesced row
+// loads, then lane 0 follows the chain inside it at LDS latency (cell output
+// and reseed stores as it goes) until the chain leaves the segment.  The tiles
+// to queue (each change of tile along the path) collect in LDS and are pushed
+// by all lanes together.  A path thus costs one global round trip per segment
+// instead of one per cell, with the same cells, checks and pushes as a
+// cell-by-cell walk.
+__device__ __forceinline__ uint64_t ShflU64(uint64_t v, int src) {
+  const uint32_t lo = __shfl(static_cast<uint32_t>(v), src, 64);
+  const uint32_t hi = __shfl(static_cast<uint32_t>(v >> 32), src, 64);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+struct WalkState {
+  uint32_t x, y, l;  // current cell (lattice coordinates)
+  uint32_t pushed;   // last queued tile
+  uint64_t len;      // cells written
+  uint32_t done, bad;
+  uint32_t npush;    // tiles waiting in s_push
+};
+constexpr uint32_t kWalkPushes = 256;
+
 template <int TX, int TY>
 __device__ void WalkConnection(const SearchArgs& a, uint32_t r, const NetStatic& ns, NetState& st,
-                               DevConnection& c, uint32_t start, bool reseed, uint32_t n) {
+                               DevConnection& c, uint32_t distance, uint32_t start, bool reseed, uint32_t n) {
+  __shared__ uint32_t s_seg[TX * TY];
+  __shared__ uint32_t s_push[kWalkPushes];
+  __shared__ WalkState s_walk;
+  const int lane = static_cast<int>(threadIdx.x & 63u);
   uint32_t* lab = a.labels + ns.label_offset;
   const uint32_t plane = a.W * a.H;
-  const uint64_t bound = min(static_cast<uint64_t>(c.distance) / max(a.min_edge, 1u) + 1,
+  const uint64_t bound = min(static_cast<uint64_t>(distance) / max(a.min_edge, 1u) + 1,
                              static_cast<uint64_t>(ns.w) * ns.h * a.L);
-  const unsigned long long offset = atomicAdd(a.cell_cursor, static_cast<unsigned long long>(bound));
+  unsigned long long offset = 0;
+  if (lane == 0) offset = atomicAdd(a.cell_cursor, static_cast<unsigned long long>(bound));
+  offset = ShflU64(offset, 0);
   if (offset + bound > a.cell_capacity) {
-    st.status = APGX_NET_INVALID;
-    atomicOr(&a.stats[7], 1ull);  // reported to the caller as resource exhaustion
+    if (lane == 0) {
+      st.status = APGX_NET_INVALID;
+      atomicOr(&a.stats[7], 1ull);  // reported to the caller as resource exhaustion
+    }
     return;
   }
-  uint32_t pushed = kNever;
-  uint32_t len = 0;
-  for (uint32_t v = start;;) {
-    const uint64_t li = LocalIndex(ns, a.W, a.H, v);
-    const uint32_t word = lab[li];
-    if (len >= bound) {
-      st.status = APGX_NET_INVALID;
-      return;
+  if (lane == 0) s_walk = WalkState{start % a.W, (start / a.W) % a.H, start / plane, kNever, 0, 0, 0, 0};
+  WaveSync();
+  for (;;) {
+    const WalkState w = s_walk;
+    if (w.npush == kWalkPushes || ((w.done || w.bad) && w.npush)) {
+      for (uint32_t i = static_cast<uint32_t>(lane); i < w.npush; i += 64) Push(a, s_push[i], n, r, 0u);
+      WaveSync();
+      if (lane == 0) s_walk.npush = 0;
+      WaveSync();
+      continue;
     }
-    a.cells[offset + len++] = v;
-    if ((word >> 3) == 0) break;
-    const uint32_t code = word & 7u;
-    const uint32_t x = v % a.W, y = (v / a.W) % a.H, l = v / plane;
-    const bool ok = code <= kFromAbove && (code != kFromBelow || l > 0) &&
-                    (code != kFromAbove || l + 1 < a.L) && (code != kFromWest || x > ns.x0) &&
-                    (code != kFromEast || x + 1 < ns.x0 + ns.w) && (code != kFromSouth || y > ns.y0) &&
-                    (code != kFromNorth || y + 1 < ns.y0 + ns.h);
-    if (!ok) {
-      st.status = APGX_NET_INVALID;
-      return;
-    }
-    if (reseed) {
-      lab[li] = kSourceWord;
-      const uint32_t g = TileOf<TX, TY>(ns, a.W, a.H, v);
-      if (g != pushed) {
-        Push(a, g, n, r, 0u);
-        pushed = g;
+    if (w.done || w.bad) break;
+    // Segment: the tile (window-relative grid) and layer of the current cell.
+    const uint32_t bx = ((w.x - ns.x0) / TX) * TX, by = ((w.y - ns.y0) / TY) * TY;
+    const uint32_t nx = min(static_cast<uint32_t>(TX), ns.w - bx), ny = min(static_cast<uint32_t>(TY), ns.h - by);
+    const uint32_t* seg = lab + (static_cast<uint64_t>(w.l) * ns.h + by) * ns.w + bx;
+    for (uint32_t y = 0; y < ny; ++y)
+      for (uint32_t x = static_cast<uint32_t>(lane); x < nx; x += 64) s_seg[y * TX + x] = seg[y * ns.w + x];
+    WaveSync();
+    if (lane == 0) {
+      WalkState u = w;
+      const uint32_t x_lo = ns.x0 + bx, y_lo = ns.y0 + by;
+      for (;;) {
+        const uint32_t lx = u.x - x_lo, ly = u.y - y_lo;  // unsigned: below the segment wraps high
+        if (u.l != w.l || lx >= nx || ly >= ny) break;    // the chain left the segment
+        const uint32_t word = s_seg[ly * TX + lx];
+        if (u.len >= bound) {
+          u.bad = 1;
+          break;
+        }
+        const uint32_t cell = (u.l * a.H + u.y) * a.W + u.x;
+        a.cells[offset + u.len++] = cell;
+        if ((word >> 3) == 0) {
+          u.done = 1;
+          break;
+        }
+        const uint32_t code = word & 7u;
+        const bool ok = code <= kFromAbove && (code != kFromBelow || u.l > 0) &&
+                        (code != kFromAbove || u.l + 1 < a.L) && (code != kFromWest || u.x > ns.x0) &&
+                        (code != kFromEast || u.x + 1 < ns.x0 + ns.w) && (code != kFromSouth || u.y > ns.y0) &&
+                        (code != kFromNorth || u.y + 1 < ns.y0 + ns.h);
+        if (!ok) {
+          u.bad = 1;
+          break;
+        }
+        if (reseed) {
+          lab[(static_cast<uint64_t>(u.l) * ns.h + (u.y - ns.y0)) * ns.w + (u.x - ns.x0)] = kSourceWord;
+          const uint32_t g = ns.tile_offset + ((u.y - ns.y0) / TY) * ns.tiles_x + (u.x - ns.x0) / TX;
+          if (g != u.pushed) {
+            s_push[u.npush++] = g;
+            u.pushed = g;
+          }
+        }
+        switch (code) {
+          case kFromBelow: --u.l; break;
+          case kFromSouth: --u.y; break;
+          case kFromWest: --u.x; break;
+          case kFromEast: ++u.x; break;
+          case kFromNorth: ++u.y; break;
+          default: ++u.l; break;
+        }
       }
+      s_walk = u;
     }
-    v = Neighbour(v, code, a.W, plane);
+    WaveSync();
   }
-  c.cell_offset = offset;
-  c.cell_count = len;
+  if (lane == 0) {
+    if (s_walk.bad) {
+      st.status = APGX_NET_INVALID;
+    } else {
+      c.cell_offset = offset;
+      c.cell_count = static_cast<uint32_t>(s_walk.len);
+    }
+  }
 }
 
 // After the round loop: paths of nets whose last pin settled.  One thread per net.
 template <int TX, int TY>
 __global__ void FinalWalkKernel(SearchArgs a) {
-  const uint32_t n = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n = blockIdx.x;  // one 64-lane workgroup per net
   if (n >= a.net_count) return;
   NetState& st = a.state[n];
   if (st.status != kFinalWalk) return;
   const NetStatic ns = a.nets[n];
   DevConnection& c = a.conns[ns.conn_offset + st.n_connected - 1];
-  st.status = APGX_NET_REACHED;
-  WalkConnection<TX, TY>(a, 0u, ns, st, c, a.pins[ns.pin_offset + c.pin_index], false, n);
+  const uint32_t distance = c.distance, pin = c.pin_index;
+  WaveSync();  // every lane has read the state before lane 0 updates it
+  if ((threadIdx.x & 63u) == 0) st.status = APGX_NET_REACHED;
+  WalkConnection<TX, TY>(a, 0u, ns, st, c, distance, a.pins[ns.pin_offset + pin], false, n);
 }
 
 struct TileShape {
@@ -1523,7 +1603,7 @@ int apgx::RouteNets(apgx_lattice* lat, const CostOverride* ov, const apgx_net_qu
         if (status != APGX_OK) return;
         if (timed) check(hipEventRecord(round_events.back(), stream), "hipEventRecord");
         void* pargs[] = {&a};
-        status = Launch(k.phase, dim3((net_count + 63) / 64), dim3(64), 0, stream, pargs, "phase launch");
+        status = Launch(k.phase, dim3(net_count), dim3(64), 0, stream, pargs, "phase launch");
       }
     };
     while (status == APGX_OK && !done) {
@@ -1559,7 +1639,7 @@ int apgx::RouteNets(apgx_lattice* lat, const CostOverride* ov, const apgx_net_qu
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (status == APGX_OK) {
       void* fargs[] = {&a};
-      status = Launch(k.final_walk, dim3((net_count + 63) / 64), dim3(64), 0, stream, fargs, "final walk launch");
+      status = Launch(k.final_walk, dim3(net_count), dim3(64), 0, stream, fargs, "final walk launch");
     }
     check(hipEventRecord(ev1, stream), "hipEventRecord");
     check(hipStreamSynchronize(stream), "hipStreamSynchronize");
