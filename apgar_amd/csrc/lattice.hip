@@ -151,6 +151,31 @@ __device__ __forceinline__ void Push(const SearchArgs& a, uint32_t tile, uint32_
   }
 }
 
+// Wave-aggregated Push, called by every lane of a wave (want = false for lanes
+// with nothing to queue), all for one round.  Keys and the once-per-round
+// stamp stay per lane; the lanes that won their stamp take their worklist
+// slots from one atomic per wave, ordered by a ballot prefix count.
+__device__ __forceinline__ void PushWave(const SearchArgs& a, bool want, uint32_t tile, uint32_t net,
+                                         uint32_t round, uint32_t key) {
+  bool fresh = false;
+  if (want) {
+    atomicMin(&a.tile_key[(round & 1) * a.tile_count + tile], key);
+    atomicMin(&a.net_minkey[(round & 1) * a.net_count + net], key);
+    fresh = atomicMax(&a.tile_stamp[tile], round) < round;
+  }
+  const uint64_t mask = __ballot(fresh);
+  if (mask == 0) return;
+  const int lane = static_cast<int>(threadIdx.x & 63u);
+  const int leader = __builtin_ctzll(mask);
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(&a.counters[2 * round], static_cast<uint32_t>(__popcll(mask)));
+  base = __shfl(base, leader, 64);
+  if (fresh) {
+    a.wl[round & 1][base + static_cast<uint32_t>(__popcll(mask & ((1ull << lane) - 1ull)))] = tile;
+    atomicMax(&a.state[net].last_push, round);
+  }
+}
+
 // Lower bound on the distance from any cell of the window rectangle
 // [x0, x1] x [y0, y1] (lattice coordinates, every layer) to the nearest
 // unconnected pin of the net: each in-layer step costs at least goal_cost and
@@ -770,14 +795,15 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
     // Epilogue: lanes 0..4 of the first wave issue the four neighbour pushes
     // and the self push together; lane 5 counts the net's round, lane 6 the
     // statistics; the popper fetches the next item meanwhile.
-    if (threadIdx.x < 7) {
+    if (threadIdx.x < 64) {  // the first wave
       const uint32_t next = r + 1;
       const int k = threadIdx.x;
       if (k == 0 && !initd) a.tile_init[g] = r;
+      bool want = false;
+      uint32_t target = g, key = kInfD;
       if (k < 5) {
-        const uint32_t key = s_side[k];
         bool exists;
-        uint32_t target, ttx = tx, tty = ty;
+        uint32_t ttx = tx, tty = ty;
         switch (k) {
           case 0: exists = tx > 0; target = g - 1; ttx = tx - 1; break;
           case 1: exists = tx + 1 < ns.tiles_x; target = g + 1; ttx = tx + 1; break;
@@ -785,11 +811,13 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
           case 3: exists = ty + 1 < ns.tiles_y; target = g + ns.tiles_x; tty = ty + 1; break;
           default: exists = !converged; target = g; break;
         }
-        if (key < kInfD && exists)
-          Push(a, target, n, next, TileKey(a, ns, a.state[n].connected, ttx, tty, kRT, kRT, key));
-      } else if (k == 5) {
+        want = s_side[k] < kInfD && exists;
+        if (want) key = TileKey(a, ns, a.state[n].connected, ttx, tty, kRT, kRT, s_side[k]);
+      }
+      PushWave(a, want, target, n, next, key);  // lanes 0..4: four neighbours and the tile itself
+      if (k == 5) {
         if (atomicMax(&a.state[n].last_counted, r) < r) atomicAdd(&a.state[n].rounds, 1u);
-      } else {
+      } else if (k == 6) {
         atomicAdd(&a.stats[0], 1ull);
         atomicAdd(&a.stats[1], static_cast<unsigned long long>(nx) * ny * L);
         atomicAdd(&a.stats[2], static_cast<unsigned long long>(sweeps));
@@ -1015,7 +1043,10 @@ __device__ void WalkConnection(const SearchArgs& a, uint32_t r, const NetStatic&
   for (;;) {
     const WalkState w = s_walk;
     if (w.npush == kWalkPushes || ((w.done || w.bad) && w.npush)) {
-      for (uint32_t i = static_cast<uint32_t>(lane); i < w.npush; i += 64) Push(a, s_push[i], n, r, 0u);
+      for (uint32_t b = 0; b < w.npush; b += 64) {
+        const uint32_t i = b + static_cast<uint32_t>(lane);
+        PushWave(a, i < w.npush, s_push[min(i, w.npush - 1)], n, r, 0u);
+      }
       WaveSync();
       if (lane == 0) s_walk.npush = 0;
       WaveSync();
