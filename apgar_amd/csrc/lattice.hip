@@ -655,6 +655,13 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
 
     uint32_t sweeps = 0, votes = 0;
     bool converged = false;
+    // A quiet layer: unchanged since the last vote, as are the layers next to
+    // it.  Its labels are then a fixed point of the in-layer passes (halos are
+    // fixed for the visit) and of its via inputs, so it skips the in-layer-only
+    // sweeps up to the next full sweep, whose via exchange is where a change
+    // of a neighbouring layer reaches it.  Every wave still meets the same
+    // barriers: those belong to full sweeps only.
+    bool quiet = false;
     const uint32_t period = max(a.sweep_mode, 1u);
 #pragma unroll 1
     while (sweeps < a.max_sweeps) {
@@ -761,6 +768,8 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
         any = false;
 #pragma unroll
         for (int k = 0; k < L; ++k) any |= flags[k] != 0;
+        quiet = __builtin_amdgcn_readfirstlane(
+                    (flags[l] | (l > 0 ? flags[l - 1] : 0u) | (l + 1 < L ? flags[l + 1] : 0u)) == 0 ? 1 : 0) != 0;
         ++votes;
       }
       ++sweeps;
@@ -768,6 +777,7 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
         converged = true;
         break;
       }
+      if (quiet) sweeps += period - 1;  // skip the in-layer-only sweeps up to the next full one
     }
 
     // Write back through the tile buffer; detect border distance changes.
