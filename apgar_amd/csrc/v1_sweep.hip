@@ -631,6 +631,10 @@ extern "C" int apgx_v1_batch_execute(apgx_v1_view* view, const apgx_v1_batch_req
   const uint64_t batch_bytes = lay.total;
   if (batch_bytes + view->persistent_bytes > req->maximum_device_bytes)
     return Fail(APGX_E_RESOURCE_EXHAUSTED, "candidate batch exceeds maximum_device_bytes");
+  // Host budget before execution, with every compact path at its largest
+  // (EstimateCandidateBatchHostBytesV1, src/gpu/planar_router.cc:738-794).
+  if (HostBytes(frontier, qn, nn, total_states) > req->maximum_host_bytes)
+    return Fail(APGX_E_RESOURCE_EXHAUSTED, "candidate batch exceeds maximum_host_bytes");
   auto* pend = new (std::nothrow) apgx_v1_pending();
   if (pend == nullptr) return Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_v1_batch_execute: host allocation");
   pend->query_count = qn;

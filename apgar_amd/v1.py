@@ -268,7 +268,8 @@ class View:
             pass
 
     def execute(self, queries, policy_edges=(), *, batch_id=1, maximum_rounds=100_000,
-                generator=GENERATOR_SWEEP, maximum_device_bytes=(1 << 63), cancel=None, fault=FAULT_NONE):
+                generator=GENERATOR_SWEEP, maximum_device_bytes=(1 << 63), cancel=None, fault=FAULT_NONE,
+                maximum_host_bytes=(1 << 63)):
         """Runs a batch and returns its Readback: results, compact paths and
         packed states (sweep; empty for the frontier), telemetry, and the
         workspace (envelopes for both; flat arrays for the frontier).
@@ -276,7 +277,7 @@ class View:
         library's readback decorator before it is read back."""
         S = len(self.board.nodes) * 9
         req = BatchRequest(1, generator, batch_id, maximum_rounds, 0, S, S, maximum_device_bytes,
-                           1 << 63)
+                           maximum_host_bytes)
         qarr = (Query * max(len(queries), 1))(*queries)
         parr = (PolicyEdge * max(len(policy_edges), 1))(*policy_edges)
         pending = C.c_void_p()

@@ -281,3 +281,25 @@ def test_frontier_budget(apgx, oracle, gpu):
     o = oracle.V1(nodes, profile.costs, profile.heading, profile.tile_w, profile.tile_h)
     _frontier_check(b, o, vc.lattice(profile, start), vc.lattice(profile, goal), rb, 0, max_rounds=40)
     view.close()
+
+
+@pytest.mark.parametrize("generator", [v1.GENERATOR_SWEEP, v1.GENERATOR_FRONTIER])
+def test_host_byte_budget(apgx, gpu, generator):
+    """A batch whose worst-case host bytes exceed maximum_host_bytes is refused
+    before execution; at exactly the estimate it runs.  Estimate: per query a
+    result, a query and (sweep) a compact path header, plus 4 B (sweep, every
+    state on a path at worst) or 28 B (frontier workspace) per state."""
+    profile, start, goal = vc.corpus()["dense_corridors"]
+    _, b = board_for(profile)
+    s, g = b.node_index(*vc.lattice(profile, start)), b.node_index(*vc.lattice(profile, goal))
+    qs = [v1.make_query(i, s, g, b, generator=generator) for i in range(3)]
+    S = len(b.nodes) * 9
+    per_state = 28 if generator == v1.GENERATOR_FRONTIER else 4
+    est = 3 * (120 + 120 + (0 if generator == v1.GENERATOR_FRONTIER else 32)) + 3 * S * per_state
+    view = v1.View(b)
+    with pytest.raises(apgx.ApgxError) as e:
+        view.execute(qs, generator=generator, maximum_host_bytes=est - 1)
+    assert "maximum_host_bytes" in e.value.detail
+    rb = view.execute(qs, generator=generator, maximum_host_bytes=est)
+    assert all(r.completion == v1.REACHED for r in rb.results)
+    view.close()
