@@ -452,3 +452,37 @@ extern "C" int apgx_v1_validate_batch(const apgx_v1_header* header, const apgx_v
   ClearError();
   return APGX_OK;
 }
+
+extern "C" int apgx_v1_generator_shape(uint32_t generator, apgx_v1_generator_shape_info* out) {
+  using namespace apgx;
+  using namespace apgx::v1;
+  if (out == nullptr) return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_v1_generator_shape: null argument");
+  GeneratorShape g{};
+  if (!DescribeGenerator(generator, &g)) return Fail(APGX_E_UNSUPPORTED, "unknown candidate generator");
+  *out = apgx_v1_generator_shape_info{g.chunk_rounds,
+                                      g.fixed_launches,
+                                      g.fixed_examined_work_per_round,
+                                      g.examined_work_per_node_per_round,
+                                      g.turn_relaxations_per_examined_work,
+                                      g.full_workspace ? APGX_V1_READBACK_FULL_WORKSPACE
+                                                       : APGX_V1_READBACK_COMPACT_PATHS,
+                                      1u,
+                                      0u};
+  ClearError();
+  return APGX_OK;
+}
+
+extern "C" int apgx_v1_expected_batch_bytes(uint32_t generator, uint64_t query_count, uint64_t policy_edge_count,
+                                            uint64_t node_count, uint64_t compact_path_states,
+                                            uint64_t* device_bytes, uint64_t* host_bytes,
+                                            uint64_t* readback_bytes) {
+  using namespace apgx;
+  using namespace apgx::v1;
+  GeneratorShape g{};
+  if (!DescribeGenerator(generator, &g)) return Fail(APGX_E_UNSUPPORTED, "unknown candidate generator");
+  if (device_bytes) *device_bytes = LayoutBatch(g.full_workspace, query_count, policy_edge_count, node_count).total;
+  if (host_bytes) *host_bytes = HostBytes(g.full_workspace, query_count, node_count, compact_path_states);
+  if (readback_bytes) *readback_bytes = ReadbackBytes(g.full_workspace, query_count, node_count, compact_path_states);
+  ClearError();
+  return APGX_OK;
+}

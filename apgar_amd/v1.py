@@ -120,6 +120,12 @@ class Validation(C.Structure):
                 ("invariant", C.c_char * 64), ("detail", C.c_char * 128)]
 
 
+class GeneratorShape(C.Structure):
+    _fields_ = [("chunk_rounds", U32), ("fixed_launches", U32), ("fixed_examined_work_per_round", U32),
+                ("examined_work_per_node_per_round", U32), ("heading_turn_relaxations_per_examined_work", U32),
+                ("readback_kind", U32), ("launches_once_per_chunk", U32), ("reserved", U32)]
+
+
 class QueryEnvelope(C.Structure):
     _fields_ = [("workspace_offset", U64), ("workspace_state_count", U64), ("examined_work", U64),
                 ("heading_turn_relaxations", U64), ("rounds", U32), ("reserved", U32)]
@@ -176,6 +182,8 @@ _SIGS = {
     "apgx_v1_batch_readback_workspace": (C.c_int, [C.c_void_p, P(U32), P(QueryEnvelope), P(U64), P(U32),
                                                    P(U64), P(U64), U64, P(U64)]),
     "apgx_v1_pending_inject_fault": (C.c_int, [C.c_void_p, U32]),
+    "apgx_v1_generator_shape": (C.c_int, [U32, P(GeneratorShape)]),
+    "apgx_v1_expected_batch_bytes": (C.c_int, [U32, U64, U64, U64, U64, P(U64), P(U64), P(U64)]),
     "apgx_v1_pending_release": (None, [C.c_void_p]),
     "apgx_v1_validate_batch": (C.c_int, [P(Header), P(Node), U64, P(BatchRequest), P(Query), U32, P(PolicyEdge),
                                          U64, P(Result), P(CompactPath), P(U32), U64, P(WorkspaceView),
@@ -395,3 +403,17 @@ def validate(board: Board, request, queries, policy_edges, results, paths, state
                                         C.byref(wv), C.byref(telemetry), costs, C.byref(out)))
     return (bool(out.valid), out.query_index, out.invariant.decode(), out.detail.decode(), list(costs)[:n],
             out.failure)
+
+
+def generator_shape(generator) -> GeneratorShape:
+    out = GeneratorShape()
+    check(_lib().apgx_v1_generator_shape(generator, C.byref(out)))
+    return out
+
+
+def expected_batch_bytes(generator, query_count, policy_edge_count, node_count, compact_path_states=0):
+    """(device, host, readback) bytes of a batch under this backend's accounting."""
+    d, h, r = U64(), U64(), U64()
+    check(_lib().apgx_v1_expected_batch_bytes(generator, query_count, policy_edge_count, node_count,
+                                              compact_path_states, C.byref(d), C.byref(h), C.byref(r)))
+    return d.value, h.value, r.value

@@ -917,6 +917,34 @@ int apgx_v1_validate_batch(const apgx_v1_header* header, const apgx_v1_node* nod
                            const apgx_v1_batch_telemetry* telemetry, uint64_t* path_costs,
                            apgx_v1_validation* out);
 
+/* This backend's execution shape and deterministic byte accounting, for a
+ * host that validates telemetry (the reference's DescribePlanarGenerator,
+ * include/apgar/gpu/planar_router.h:71-99, and ExpectedCandidateBatchBytes /
+ * EstimateCandidateBatchHostBytesV1, src/gpu/planar_router.cc:1701-1730,
+ * 738-794, model the CUDA backend's launches and buffers; behind this
+ * library a host takes these figures instead).  Launches per batch =
+ * fixed_launches + ceil(dispatched_rounds / chunk_rounds) + finalization;
+ * the sweep fuses the reference's per-round kernels into one launch per
+ * chunk, so launches_once_per_chunk is 1 for both generators. */
+typedef struct apgx_v1_generator_shape_info {
+  uint32_t chunk_rounds;
+  uint32_t fixed_launches;
+  uint32_t fixed_examined_work_per_round;
+  uint32_t examined_work_per_node_per_round;
+  uint32_t heading_turn_relaxations_per_examined_work;
+  uint32_t readback_kind;                  /* APGX_V1_READBACK_* */
+  uint32_t launches_once_per_chunk;
+  uint32_t reserved;
+} apgx_v1_generator_shape_info;
+
+int apgx_v1_generator_shape(uint32_t generator, apgx_v1_generator_shape_info* out);
+/* batch_device_bytes, batch_host_bytes and device_to_host_readback_bytes of a
+ * batch; compact_path_states: packed path states read back (sweep; the
+ * pre-execution host budget uses query_count * represented states). */
+int apgx_v1_expected_batch_bytes(uint32_t generator, uint64_t query_count, uint64_t policy_edge_count,
+                                 uint64_t node_count, uint64_t compact_path_states, uint64_t* device_bytes,
+                                 uint64_t* host_bytes, uint64_t* readback_bytes);
+
 /* ------------------------------------------------------------------------ */
 /* Candidate-batch invariant replays (gpu_batch_replay v1)                   */
 /* ------------------------------------------------------------------------ */

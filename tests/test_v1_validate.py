@@ -280,3 +280,28 @@ def test_library_decorator_matches_restatement(apgx, gpu):
             ok, _, inv, detail, _, _ = _validate(b, req, queries, lib)
             assert not ok and inv == CONTRACT[fault][1], (fault, inv, detail)
     view.close()
+
+
+def test_generator_shape_matches_the_reference_descriptor(apgx):
+    """DescribePlanarGenerator (include/apgar/gpu/planar_router.h:71-99): chunk
+    rounds 32 / 8, fixed launches 2 / 4 and the examined-work bounds are the
+    reference's; the sweep differs only in launching once per chunk."""
+    f, sw = v1.generator_shape(v1.GENERATOR_FRONTIER), v1.generator_shape(v1.GENERATOR_SWEEP)
+    assert (f.chunk_rounds, f.fixed_launches, f.fixed_examined_work_per_round, f.examined_work_per_node_per_round,
+            f.heading_turn_relaxations_per_examined_work, f.readback_kind, f.launches_once_per_chunk) == \
+        (32, 2, 8, 0, 0, v1.READBACK_FULL_WORKSPACE, 1)
+    assert (sw.chunk_rounds, sw.fixed_launches, sw.fixed_examined_work_per_round, sw.examined_work_per_node_per_round,
+            sw.heading_turn_relaxations_per_examined_work, sw.readback_kind, sw.launches_once_per_chunk) == \
+        (8, 4, 0, 8, 8, v1.READBACK_COMPACT_PATHS, 1)
+    with pytest.raises(apgx.ApgxError):
+        v1.generator_shape(7)
+
+
+@pytest.mark.parametrize("generator", [v1.GENERATOR_SWEEP, v1.GENERATOR_FRONTIER])
+@pytest.mark.parametrize("qn, pn, nn, used", [(1, 0, 5, 0), (3, 2, 100, 50), (64, 17, 1234, 900)])
+def test_expected_batch_bytes(apgx, generator, qn, pn, nn, used):
+    frontier = generator == v1.GENERATOR_FRONTIER
+    d, h, r = v1.expected_batch_bytes(generator, qn, pn, nn, used)
+    S = qn * nn * 9
+    readback = qn * 120 + (S * 28 if frontier else qn * 32 + 4 * used)
+    assert (d, h, r) == (layout_bytes(frontier, qn, pn, nn), readback + qn * 120, readback)
