@@ -69,7 +69,8 @@ class RouteOptions(C.Structure):
                 ("rounds_per_sync", C.c_uint32), ("max_device_bytes", C.c_uint64),
                 ("use_graph", C.c_uint32), ("profile_rounds", C.c_uint32),
                 ("frontier_delta", C.c_uint32), ("max_sweeps", C.c_uint32),
-                ("no_goal_bound", C.c_uint32), ("sweep_schedule", C.c_uint32), ("reserved", C.c_uint32)]
+                ("no_goal_bound", C.c_uint32), ("sweep_schedule", C.c_uint32), ("landmarks", C.c_uint32),
+                ("landmark_weight", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class RouteStats(C.Structure):
@@ -214,6 +215,9 @@ _SIGNATURES = {
     "apgx_lattice_create_v2": (C.c_int, [_P(V2BoardHeader), C.c_void_p, C.c_uint64, _P(C.c_void_p)]),
     "apgx_lattice_release": (None, [C.c_void_p]),
     "apgx_lattice_device_costs": (C.c_void_p, [C.c_void_p]),
+    "apgx_lattice_build_landmarks": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double)]),
+    "apgx_lattice_landmark_count": (C.c_uint32, [C.c_void_p]),
+    "apgx_lattice_landmark_field": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_uint32), C.c_uint64]),
     "apgx_route_nets": (C.c_int, [C.c_void_p, _P(NetQuery), C.c_uint32, _P(C.c_uint32), C.c_uint64,
                                   _P(RouteOptions), _P(C.c_void_p)]),
     "apgx_search_sizes": (C.c_int, [C.c_void_p, _P(C.c_uint64), _P(C.c_uint64)]),
@@ -453,12 +457,13 @@ class Lattice:
 
     def route(self, nets, pins: np.ndarray, *, max_rounds=0, tile=(0, 0), rounds_per_sync=0,
               max_device_bytes=0, use_graph=0, profile_rounds=0, frontier_delta=0, max_sweeps=0,
-              goal_bound=True, sweep_schedule=0):
-        """Routes nets (a ctypes NetQuery array) and returns (routes, stats)."""
+              goal_bound=True, sweep_schedule=0, landmarks=False, landmark_weight=0):
+        """Routes nets (a ctypes NetQuery array) and returns (routes, stats).
+        landmarks=True adds the ALT bound of the fields build_landmarks made."""
         pins = np.ascontiguousarray(pins, dtype=np.uint32)
         opts = RouteOptions(max_rounds, tile[0], tile[1], rounds_per_sync, max_device_bytes,
                             use_graph, profile_rounds, frontier_delta, max_sweeps,
-                            0 if goal_bound else 1, sweep_schedule)
+                            0 if goal_bound else 1, sweep_schedule, 1 if landmarks else 0, landmark_weight)
         search = C.c_void_p()
         check(lib().apgx_route_nets(self.handle, nets, len(nets), u32p(pins), pins.size,
                                     C.byref(opts), C.byref(search)))
@@ -485,6 +490,25 @@ class Lattice:
             routes.append(NetRoute(r.net_id, r.status, r.cost, r.rounds, cs))
         return routes, stats
 
+    def build_landmarks(self, count: int = 8) -> float:
+        """Builds `count` landmark distance fields (apgx_lattice_build_landmarks);
+        returns the build's device milliseconds."""
+        ms = C.c_double(0.0)
+        check(lib().apgx_lattice_build_landmarks(self.handle, count, C.byref(ms)))
+        return ms.value
+
+    @property
+    def landmark_count(self) -> int:
+        return int(lib().apgx_lattice_landmark_count(self.handle))
+
+    def landmark_field(self, index: int) -> np.ndarray:
+        """d(landmark index, cell) as an (L, H, W) uint32 array (0x1FFFFFFF:
+        blocked or unreachable)."""
+        L, H, W = self.shape
+        out = np.empty(L * H * W, dtype=np.uint32)
+        check(lib().apgx_lattice_landmark_field(self.handle, index, u32p(out), out.size))
+        return out.reshape(L, H, W)
+
     def autotune(self, nets, pins: np.ndarray, sample_nets=256, **base):
         """Picks tile shape and schedule on a strided sample of the nets.
 
@@ -498,6 +522,8 @@ class Lattice:
                 b.tile_w, b.tile_h = v
             elif k == "goal_bound":
                 b.no_goal_bound = 0 if v else 1
+            elif k == "landmarks":
+                b.landmarks = 1 if v else 0
             else:
                 setattr(b, k, v)
         out, rep = RouteOptions(), TuneReport()

@@ -59,7 +59,13 @@ constexpr uint32_t kDefaultSweepSchedule = 2;
 // width 96..576): 10 sweeps per register-tile visit and a bucket of 48 mean
 // cell costs give 12.0k queries/s against 11.3k for (64, 64 mean costs).
 constexpr uint32_t kRegDefaultSweeps = 10;
-constexpr double kDefaultDeltaCells = 48.0;  // apgx_route_options.sweep_schedule 0
+constexpr double kDefaultDeltaCells = 48.0;  // apgx_route_options.frontier_delta 0
+// With landmarks (four_layer_1024, 1024 pairs, 8 fields): tiles ordered by
+// distance + 15/16 of the bound with 32 sweeps per register-tile visit give
+// 19.5k queries/s against 16.0k for 12/16 with 10 sweeps and 13.1k without
+// landmarks; the full bound (16/16) puts a route's tiles in one bucket (19.0k).
+constexpr uint32_t kDefaultLandmarkWeight = 15;
+constexpr uint32_t kRegLandmarkSweeps = 32;
 constexpr int kMaxTileCells = 16384;
 constexpr int kKeep = kMaxTileCells / kThreads;  // labels kept in registers per thread
 
@@ -121,6 +127,13 @@ struct SearchArgs {
   // only while its key is within `delta` of its net's smallest pending key.
   uint32_t* tile_key;    // 2 * tile_count
   uint32_t* net_minkey;  // 2 * net_count
+  // With landmarks, tile_key orders tiles by distance plus lm_weight / 16 of
+  // the goal bound, and tile_fkey / net_minf carry distance plus the whole
+  // bound, which alone decides when a pin is settled.  (Without landmarks the
+  // two keys are equal and only tile_key / net_minkey are kept.)
+  uint32_t* tile_fkey;   // 2 * tile_count
+  uint32_t* net_minf;    // 2 * net_count
+  uint32_t lm_weight;
   uint32_t tile_count;
   uint32_t delta;
   uint32_t min_edge;     // smallest edge weight: bounds path lengths by distance / min_edge + 1
@@ -131,7 +144,19 @@ struct SearchArgs {
   // Register tiles: every sweep_mode-th sweep (0, k, 2k, ...) is full (via
   // exchange and stability vote); the others relax in-layer edges only.
   uint32_t sweep_mode;
+  // ALT bound: lm_pin[pin slot * lm_count + k] = d(landmark k, pin) and
+  // lm_colmax[k * lm_columns + column] = the largest finite d(landmark k, c)
+  // over the cells c of a 64 x 64 column (all layers); lm_count 0: off.
+  const uint32_t* lm_pin;
+  const uint32_t* lm_colmax;
+  uint32_t lm_count;
+  uint32_t lm_columns_x;
+  uint32_t lm_columns;
+  // 1: no pin is settled; each net ends when its queue is empty.
+  uint32_t explore;
 };
+
+constexpr uint32_t kLmColumn = 64;  // landmark column side (cells)
 
 template <typename CostT>
 __device__ __forceinline__ uint32_t LoadCost(const void* cost, uint64_t index) {
@@ -141,9 +166,13 @@ __device__ __forceinline__ uint32_t LoadCost(const void* cost, uint64_t index) {
 // Queues `tile` for `round` with entry key `key` (a lower bound, minus the
 // entry cost, of any label the pending change can create in the tile).
 __device__ __forceinline__ void Push(const SearchArgs& a, uint32_t tile, uint32_t net,
-                                     uint32_t round, uint32_t key) {
+                                     uint32_t round, uint32_t key, uint32_t fkey) {
   atomicMin(&a.tile_key[(round & 1) * a.tile_count + tile], key);
   atomicMin(&a.net_minkey[(round & 1) * a.net_count + net], key);
+  if (a.lm_count) {
+    atomicMin(&a.tile_fkey[(round & 1) * a.tile_count + tile], fkey);
+    atomicMin(&a.net_minf[(round & 1) * a.net_count + net], fkey);
+  }
   if (atomicMax(&a.tile_stamp[tile], round) < round) {
     const uint32_t slot = atomicAdd(&a.counters[2 * round], 1u);
     a.wl[round & 1][slot] = tile;
@@ -156,11 +185,15 @@ __device__ __forceinline__ void Push(const SearchArgs& a, uint32_t tile, uint32_
 // stamp stay per lane; the lanes that won their stamp take their worklist
 // slots from one atomic per wave, ordered by a ballot prefix count.
 __device__ __forceinline__ void PushWave(const SearchArgs& a, bool want, uint32_t tile, uint32_t net,
-                                         uint32_t round, uint32_t key) {
+                                         uint32_t round, uint32_t key, uint32_t fkey) {
   bool fresh = false;
   if (want) {
     atomicMin(&a.tile_key[(round & 1) * a.tile_count + tile], key);
     atomicMin(&a.net_minkey[(round & 1) * a.net_count + net], key);
+    if (a.lm_count) {
+      atomicMin(&a.tile_fkey[(round & 1) * a.tile_count + tile], fkey);
+      atomicMin(&a.net_minf[(round & 1) * a.net_count + net], fkey);
+    }
     fresh = atomicMax(&a.tile_stamp[tile], round) < round;
   }
   const uint64_t mask = __ballot(fresh);
@@ -178,36 +211,65 @@ __device__ __forceinline__ void PushWave(const SearchArgs& a, bool want, uint32_
 
 // Lower bound on the distance from any cell of the window rectangle
 // [x0, x1] x [y0, y1] (lattice coordinates, every layer) to the nearest
-// unconnected pin of the net: each in-layer step costs at least goal_cost and
-// vias cost nothing in the bound.  The per-cell bound goal_cost * Manhattan
-// distance is consistent (it drops by at most one step's cost along an edge),
-// so adding it to tile keys orders tiles by A* priority without changing any
-// label: a change queued with key g can only create labels whose own
-// priority exceeds g + bound.
+// unconnected pin of the net, the minimum over those pins of two per-pin
+// bounds:
+//  - goal_cost * Manhattan distance: each in-layer step costs at least
+//    goal_cost and vias cost nothing in the bound;
+//  - ALT: for a landmark l with d(l, p) finite, d(c, p) >= d(l, p) - d(l, c)
+//    (triangle inequality of the metric the fields were built on, which no
+//    search metric undercuts), and d(l, c) <= the largest finite d(l, .) over
+//    the 64 x 64 columns the rectangle touches.  Cells unreachable from l
+//    cannot reach p (adjacency is symmetric), so leaving them out of the
+//    maxima is sound.
+// Each per-cell bound is consistent (it drops by at most the weight of an
+// edge), and so are their maximum over landmarks, the minimum over pins and
+// the clamp below kInfD; adding the bound to tile keys therefore orders tiles
+// by A* priority without changing any label: a change queued with key g can
+// only create labels whose own priority exceeds g + bound.
 __device__ __forceinline__ uint32_t GoalBound(const SearchArgs& a, const NetStatic& ns, uint32_t connected,
                                               uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1) {
-  if (a.goal_cost == 0) return 0;
-  uint32_t best = 0xFFFFFFFFu;
+  if (a.goal_cost == 0 && a.lm_count == 0) return 0;
+  const uint32_t cx0 = x0 / kLmColumn, cx1 = x1 / kLmColumn;
+  const uint32_t cy0 = y0 / kLmColumn, cy1 = y1 / kLmColumn;
+  uint64_t best = ~0ull;
   for (uint32_t p = 0; p < ns.pin_count; ++p) {
     if (connected & (1u << p)) continue;
     const uint32_t cell = a.pins[ns.pin_offset + p];
     const uint32_t x = cell % a.W, y = (cell / a.W) % a.H;
     const uint32_t dx = x < x0 ? x0 - x : (x > x1 ? x - x1 : 0u);
     const uint32_t dy = y < y0 ? y0 - y : (y > y1 ? y - y1 : 0u);
-    best = min(best, dx + dy);
+    uint64_t b = static_cast<uint64_t>(dx + dy) * a.goal_cost;
+    const uint32_t* dp = a.lm_pin + static_cast<uint64_t>(ns.pin_offset + p) * a.lm_count;
+    for (uint32_t k = 0; k < a.lm_count; ++k) {
+      const uint32_t to_pin = dp[k];
+      if (to_pin >= kInfD) continue;
+      const uint32_t* col = a.lm_colmax + static_cast<uint64_t>(k) * a.lm_columns;
+      uint32_t far = 0;
+      for (uint32_t cy = cy0; cy <= cy1; ++cy)
+        for (uint32_t cx = cx0; cx <= cx1; ++cx) far = max(far, col[cy * a.lm_columns_x + cx]);
+      if (to_pin > far) b = max(b, static_cast<uint64_t>(to_pin - far));
+    }
+    best = min(best, b);
   }
-  return best == 0xFFFFFFFFu ? 0u : best * a.goal_cost;
+  return best == ~0ull ? 0u : static_cast<uint32_t>(min(best, static_cast<uint64_t>(kInfD)));
 }
 
-// Tile key from the smallest distance a change starts from and the goal bound
-// of the tile it enters; stays below kInfD (kInfD means nothing queued).
-__device__ __forceinline__ uint32_t TileKey(const SearchArgs& a, const NetStatic& ns, uint32_t connected,
-                                            uint32_t tx, uint32_t ty, uint32_t tw, uint32_t th,
-                                            uint32_t dist) {
+// Tile keys from the smallest distance a change starts from and the goal
+// bound of the tile it enters; both stay below kInfD (kInfD means nothing
+// queued).  x: ordering key, y: settling key (equal without landmarks).
+// lm_weight / 16 of a consistent bound is still consistent, so the ordering
+// key is also a valid lower bound; weighting it keeps the tiles along a route
+// in distance order where the full ALT bound would put them all in one bucket.
+__device__ __forceinline__ uint2 TileKeys(const SearchArgs& a, const NetStatic& ns, uint32_t connected,
+                                          uint32_t tx, uint32_t ty, uint32_t tw, uint32_t th, uint32_t dist) {
   const uint32_t x0 = ns.x0 + tx * tw, y0 = ns.y0 + ty * th;
   const uint32_t x1 = min(x0 + tw, ns.x0 + ns.w) - 1, y1 = min(y0 + th, ns.y0 + ns.h) - 1;
-  const uint64_t k = static_cast<uint64_t>(dist) + GoalBound(a, ns, connected, x0, x1, y0, y1);
-  return static_cast<uint32_t>(min(k, static_cast<uint64_t>(kInfD - 1)));
+  const uint64_t b = GoalBound(a, ns, connected, x0, x1, y0, y1);
+  const uint64_t cap = kInfD - 1;
+  const uint32_t f = static_cast<uint32_t>(min(static_cast<uint64_t>(dist) + b, cap));
+  if (a.lm_count == 0) return make_uint2(f, f);
+  const uint32_t k = static_cast<uint32_t>(min(static_cast<uint64_t>(dist) + ((b * a.lm_weight) >> 4), cap));
+  return make_uint2(k, f);
 }
 
 // Admission of a popped item: items of finished nets are dropped, items whose
@@ -218,12 +280,18 @@ __device__ __forceinline__ bool AdmitItem(const SearchArgs& a, uint32_t r, uint3
   uint32_t* slot = &a.tile_key[(r & 1) * a.tile_count + g];
   const uint32_t key = *slot;
   *slot = kInfD;
+  uint32_t fkey = key;
+  if (a.lm_count) {
+    uint32_t* fslot = &a.tile_fkey[(r & 1) * a.tile_count + g];
+    fkey = *fslot;
+    *fslot = kInfD;
+  }
   const uint32_t n = a.tile_net[g];
   if (a.state[n].status != kSearching) return false;
   const uint32_t base = a.net_minkey[(r & 1) * a.net_count + n];
   const uint32_t limit = base >= kInfD - a.delta ? kInfD : base + a.delta;
   if (key > limit) {
-    Push(a, g, n, r + 1, key);
+    Push(a, g, n, r + 1, key, fkey);
     return false;
   }
   return true;
@@ -426,14 +494,23 @@ __device__ void ProcessTile(const SearchArgs& a, uint32_t r, uint32_t g, uint32_
     if (!initd) a.tile_init[g] = r;
     const uint32_t next = r + 1;
     const uint32_t conn = a.state[n].connected;
-    if (s_flags[0] < kInfD && tx > 0) Push(a, g - 1, n, next, TileKey(a, ns, conn, tx - 1, ty, TX, TY, s_flags[0]));
-    if (s_flags[1] < kInfD && tx + 1 < ns.tiles_x)
-      Push(a, g + 1, n, next, TileKey(a, ns, conn, tx + 1, ty, TX, TY, s_flags[1]));
-    if (s_flags[2] < kInfD && ty > 0)
-      Push(a, g - ns.tiles_x, n, next, TileKey(a, ns, conn, tx, ty - 1, TX, TY, s_flags[2]));
-    if (s_flags[3] < kInfD && ty + 1 < ns.tiles_y)
-      Push(a, g + ns.tiles_x, n, next, TileKey(a, ns, conn, tx, ty + 1, TX, TY, s_flags[3]));
-    if (!converged) Push(a, g, n, next, 0);
+    if (s_flags[0] < kInfD && tx > 0) {
+      const uint2 k = TileKeys(a, ns, conn, tx - 1, ty, TX, TY, s_flags[0]);
+      Push(a, g - 1, n, next, k.x, k.y);
+    }
+    if (s_flags[1] < kInfD && tx + 1 < ns.tiles_x) {
+      const uint2 k = TileKeys(a, ns, conn, tx + 1, ty, TX, TY, s_flags[1]);
+      Push(a, g + 1, n, next, k.x, k.y);
+    }
+    if (s_flags[2] < kInfD && ty > 0) {
+      const uint2 k = TileKeys(a, ns, conn, tx, ty - 1, TX, TY, s_flags[2]);
+      Push(a, g - ns.tiles_x, n, next, k.x, k.y);
+    }
+    if (s_flags[3] < kInfD && ty + 1 < ns.tiles_y) {
+      const uint2 k = TileKeys(a, ns, conn, tx, ty + 1, TX, TY, s_flags[3]);
+      Push(a, g + ns.tiles_x, n, next, k.x, k.y);
+    }
+    if (!converged) Push(a, g, n, next, 0, 0);
     if (atomicMax(&a.state[n].last_counted, r) < r) atomicAdd(&a.state[n].rounds, 1u);
     atomicAdd(&a.stats[0], 1ull);
     atomicAdd(&a.stats[1], static_cast<unsigned long long>(nx) * ny * L);
@@ -810,7 +887,7 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
       const int k = threadIdx.x;
       if (k == 0 && !initd) a.tile_init[g] = r;
       bool want = false;
-      uint32_t target = g, key = kInfD;
+      uint32_t target = g, key = kInfD, fkey = kInfD;
       if (k < 5) {
         bool exists;
         uint32_t ttx = tx, tty = ty;
@@ -822,9 +899,13 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
           default: exists = !converged; target = g; break;
         }
         want = s_side[k] < kInfD && exists;
-        if (want) key = TileKey(a, ns, a.state[n].connected, ttx, tty, kRT, kRT, s_side[k]);
+        if (want) {
+          const uint2 kk = TileKeys(a, ns, a.state[n].connected, ttx, tty, kRT, kRT, s_side[k]);
+          key = kk.x;
+          fkey = kk.y;
+        }
       }
-      PushWave(a, want, target, n, next, key);  // lanes 0..4: four neighbours and the tile itself
+      PushWave(a, want, target, n, next, key, fkey);  // lanes 0..4: four neighbours and the tile itself
       if (k == 5) {
         if (atomicMax(&a.state[n].last_counted, r) < r) atomicAdd(&a.state[n].rounds, 1u);
       } else if (k == 6) {
@@ -899,7 +980,7 @@ __global__ __launch_bounds__(kThreads) void SeedKernel(SearchArgs a) {
   if (threadIdx.x == 0) {
     a.tile_init[g] = 0;
     a.state[n].connected = 1u;
-    Push(a, g, n, 1u, 0u);
+    Push(a, g, n, 1u, 0u, 0u);
   }
 }
 
@@ -949,10 +1030,17 @@ __device__ void PhaseNet(const SearchArgs& a, uint32_t r, uint32_t n) {
   const bool lead = (threadIdx.x & 63u) == 0;
   // The key buffer read by the round just executed collects pushes of the
   // round after the next one.
-  if (lead) a.net_minkey[((r - 1) & 1) * a.net_count + n] = kInfD;
+  if (lead) {
+    a.net_minkey[((r - 1) & 1) * a.net_count + n] = kInfD;
+    if (a.lm_count) a.net_minf[((r - 1) & 1) * a.net_count + n] = kInfD;
+  }
   NetState& st = a.state[n];
   if (st.status != kSearching) return;
   const uint32_t pending = a.net_minkey[(r & 1) * a.net_count + n];
+  if (a.explore) {
+    if (lead && pending >= kInfD) st.status = APGX_NET_DISCONNECTED;
+    return;
+  }
   const NetStatic ns = a.nets[n];
   uint32_t* lab = a.labels + ns.label_offset;
   const uint32_t plane = a.W * a.H;
@@ -967,14 +1055,14 @@ __device__ void PhaseNet(const SearchArgs& a, uint32_t r, uint32_t n) {
       best_pin = p;
     }
   }
-  // Pending work can only create labels above its keys, so the nearest pin is
-  // settled (its distance and every label on its predecessor chain exact) once
-  // its distance is at most the smallest pending key.
+  // Pending work can only create labels above its settling keys, so the
+  // nearest pin is settled (its distance and every label on its predecessor
+  // chain exact) once its distance is at most the smallest pending one.
   if (best_pin == kNever) {
     if (lead && pending >= kInfD) st.status = APGX_NET_DISCONNECTED;
     return;
   }
-  if (best > pending) return;
+  if (best > (a.lm_count ? a.net_minf[(r & 1) * a.net_count + n] : pending)) return;
   const uint32_t k = st.n_connected;
   const uint32_t cost = st.cost, connected = st.connected;
   const bool last = k + 2 == ns.pin_count;
@@ -1055,7 +1143,7 @@ __device__ void WalkConnection(const SearchArgs& a, uint32_t r, const NetStatic&
     if (w.npush == kWalkPushes || ((w.done || w.bad) && w.npush)) {
       for (uint32_t b = 0; b < w.npush; b += 64) {
         const uint32_t i = b + static_cast<uint32_t>(lane);
-        PushWave(a, i < w.npush, s_push[min(i, w.npush - 1)], n, r, 0u);
+        PushWave(a, i < w.npush, s_push[min(i, w.npush - 1)], n, r, 0u, 0u);
       }
       WaveSync();
       if (lane == 0) s_walk.npush = 0;
@@ -1142,6 +1230,56 @@ __global__ void FinalWalkKernel(SearchArgs a) {
   WalkConnection<TX, TY>(a, 0u, ns, st, c, distance, a.pins[ns.pin_offset + pin], false, n);
 }
 
+// Per pin slot and landmark: d(landmark k, pin) from the landmark fields
+// (kInfD for a slot whose cell lies outside the lattice: its net is invalid).
+__global__ void LandmarkPinKernel(const uint32_t* __restrict__ pins, uint64_t pin_count,
+                                  const uint32_t* __restrict__ dist, uint64_t cells, uint32_t count,
+                                  uint32_t* __restrict__ out) {
+  const uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  if (i >= pin_count * count) return;
+  const uint32_t cell = pins[i / count];
+  out[i] = cell < cells ? dist[(i % count) * cells + cell] : kInfD;
+}
+
+// Landmark fields from an exploring search whose net k (window: the whole
+// lattice, so label index = cell index) started at landmark k.  One workgroup
+// per (64 x 64 column, landmark): writes d(landmark, cell) for the column's
+// cells (kInfD where the cell is blocked, unreached or its tile never
+// initialised) and the largest finite distance (0 if none) to colmax.
+__global__ __launch_bounds__(256) void LandmarkExtractKernel(const uint32_t* __restrict__ labels,
+                                                             const uint32_t* __restrict__ tile_init,
+                                                             const NetStatic* __restrict__ nets, uint32_t W,
+                                                             uint32_t H, uint32_t L, uint32_t tx, uint32_t ty,
+                                                             uint32_t columns_x, uint32_t* __restrict__ dist,
+                                                             uint32_t* __restrict__ colmax) {
+  __shared__ uint32_t s_max[4];
+  const uint32_t k = blockIdx.y, column = blockIdx.x;
+  const NetStatic ns = nets[k];
+  const uint32_t x0 = (column % columns_x) * kLmColumn, y0 = (column / columns_x) * kLmColumn;
+  const uint32_t nx = min(kLmColumn, W - x0), ny = min(kLmColumn, H - y0);
+  const uint64_t cells = static_cast<uint64_t>(W) * H * L;
+  uint32_t m = 0;
+  for (uint32_t i = threadIdx.x; i < kLmColumn * kLmColumn * L; i += blockDim.x) {
+    const uint32_t x = i % kLmColumn, y = (i / kLmColumn) % kLmColumn, l = i / (kLmColumn * kLmColumn);
+    if (x >= nx || y >= ny) continue;
+    const uint32_t X = x0 + x, Y = y0 + y;
+    const uint64_t c = (static_cast<uint64_t>(l) * H + Y) * W + X;
+    const uint32_t g = ns.tile_offset + (Y / ty) * ns.tiles_x + X / tx;
+    uint32_t d = kInfD;
+    if (tile_init[g] != kNever) d = min(labels[ns.label_offset + c] >> 3, kInfD);
+    dist[k * cells + c] = d;
+    if (d < kInfD) m = max(m, d);
+  }
+  for (int off = 32; off > 0; off >>= 1) m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), off, 64)));
+  if ((threadIdx.x & 63u) == 0) s_max[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t v = 0;
+    for (uint32_t w = 0; w < blockDim.x / 64; ++w) v = max(v, s_max[w]);
+    colmax[static_cast<uint64_t>(k) * gridDim.x + column] = v;
+  }
+}
+
 struct TileShape {
   int tx, ty;
 };
@@ -1194,6 +1332,8 @@ struct apgx_search {
   apgx::DevConnection* d_conns = nullptr;
   uint32_t* d_cells = nullptr;
   unsigned long long* d_cursor = nullptr;
+  const apgx::NetStatic* d_nets = nullptr;  // for ExtractLandmarks
+  const uint32_t* d_tile_init = nullptr;
   uint64_t total_connections = 0;
   apgx_route_stats stats{};
   // host copies filled at the end of the search
@@ -1305,6 +1445,8 @@ extern "C" void apgx_lattice_release(apgx_lattice* lat) {
   if (lat->d_cost) (void)hipFree(lat->d_cost);
   if (lat->cached_labels) (void)hipFree(lat->cached_labels);
   if (lat->cached_block) (void)hipFree(lat->cached_block);
+  if (lat->d_lm_dist) (void)hipFree(lat->d_lm_dist);
+  if (lat->d_lm_colmax) (void)hipFree(lat->d_lm_colmax);
   for (hipEvent_t ev : lat->events)
     if (ev) (void)hipEventDestroy(ev);
   if (lat->stream) (void)hipStreamDestroy(lat->stream);
@@ -1377,7 +1519,7 @@ extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, ui
 
 int apgx::RouteNets(apgx_lattice* lat, const CostOverride* ov, const apgx_net_query* nets,
                     uint32_t net_count, const uint32_t* pins, uint64_t pin_count,
-                    const apgx_route_options* options, apgx_search** out) {
+                    const apgx_route_options* options, apgx_search** out, bool explore) {
   using namespace apgx;
   const auto wall_start = std::chrono::steady_clock::now();
   if (lat == nullptr || out == nullptr || (net_count != 0 && (nets == nullptr || pins == nullptr)))
@@ -1491,10 +1633,14 @@ int apgx::RouteNets(apgx_lattice* lat, const CostOverride* ov, const apgx_net_qu
   Part p_cells{4 * cell_capacity, 0};
   Part p_tile_key{8 * std::max<uint64_t>(tiles, 1), 0};
   Part p_net_key{8 * std::max<uint32_t>(net_count, 1), 0};
+  Part p_tile_fkey{8 * std::max<uint64_t>(tiles, 1), 0};
+  Part p_net_fkey{8 * std::max<uint32_t>(net_count, 1), 0};
   Part p_round{8, 0};
+  const bool landmarks = opt.landmarks != 0 && !opt.no_goal_bound && !explore;
+  Part p_lm_pin{landmarks ? 4 * std::max<uint64_t>(pin_count, 1) * APGX_MAX_LANDMARKS : 4, 0};
   Part* parts[] = {&p_nets, &p_state, &p_tile_net, &p_stamp, &p_init, &p_wl0, &p_wl1,
                    &p_counters, &p_stats, &p_pins, &p_conns, &p_cursor, &p_cells,
-                   &p_tile_key, &p_net_key, &p_round};
+                   &p_tile_key, &p_net_key, &p_round, &p_lm_pin, &p_tile_fkey, &p_net_fkey};
   size_t total = 0;
   for (Part* p : parts) { p->offset = total; total += align(p->bytes); }
   const size_t label_bytes = static_cast<size_t>(std::max<uint64_t>(label_words, 1)) * 4;
@@ -1550,14 +1696,19 @@ int apgx::RouteNets(apgx_lattice* lat, const CostOverride* ov, const apgx_net_qu
   a.cell_capacity = cell_capacity;
   // Default visit length: register tiles stop after kRegDefaultSweeps sweeps
   // and re-queue (a tile's halos usually improve again before a longer visit
-  // would converge); LDS tiles run to convergence.
-  a.max_sweeps = opt.max_sweeps ? opt.max_sweeps : (reg_tile ? kRegDefaultSweeps : 64u);
+  // would converge); with landmarks the visited tiles lie along the routes,
+  // where longer visits pay; LDS tiles run to convergence.
+  a.max_sweeps = opt.max_sweeps ? opt.max_sweeps
+                                 : (reg_tile ? (landmarks ? kRegLandmarkSweeps : kRegDefaultSweeps) : 64u);
   a.tile_key = reinterpret_cast<uint32_t*>(at(p_tile_key));
   a.net_minkey = reinterpret_cast<uint32_t*>(at(p_net_key));
+  a.tile_fkey = reinterpret_cast<uint32_t*>(at(p_tile_fkey));
+  a.net_minf = reinterpret_cast<uint32_t*>(at(p_net_fkey));
   a.round_ptr = reinterpret_cast<uint32_t*>(at(p_round));
   a.tile_count = static_cast<uint32_t>(tiles);
   a.min_edge = std::max(lat->min_edge, 1u);
-  a.goal_cost = opt.no_goal_bound ? 0u : lat->min_cell_cost;
+  a.goal_cost = (opt.no_goal_bound || explore) ? 0u : lat->min_cell_cost;
+  a.explore = explore ? 1u : 0u;
   a.via_pays_cell = (ov && ov->via_pays_cell) ? 1u : 0u;
   a.sweep_mode = opt.sweep_schedule == 0 ? kDefaultSweepSchedule : opt.sweep_schedule;
   // Bucket width: about three quarters of a tile width of distance by
@@ -1573,8 +1724,25 @@ int apgx::RouteNets(apgx_lattice* lat, const CostOverride* ov, const apgx_net_qu
   s->d_conns = a.conns;
   s->d_cells = a.cells;
   s->d_cursor = a.cell_cursor;
+  s->d_nets = a.nets;
+  s->d_tile_init = a.tile_init;
 
   std::unique_lock<std::mutex> lease(lat->exec_mutex);
+  // Landmark fields are replaced only under the lease, so they stay put for
+  // the whole search.
+  if (landmarks) {
+    if (lat->lm_count == 0) {
+      lease.unlock();
+      apgx_search_release(s);
+      return Fail(APGX_E_UNSUPPORTED, "apgx_route_nets: landmarks requested but none are built");
+    }
+    a.lm_pin = reinterpret_cast<uint32_t*>(at(p_lm_pin));
+    a.lm_colmax = lat->d_lm_colmax;
+    a.lm_count = lat->lm_count;
+    a.lm_columns_x = (d.width + kLmColumn - 1) / kLmColumn;
+    a.lm_columns = a.lm_columns_x * ((d.height + kLmColumn - 1) / kLmColumn);
+    a.lm_weight = opt.landmark_weight ? std::min(opt.landmark_weight, 16u) : kDefaultLandmarkWeight;
+  }
   hipStream_t stream = lat->stream;
   hipEvent_t ev0 = lat->events[0], ev1 = lat->events[1];
   int status = APGX_OK;
@@ -1595,8 +1763,16 @@ int apgx::RouteNets(apgx_lattice* lat, const CostOverride* ov, const apgx_net_qu
   check(hipMemsetAsync(a.cell_cursor, 0, 8, stream), "cursor clear");
   check(hipMemsetAsync(a.tile_key, 0xFF, 8 * std::max<uint64_t>(tiles, 1), stream), "key clear");
   check(hipMemsetAsync(a.net_minkey, 0xFF, 8 * std::max<uint32_t>(net_count, 1), stream), "key clear");
+  check(hipMemsetAsync(a.tile_fkey, 0xFF, 8 * std::max<uint64_t>(tiles, 1), stream), "key clear");
+  check(hipMemsetAsync(a.net_minf, 0xFF, 8 * std::max<uint32_t>(net_count, 1), stream), "key clear");
   check(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(a.round_ptr), 1u, 1, stream), "round init");
   check(hipMemsetD32Async(reinterpret_cast<hipDeviceptr_t>(a.round_ptr + 1), 0u, 1, stream), "round init");
+  if (status == APGX_OK && landmarks && pin_count) {
+    const uint64_t slots = pin_count * a.lm_count;
+    LandmarkPinKernel<<<dim3(static_cast<uint32_t>((slots + 255) / 256)), dim3(256), 0, stream>>>(
+        a.pins, pin_count, lat->d_lm_dist, plane * d.layers, a.lm_count, const_cast<uint32_t*>(a.lm_pin));
+    check(hipGetLastError(), "landmark pin launch");
+  }
   if (status == APGX_OK && lds > 65536)
     check(hipFuncSetAttribute(k.round, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)), "hipFuncSetAttribute");
 
@@ -1902,6 +2078,159 @@ extern "C" int apgx_autotune(apgx_lattice* lat, const apgx_net_query* nets, uint
     report->best = best;
     for (uint32_t i = 0; i < cands.size(); ++i) report->candidates[i] = cands[i];
   }
+  ClearError();
+  return APGX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Landmark fields for the ALT goal bound
+// ---------------------------------------------------------------------------
+
+namespace apgx {
+namespace {
+
+// Free cell nearest (x, y) by Chebyshev rings, layer 0 first within a ring;
+// kNever when the lattice has none.
+uint32_t NearestFree(const apgx_lattice* lat, uint32_t x, uint32_t y) {
+  const apgx_lattice_desc& d = lat->desc;
+  const int64_t W = d.width, H = d.height, cx = x, cy = y;
+  const int64_t reach = std::max(W, H);
+  for (int64_t r = 0; r <= reach; ++r) {
+    for (uint32_t l = 0; l < d.layers; ++l) {
+      auto probe = [&](int64_t px, int64_t py) -> uint32_t {
+        if (px < 0 || py < 0 || px >= W || py >= H) return kNever;
+        const uint64_t c = (static_cast<uint64_t>(l) * H + py) * W + px;
+        return lat->HostCost(c) ? static_cast<uint32_t>(c) : kNever;
+      };
+      for (int64_t dx = -r; dx <= r; ++dx) {
+        if (uint32_t c = probe(cx + dx, cy - r); c != kNever) return c;
+        if (uint32_t c = probe(cx + dx, cy + r); c != kNever) return c;
+      }
+      for (int64_t dy = -r + 1; dy <= r - 1; ++dy) {
+        if (uint32_t c = probe(cx - r, cy + dy); c != kNever) return c;
+        if (uint32_t c = probe(cx + r, cy + dy); c != kNever) return c;
+      }
+    }
+  }
+  return kNever;
+}
+
+}  // namespace
+}  // namespace apgx
+
+extern "C" int apgx_lattice_build_landmarks(apgx_lattice* lat, uint32_t count, double* build_ms) {
+  using namespace apgx;
+  if (build_ms) *build_ms = 0.0;
+  if (lat == nullptr) return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_lattice_build_landmarks: null lattice");
+  if (count > APGX_MAX_LANDMARKS)
+    return Fail(APGX_E_UNSUPPORTED, "apgx_lattice_build_landmarks: count exceeds APGX_MAX_LANDMARKS");
+  int device = 0;
+  APGX_HIP(hipGetDevice(&device), "hipGetDevice");
+  if (device != lat->device)
+    return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_lattice_build_landmarks: current device differs from the upload device");
+  auto replace = [&](uint32_t n, uint32_t* dist, uint32_t* colmax) {
+    std::lock_guard<std::mutex> lease(lat->exec_mutex);
+    if (lat->d_lm_dist) (void)hipFree(lat->d_lm_dist);
+    if (lat->d_lm_colmax) (void)hipFree(lat->d_lm_colmax);
+    lat->d_lm_dist = dist;
+    lat->d_lm_colmax = colmax;
+    lat->lm_count = n;
+  };
+  if (count == 0) {
+    replace(0, nullptr, nullptr);
+    ClearError();
+    return APGX_OK;
+  }
+  if (lat->free_cells < 2)
+    return Fail(APGX_E_UNSUPPORTED, "apgx_lattice_build_landmarks: the lattice has fewer than two free cells");
+  const apgx_lattice_desc& d = lat->desc;
+  const uint32_t W = d.width, H = d.height;
+  const uint32_t spots[8][2] = {{0, 0}, {W - 1, H - 1}, {W - 1, 0}, {0, H - 1},
+                                {W / 2, 0}, {W / 2, H - 1}, {0, H / 2}, {W - 1, H / 2}};
+  std::vector<uint32_t> marks;
+  for (const auto& sp : spots) {
+    if (marks.size() == count) break;
+    const uint32_t c = NearestFree(lat, sp[0], sp[1]);
+    if (c != kNever && std::find(marks.begin(), marks.end(), c) == marks.end()) marks.push_back(c);
+  }
+  // Second pin of each exploring net: any other free cell (never settled).
+  const uint64_t cells = static_cast<uint64_t>(W) * H * d.layers;
+  uint32_t free0 = kNever, free1 = kNever;
+  for (uint64_t c = 0; c < cells && free1 == kNever; ++c) {
+    if (lat->HostCost(c) == 0) continue;
+    if (free0 == kNever) free0 = static_cast<uint32_t>(c);
+    else free1 = static_cast<uint32_t>(c);
+  }
+  const uint32_t n = static_cast<uint32_t>(marks.size());
+  std::vector<apgx_net_query> q(n);
+  std::vector<uint32_t> pins(2 * n);
+  for (uint32_t k = 0; k < n; ++k) {
+    q[k] = apgx_net_query{k, 2 * k, 2, apgx_window{0, 0, 0, 0}};
+    pins[2 * k] = marks[k];
+    pins[2 * k + 1] = marks[k] != free0 ? free0 : free1;
+  }
+  apgx_route_options opt{};
+  opt.no_goal_bound = 1;
+  apgx_search* s = nullptr;
+  if (int rc = RouteNets(lat, nullptr, q.data(), n, pins.data(), pins.size(), &opt, &s, true); rc != APGX_OK)
+    return rc;
+  for (uint32_t k = 0; k < n; ++k) {
+    if (s->state[k].status != APGX_NET_DISCONNECTED) {
+      apgx_search_release(s);
+      return Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_lattice_build_landmarks: an exploring search did not finish");
+    }
+  }
+  const uint32_t columns_x = (W + kLmColumn - 1) / kLmColumn;
+  const uint32_t columns = columns_x * ((H + kLmColumn - 1) / kLmColumn);
+  uint32_t* dist = nullptr;
+  uint32_t* colmax = nullptr;
+  hipError_t e = hipMalloc(&dist, 4 * cells * n);
+  if (e == hipSuccess) e = hipMalloc(&colmax, 4ull * columns * n);
+  if (e != hipSuccess) {
+    if (dist) (void)hipFree(dist);
+    apgx_search_release(s);
+    return HipFail(e, "landmark field allocation");
+  }
+  float extract_ms = 0.f;
+  {
+    std::lock_guard<std::mutex> lease(lat->exec_mutex);
+    hipStream_t stream = lat->stream;
+    e = hipEventRecord(lat->events[0], stream);
+    if (e == hipSuccess) {
+      LandmarkExtractKernel<<<dim3(columns, n), dim3(256), 0, stream>>>(
+          static_cast<const uint32_t*>(s->d_labels), s->d_tile_init, s->d_nets, W, H, d.layers,
+          static_cast<uint32_t>(s->shape.tx), static_cast<uint32_t>(s->shape.ty), columns_x, dist, colmax);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(lat->events[1], stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e == hipSuccess) e = hipEventElapsedTime(&extract_ms, lat->events[0], lat->events[1]);
+  }
+  const double search_ms = s->stats.device_ms;
+  apgx_search_release(s);
+  if (e != hipSuccess) {
+    (void)hipFree(dist);
+    (void)hipFree(colmax);
+    return HipFail(e, "landmark extraction");
+  }
+  replace(n, dist, colmax);
+  if (build_ms) *build_ms = search_ms + extract_ms;
+  ClearError();
+  return APGX_OK;
+}
+
+extern "C" uint32_t apgx_lattice_landmark_count(const apgx_lattice* lat) {
+  return lat ? lat->lm_count : 0u;
+}
+
+extern "C" int apgx_lattice_landmark_field(apgx_lattice* lat, uint32_t index, uint32_t* out, uint64_t cell_count) {
+  using namespace apgx;
+  if (lat == nullptr || out == nullptr) return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_lattice_landmark_field: null argument");
+  const uint64_t cells = static_cast<uint64_t>(lat->desc.width) * lat->desc.height * lat->desc.layers;
+  std::lock_guard<std::mutex> lease(lat->exec_mutex);
+  if (index >= lat->lm_count) return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_lattice_landmark_field: no such field");
+  if (cell_count < cells) return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_lattice_landmark_field: output too small");
+  APGX_HIP(hipMemcpy(out, lat->d_lm_dist + index * cells, 4 * cells, hipMemcpyDeviceToHost), "landmark field readback");
   ClearError();
   return APGX_OK;
 }

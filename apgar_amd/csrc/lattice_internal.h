@@ -32,6 +32,11 @@ struct apgx_lattice {
   hipEvent_t events[2] = {nullptr, nullptr};
   size_t lds_limit = 0;       // device attributes, queried once at creation
   uint32_t cu_count = 0;
+  // Landmark fields (apgx_lattice_build_landmarks): lm_count x cells
+  // distances and lm_count x column maxima; replaced under exec_mutex.
+  uint32_t lm_count = 0;
+  uint32_t* d_lm_dist = nullptr;
+  uint32_t* d_lm_colmax = nullptr;
   std::vector<uint8_t> host_costs;  // host copy for request preflight
   uint32_t HostCost(uint64_t cell) const {
     return desc.cost_bytes == 2 ? reinterpret_cast<const uint16_t*>(host_costs.data())[cell]
@@ -53,8 +58,11 @@ struct CostOverride {
   uint32_t via_pays_cell;
 };
 
+// explore: every net searches until its queue is empty (no pin is ever
+// settled; status DISCONNECTED at the end) and keeps its labels for
+// ExtractLandmarks.  Used to build landmark fields.
 int RouteNets(apgx_lattice* lat, const CostOverride* override_plane, const apgx_net_query* nets,
               uint32_t net_count, const uint32_t* pins, uint64_t pin_count,
-              const apgx_route_options* options, apgx_search** out);
+              const apgx_route_options* options, apgx_search** out, bool explore = false);
 
 }  // namespace apgx

@@ -9,6 +9,11 @@ step (hipEvent from the seed kernel to the final path walk).  The rate that
 includes the host side (query upload, result readback) is reported beside it
 as wall_queries_per_s.
 
+By default the searches add the ALT bound of 8 landmark distance fields
+(--landmarks 0 turns it off).  The fields are built once per lattice, like the
+upload, before the warmup; their build time and the rate with that build
+charged to a single step are reported in "landmarks".
+
 --workload eight_layer_2048: an 8 x 2048 x 2048 lattice with the same model,
 4096 nets of 2-8 pins and log-uniform windows of side 32-2048.  Tile shape and
 schedule come from the autotuner on a 256-net sample (reported, not timed), and
@@ -52,6 +57,10 @@ def parse_args():
     p.add_argument("--delta", type=int, default=0, help="tile scheduling bucket width (0: default)")
     p.add_argument("--max-sweeps", type=int, default=0)
     p.add_argument("--no-goal-bound", action="store_true", help="order tiles by distance only")
+    p.add_argument("--landmarks", type=int, default=8,
+                   help="ALT landmark fields added to the goal bound (0: off); built once per lattice "
+                        "before the warmup, their build time reported beside the rate")
+    p.add_argument("--landmark-weight", type=int, default=0, help="ordering weight of the bound in 1/16 (0: default)")
     p.add_argument("--sweep-schedule", type=int, default=0,
                    help="register tiles: 1 full sweeps, 2 alternate in-layer-only and full (0: default)")
     p.add_argument("--graph", type=int, default=-1, help="1/0: replay round chunks as hipGraphs "
@@ -75,14 +84,18 @@ def parse_args():
     return p.parse_args()
 
 
-def read_traffic(workload):
-    """HBM bytes per round-kernel launch from the committed rocprofv3 PMC summary."""
-    f = ROOT / "profiles" / f"round_kernel_{workload}_traffic.txt"
+def read_traffic(workload, landmarks, launches):
+    """HBM bytes per round-kernel launch from the committed rocprofv3 PMC summary
+    of the same configuration (None when there is none).  A summary may give
+    bytes per search instead, which are spread over this run's launches."""
+    f = ROOT / "profiles" / f"round_kernel_{workload}{'' if landmarks else '_no_landmarks'}_traffic.txt"
     if not f.exists():
         return None
     for line in f.read_text().splitlines():
         if line.startswith("traffic_bytes_per_launch:"):
             return float(line.split(":", 1)[1])
+        if line.startswith("traffic_bytes_per_search:"):
+            return float(line.split(":", 1)[1]) / launches
     return None
 
 
@@ -277,6 +290,12 @@ def main():
                 max_sweeps=args.max_sweeps, goal_bound=not args.no_goal_bound,
                 sweep_schedule=args.sweep_schedule,
                 use_graph=(1 if big else 0) if args.graph < 0 else args.graph)
+    landmark_ms = None
+    if args.landmarks:
+        landmark_ms = lat.build_landmarks(args.landmarks)
+        opts["landmarks"] = True
+        opts["landmark_weight"] = args.landmark_weight
+        log(f"[bench] {args.landmarks} landmark fields built in {landmark_ms:.2f} ms (device)")
     tune = None
     if big:
         count = args.queries or spec["nets"]
@@ -365,7 +384,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": read_traffic(args.workload),
+            "traffic": read_traffic(args.workload, args.landmarks, launches),
             "kernel": (f"RegRoundKernel<{L}>" if (st.tile_w, st.tile_h) == (64, 64)
                        else f"RoundKernel<{st.tile_w},{st.tile_h}>"),
             "launches": launches,
@@ -373,6 +392,12 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
     }
+    if landmark_ms is not None:
+        # Per-lattice preparation (like the upload): built once, reused by every
+        # batch on the lattice.  The second rate charges it to a single step.
+        line["landmarks"] = {"count": args.landmarks, "build_ms": landmark_ms,
+                             f"{what}_per_s_with_build_in_one_step": count / ((ms + landmark_ms) / 1e3)}
+        line["config"]["landmarks"] = args.landmarks
     if tune:
         line["autotune"] = tune
     if not args.no_cpu:

@@ -166,6 +166,25 @@ void apgx_lattice_release(apgx_lattice* lattice);
 /* Device address of the lattice's cost plane (read-only for callers). */
 const void* apgx_lattice_device_costs(const apgx_lattice* lattice);
 
+/* Landmark distance fields for the ALT goal bound (no counterpart in the
+ * reference, whose searches are single-query A*: an optional accelerator of
+ * apgx_route_nets that changes no result).  Builds `count` (1 ..
+ * APGX_MAX_LANDMARKS) fields d(landmark, cell) on the lattice's own cost
+ * plane with vias costing via_cost (the cheapest metric any search on this
+ * lattice uses: override planes of the allocator cost at least as much), by
+ * exploring searches of this library, plus the per-column (64 x 64 cells, all
+ * layers) maxima of the finite distances.  Landmarks are the free cells
+ * nearest the board's corners, then its edge midpoints, on layer 0 first.
+ * Replaces earlier fields; count 0 frees them.  *build_ms (optional) receives
+ * the device time of the build.  Device memory: 4 bytes per cell per field. */
+#define APGX_MAX_LANDMARKS 8u
+int apgx_lattice_build_landmarks(apgx_lattice* lattice, uint32_t count, double* build_ms);
+/* Number of landmark fields built (0: none). */
+uint32_t apgx_lattice_landmark_count(const apgx_lattice* lattice);
+/* Copies field `index` (cells in lattice order; 0x1FFFFFFF: blocked or
+ * unreachable) into host memory of `cell_count` entries. */
+int apgx_lattice_landmark_field(apgx_lattice* lattice, uint32_t index, uint32_t* out, uint64_t cell_count);
+
 typedef struct apgx_window {
   uint32_t x0;
   uint32_t y0;
@@ -214,12 +233,19 @@ typedef struct apgx_route_options {
   uint32_t frontier_delta;    /* distance bucket width of tile scheduling; 0: 48 x mean cell
                                  cost; 0xFFFFFFFF: unordered (every queued tile every round) */
   uint32_t max_sweeps;        /* sweeps per tile visit before re-queueing; 0: 10 on register
-                                 tiles, 64 on LDS tiles */
+                                 tiles (32 with landmarks), 64 on LDS tiles */
   uint32_t no_goal_bound;     /* 1: order tiles by distance only; 0: distance plus the
                                  consistent lower bound to the nearest unconnected pin */
   uint32_t sweep_schedule;    /* register tiles: k = one full sweep (via exchange and
                                  stability test) every k sweeps, the rest in-layer only;
                                  1 every sweep full; 0 default (2) */
+  uint32_t landmarks;         /* 1: raise the goal bound with the ALT bound of the
+                                 lattice's landmark fields (apgx_lattice_build_landmarks;
+                                 APGX_E_UNSUPPORTED when none are built); 0: off.
+                                 Distances and paths are the same either way. */
+  uint32_t landmark_weight;   /* with landmarks: tiles are ordered by distance plus
+                                 landmark_weight / 16 of the bound (1 .. 16; 0: 15), pins
+                                 settle on distance plus the whole bound */
   uint32_t reserved;
 } apgx_route_options;
 
