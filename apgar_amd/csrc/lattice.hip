@@ -157,6 +157,7 @@ struct SearchArgs {
 };
 
 constexpr uint32_t kLmColumn = 64;  // landmark column side (cells)
+constexpr uint32_t kLmMinTiles = 4;  // smallest window (in tiles) that uses landmarks
 
 template <typename CostT>
 __device__ __forceinline__ uint32_t LoadCost(const void* cost, uint64_t index) {
@@ -229,6 +230,9 @@ __device__ __forceinline__ void PushWave(const SearchArgs& a, bool want, uint32_
 __device__ __forceinline__ uint32_t GoalBound(const SearchArgs& a, const NetStatic& ns, uint32_t connected,
                                               uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1) {
   if (a.goal_cost == 0 && a.lm_count == 0) return 0;
+  // Windows of at most kLmMinTiles tiles search too little for the landmark
+  // loop to pay; the choice is fixed per net, so its keys stay consistent.
+  const uint32_t lm_count = ns.tiles_x * ns.tiles_y > kLmMinTiles ? a.lm_count : 0u;
   const uint32_t cx0 = x0 / kLmColumn, cx1 = x1 / kLmColumn;
   const uint32_t cy0 = y0 / kLmColumn, cy1 = y1 / kLmColumn;
   uint64_t best = ~0ull;
@@ -240,7 +244,7 @@ __device__ __forceinline__ uint32_t GoalBound(const SearchArgs& a, const NetStat
     const uint32_t dy = y < y0 ? y0 - y : (y > y1 ? y - y1 : 0u);
     uint64_t b = static_cast<uint64_t>(dx + dy) * a.goal_cost;
     const uint32_t* dp = a.lm_pin + static_cast<uint64_t>(ns.pin_offset + p) * a.lm_count;
-    for (uint32_t k = 0; k < a.lm_count; ++k) {
+    for (uint32_t k = 0; k < lm_count; ++k) {
       const uint32_t to_pin = dp[k];
       if (to_pin >= kInfD) continue;
       const uint32_t* col = a.lm_colmax + static_cast<uint64_t>(k) * a.lm_columns;
@@ -2047,6 +2051,7 @@ extern "C" int apgx_autotune(apgx_lattice* lat, const apgx_net_query* nets, uint
       if (t.tx == 64 && t.ty == 64 && reg) {
         cands.push_back(apgx_tune_candidate{64, 64, dl, 16, 0.0});
         cands.push_back(apgx_tune_candidate{64, 64, dl, kRegDefaultSweeps, 0.0});
+        if (opt.landmarks) cands.push_back(apgx_tune_candidate{64, 64, dl, kRegLandmarkSweeps, 0.0});
       }
     }
   }
@@ -2145,8 +2150,10 @@ extern "C" int apgx_lattice_build_landmarks(apgx_lattice* lat, uint32_t count, d
     return Fail(APGX_E_UNSUPPORTED, "apgx_lattice_build_landmarks: the lattice has fewer than two free cells");
   const apgx_lattice_desc& d = lat->desc;
   const uint32_t W = d.width, H = d.height;
-  const uint32_t spots[8][2] = {{0, 0}, {W - 1, H - 1}, {W - 1, 0}, {0, H - 1},
-                                {W / 2, 0}, {W / 2, H - 1}, {0, H / 2}, {W - 1, H / 2}};
+  const uint32_t spots[16][2] = {{0, 0}, {W - 1, H - 1}, {W - 1, 0}, {0, H - 1},
+                                 {W / 2, 0}, {W / 2, H - 1}, {0, H / 2}, {W - 1, H / 2},
+                                 {W / 4, 0}, {3 * (W / 4), H - 1}, {W - 1, H / 4}, {0, 3 * (H / 4)},
+                                 {3 * (W / 4), 0}, {W / 4, H - 1}, {W - 1, 3 * (H / 4)}, {0, H / 4}};
   std::vector<uint32_t> marks;
   for (const auto& sp : spots) {
     if (marks.size() == count) break;

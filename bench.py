@@ -9,7 +9,7 @@ step (hipEvent from the seed kernel to the final path walk).  The rate that
 includes the host side (query upload, result readback) is reported beside it
 as wall_queries_per_s.
 
-By default the searches add the ALT bound of 8 landmark distance fields
+By default the searches add the ALT bound of 16 landmark distance fields
 (--landmarks 0 turns it off).  The fields are built once per lattice, like the
 upload, before the warmup; their build time and the rate with that build
 charged to a single step are reported in "landmarks".
@@ -57,7 +57,7 @@ def parse_args():
     p.add_argument("--delta", type=int, default=0, help="tile scheduling bucket width (0: default)")
     p.add_argument("--max-sweeps", type=int, default=0)
     p.add_argument("--no-goal-bound", action="store_true", help="order tiles by distance only")
-    p.add_argument("--landmarks", type=int, default=8,
+    p.add_argument("--landmarks", type=int, default=16,
                    help="ALT landmark fields added to the goal bound (0: off); built once per lattice "
                         "before the warmup, their build time reported beside the rate")
     p.add_argument("--landmark-weight", type=int, default=0, help="ordering weight of the bound in 1/16 (0: default)")

@@ -174,10 +174,11 @@ const void* apgx_lattice_device_costs(const apgx_lattice* lattice);
  * lattice uses: override planes of the allocator cost at least as much), by
  * exploring searches of this library, plus the per-column (64 x 64 cells, all
  * layers) maxima of the finite distances.  Landmarks are the free cells
- * nearest the board's corners, then its edge midpoints, on layer 0 first.
+ * nearest the board's corners, then its edge midpoints, then its edge
+ * quarter points, on layer 0 first.
  * Replaces earlier fields; count 0 frees them.  *build_ms (optional) receives
  * the device time of the build.  Device memory: 4 bytes per cell per field. */
-#define APGX_MAX_LANDMARKS 8u
+#define APGX_MAX_LANDMARKS 16u
 int apgx_lattice_build_landmarks(apgx_lattice* lattice, uint32_t count, double* build_ms);
 /* Number of landmark fields built (0: none). */
 uint32_t apgx_lattice_landmark_count(const apgx_lattice* lattice);

@@ -24,7 +24,8 @@ def _say(msg):
     print(msg, flush=True)
 
 
-@pytest.mark.parametrize("shape, via, count", [((1, 70, 90), 0, 8), ((3, 130, 100), 20, 8), ((2, 64, 200), 5, 3)])
+@pytest.mark.parametrize("shape, via, count", [((1, 70, 90), 0, 8), ((3, 130, 100), 20, 8), ((2, 64, 200), 5, 3),
+                                               ((2, 96, 96), 4, 16)])
 def test_fields_match_oracle(apgx, oracle, gpu, shape, via, count):
     _say(f"[case] landmark fields {shape} via={via} count={count}")
     rng = np.random.default_rng(sum(shape) + count)
@@ -59,7 +60,7 @@ def test_route_without_fields_is_refused(apgx, gpu):
         lat.route_pairs(np.array([[0, 255]]), landmarks=True)
     assert "none are built" in e.value.detail
     with pytest.raises(apgx.ApgxError):
-        lat.build_landmarks(9)
+        lat.build_landmarks(17)
     lat.close()
 
 
@@ -72,7 +73,7 @@ def test_pairs_match_oracle(apgx, oracle, gpu, seed):
     via = 0 if L == 1 else [20, 1, 7, 3][seed % 4]
     tile = [(0, 0), (32, 16), (32, 32), (64, 32)][seed % 4]
     delta = [0, 0xFFFFFFFF, 1, 50][(seed // 2) % 4]
-    count = [8, 1, 4, 8][seed % 4]
+    count = [8, 1, 4, 16][seed % 4]
     _say(f"[case] landmark pairs seed={seed} shape={(L, H, W)} via={via} tile={tile} delta={delta} count={count}")
     free = np.flatnonzero(cost.reshape(-1))
     pairs = rng.choice(free, size=(16, 2), replace=True)
