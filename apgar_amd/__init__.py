@@ -79,7 +79,8 @@ class RouteStats(C.Structure):
                 ("relaxations", C.c_uint64), ("inner_sweeps", C.c_uint64),
                 ("workspace_bytes", C.c_uint64), ("rounds", C.c_uint32), ("phases", C.c_uint32),
                 ("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("host_syncs", C.c_uint32),
-                ("graph_launches", C.c_uint32), ("round_kernel_ms", C.c_double)]
+                ("graph_launches", C.c_uint32), ("round_kernel_ms", C.c_double),
+                ("tiles_touched", C.c_uint64)]
 
 
 class TuneCandidate(C.Structure):

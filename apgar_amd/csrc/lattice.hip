@@ -520,6 +520,7 @@ __device__ void ProcessTile(const SearchArgs& a, uint32_t r, uint32_t g, uint32_
     atomicAdd(&a.stats[1], static_cast<unsigned long long>(nx) * ny * L);
     atomicAdd(&a.stats[2], static_cast<unsigned long long>(sweeps));
     atomicAdd(&a.stats[3], static_cast<unsigned long long>(sweeps) * nx * ny * L);
+    if (!initd) atomicAdd(&a.stats[4], 1ull);
   }
 }
 
@@ -917,6 +918,7 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
         atomicAdd(&a.stats[1], static_cast<unsigned long long>(nx) * ny * L);
         atomicAdd(&a.stats[2], static_cast<unsigned long long>(sweeps));
         atomicAdd(&a.stats[3], static_cast<unsigned long long>(sweeps) * nx * ny * L);
+        if (!initd) atomicAdd(&a.stats[4], 1ull);
       }
     }
     if (popper) s_item[(iter + 1) & 1] = PopAdmitted(a, r, count);
@@ -1905,6 +1907,7 @@ int apgx::RouteNets(apgx_lattice* lat, const CostOverride* ov, const apgx_net_qu
   rs.tile_visits = stats[0];
   rs.cell_visits = stats[1];
   rs.inner_sweeps = stats[2];
+  rs.tiles_touched = stats[4];
   rs.relaxations = stats[3] * (d.layers > 1 && d.via_cost ? 6ull : 4ull);
   rs.algorithmic_bytes = stats[1] * (static_cast<uint64_t>(cost_bytes) + 8ull);
   rs.workspace_bytes = total + s->label_bytes;

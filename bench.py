@@ -376,6 +376,7 @@ def main():
         "reached": reached,
         "rounds": st.rounds,
         "tile_visits": st.tile_visits,
+        "tiles_touched": st.tiles_touched,
         "inner_sweeps_per_visit": st.inner_sweeps / max(st.tile_visits, 1),
         "relaxations_per_s": st.relaxations / (ms / 1e3),
         "roofline": {

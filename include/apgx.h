@@ -266,6 +266,7 @@ typedef struct apgx_route_stats {
   uint32_t host_syncs;
   uint32_t graph_launches;
   double round_kernel_ms;        /* summed hipEvent time of the round kernels */
+  uint64_t tiles_touched;        /* distinct tiles visited (tile_visits counts revisits) */
 } apgx_route_stats;
 
 typedef struct apgx_search apgx_search;
