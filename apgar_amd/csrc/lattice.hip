@@ -872,12 +872,31 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
     for (int y = 0; y < kRT; ++y) col_min = min(col_min, lab[y]);
     if (!converged) atomicMin(&s_side[4], col_min >> 3);
     {
-      const uint32_t ds = T[lane] >> 3, dn = T[(ny - 1) * kRP + lane] >> 3;
-      const uint32_t dw = T[lane * kRP] >> 3, de = T[lane * kRP + nx - 1] >> 3;
-      if (xin && ds != (orig_s >> 3)) atomicMin(&s_side[2], ds);
-      if (xin && dn != (orig_n >> 3)) atomicMin(&s_side[3], dn);
-      if (lane < ny && dw != (orig_w >> 3)) atomicMin(&s_side[0], dw);
-      if (lane < ny && de != (orig_e >> 3)) atomicMin(&s_side[1], de);
+      // A border cell whose distance changed wakes the neighbouring tile only
+      // if relaxing the edge into the neighbour's cell beats that cell's label
+      // as read at the start of this visit (the halo; kInf when the neighbour
+      // was not written before this round).  Labels only decrease and a tile's
+      // labels are written by its own visit alone, so the current label is no
+      // higher than the halo: the test may wake a tile needlessly but never
+      // misses a change the neighbour's fixed point depends on.
+      const uint32_t ws = T[lane], wn = T[(ny - 1) * kRP + lane];
+      const uint32_t ww = T[lane * kRP], we = T[lane * kRP + nx - 1];
+      auto cost8 = [&](int64_t off) -> uint32_t {
+        const uint32_t c = cost_bytes == 2 ? cbase16[off] : cbase32[off];
+        return c ? c << 3 : kBlocked8;
+      };
+      if (xin && ty > 0 && (ws >> 3) != (orig_s >> 3) &&
+          Relax8(ws, cost8(static_cast<int64_t>(lane) - static_cast<int64_t>(a.W)), kFromNorth) < halo_s)
+        atomicMin(&s_side[2], ws >> 3);
+      if (xin && ty + 1 < ns.tiles_y && (wn >> 3) != (orig_n >> 3) &&
+          Relax8(wn, cost8(static_cast<int64_t>(kRT) * a.W + lane), kFromSouth) < halo_n)
+        atomicMin(&s_side[3], wn >> 3);
+      if (lane < ny && tx > 0 && (ww >> 3) != (orig_w >> 3) &&
+          Relax8(ww, cost8(static_cast<int64_t>(lane) * a.W - 1), kFromEast) < halo_w)
+        atomicMin(&s_side[0], ww >> 3);
+      if (lane < ny && tx + 1 < ns.tiles_x && (we >> 3) != (orig_e >> 3) &&
+          Relax8(we, cost8(static_cast<int64_t>(lane) * a.W + kRT), kFromWest) < halo_e)
+        atomicMin(&s_side[1], we >> 3);
     }
     if (xin) {
 #pragma unroll 8

@@ -74,13 +74,15 @@ def test_pairs_match_oracle(apgx, oracle, gpu, seed):
     tile = [(0, 0), (32, 16), (32, 32), (64, 32)][seed % 4]
     delta = [0, 0xFFFFFFFF, 1, 50][(seed // 2) % 4]
     count = [8, 1, 4, 16][seed % 4]
-    _say(f"[case] landmark pairs seed={seed} shape={(L, H, W)} via={via} tile={tile} delta={delta} count={count}")
+    weight = [0, 16, 1, 8, 16, 0, 12, 1][seed]
+    _say(f"[case] landmark pairs seed={seed} shape={(L, H, W)} via={via} tile={tile} delta={delta} "
+         f"count={count} weight={weight}")
     free = np.flatnonzero(cost.reshape(-1))
     pairs = rng.choice(free, size=(16, 2), replace=True)
     pairs = pairs[pairs[:, 0] != pairs[:, 1]]
     lat = apgx.Lattice(cost, via_cost=via)
     lat.build_landmarks(count)
-    routes, _ = lat.route_pairs(pairs, tile=tile, frontier_delta=delta, landmarks=True)
+    routes, _ = lat.route_pairs(pairs, tile=tile, frontier_delta=delta, landmarks=True, landmark_weight=weight)
     o = oracle.Dense(cost, via)
     for (s, t), r in zip(pairs, routes):
         dist = o.dijkstra([int(s)])
