@@ -484,11 +484,24 @@ __device__ void ProcessTile(const SearchArgs& a, uint32_t r, uint32_t g, uint32_
         const uint32_t v = s_lab[(l * (TY + 2) + y + 1) * PX + x + 1];
         if (v != orig[k] || !initd) lab[(l * h + by + y) * w + bx + x] = v;
         if ((v >> 3) != (orig[k] >> 3)) {
+          // Wake test, as in RegRoundKernel: the neighbour's cell across the
+          // border is queued only if this edge beats its halo label (the
+          // staged ring, untouched by the sweeps; kInf when not yet written).
           const uint32_t d = v >> 3;
-          if (x == 0) atomicMin(&s_flags[0], d);
-          if (x == nx - 1) atomicMin(&s_flags[1], d);
-          if (y == 0) atomicMin(&s_flags[2], d);
-          if (y == ny - 1) atomicMin(&s_flags[3], d);
+          const uint64_t gidx = (static_cast<uint64_t>(l) * a.H + ns.y0 + by + y) * a.W + ns.x0 + bx + x;
+          auto wakes = [&](int64_t off, uint32_t halo, uint32_t code) {
+            const uint64_t cell = static_cast<uint64_t>(static_cast<int64_t>(gidx) + off);
+            const uint32_t c = cost_bytes == 2 ? LoadCost<uint16_t>(a.cost, cell) : LoadCost<uint32_t>(a.cost, cell);
+            return c != 0 && Step(d, c, code) < halo;
+          };
+          const uint32_t* ring = s_lab + l * (TY + 2) * PX;
+          if (x == 0 && tx > 0 && wakes(-1, ring[(y + 1) * PX], kFromEast)) atomicMin(&s_flags[0], d);
+          if (x == nx - 1 && tx + 1 < ns.tiles_x && wakes(1, ring[(y + 1) * PX + TX + 1], kFromWest))
+            atomicMin(&s_flags[1], d);
+          if (y == 0 && ty > 0 && wakes(-static_cast<int64_t>(a.W), ring[x + 1], kFromNorth))
+            atomicMin(&s_flags[2], d);
+          if (y == ny - 1 && ty + 1 < ns.tiles_y && wakes(static_cast<int64_t>(a.W), ring[(TY + 1) * PX + x + 1], kFromSouth))
+            atomicMin(&s_flags[3], d);
         }
       }
     }
