@@ -292,8 +292,10 @@ void apgx_search_release(apgx_search* search);
  * writes the fastest candidate into *tuned; every other field is copied from
  * *base (NULL: defaults).  The set: tiles 64x64 (register-resident, when the
  * lattice allows), 64x32 and 32x32 (LDS); delta 16, 48 and 256 x mean cell
- * cost and unordered; max_sweeps 64, 16 and 10 on 64x64 tiles.  Results do not
- * depend on the choice (labels are the unique fixed point); only speed does.
+ * cost and unordered; max_sweeps 64, 16 and 10 on 64x64 tiles (and 32 when
+ * base->landmarks is set).  Each candidate is timed twice, the faster run
+ * kept.  Results do not depend on the choice (labels are the unique fixed
+ * point); only speed does.
  * No reference counterpart: the reference has one fixed launch shape. */
 #define APGX_TUNE_MAX_CANDIDATES 24u
 typedef struct apgx_tune_candidate {
@@ -301,7 +303,7 @@ typedef struct apgx_tune_candidate {
   uint32_t tile_h;
   uint32_t frontier_delta;
   uint32_t max_sweeps;
-  double device_ms;            /* search time of the sample */
+  double device_ms;            /* search time of the sample (faster of two runs) */
 } apgx_tune_candidate;
 
 typedef struct apgx_tune_report {
