@@ -99,6 +99,21 @@ def read_traffic(workload, landmarks, launches):
     return None
 
 
+def read_issue(workload, landmarks):
+    """VALU and LDS issue utilisation of the round kernel from the committed
+    rocprofv3 SQ-counter summary of the same configuration (None when there is
+    none): the kernel's practical bound, beside the HBM roofline."""
+    f = ROOT / "profiles" / f"round_kernel_{workload}{'' if landmarks else '_no_landmarks'}_sq_counters.txt"
+    if not f.exists():
+        return None
+    out = {}
+    for line in f.read_text().splitlines():
+        for key in ("valu_busy", "lds_busy"):
+            if line.startswith(key + ":"):
+                out[key] = float(line.split(":", 1)[1])
+    return dict(out, source=f"profiles/{f.name}") if out else None
+
+
 def cpu_pairs_baseline(cost, via, pairs, sample, threads):
     """CPU A* oracle (restated reference algorithm) on a bounded sample of pairs."""
     from oracle import py as oracle_py
@@ -393,6 +408,9 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
     }
+    issue = read_issue(args.workload, args.landmarks)
+    if issue:
+        line["issue"] = issue
     if landmark_ms is not None:
         # Per-lattice preparation (like the upload): built once, reused by every
         # batch on the lattice.  The second rate charges it to a single step.
