@@ -242,7 +242,8 @@ typedef struct apgx_route_options {
                                  1 every sweep full; 0 default (2) */
   uint32_t landmarks;         /* 1: raise the goal bound with the ALT bound of the
                                  lattice's landmark fields (apgx_lattice_build_landmarks;
-                                 APGX_E_UNSUPPORTED when none are built); 0: off.
+                                 APGX_E_UNSUPPORTED when none are built); 0: off; ignored
+                                 with no_goal_bound.
                                  Distances and paths are the same either way. */
   uint32_t landmark_weight;   /* with landmarks: tiles are ordered by distance plus
                                  landmark_weight / 16 of the bound (1 .. 16; 0: 15), pins
