@@ -145,8 +145,10 @@ struct SearchArgs {
   // exchange and stability vote); the others relax in-layer edges only.
   uint32_t sweep_mode;
   // ALT bound: lm_pin[pin slot * lm_count + k] = d(landmark k, pin) and
-  // lm_colmax[k * lm_columns + column] = the largest finite d(landmark k, c)
-  // over the cells c of a 64 x 64 column (all layers); lm_count 0: off.
+  // lm_colmax[(k * lm_columns + column) * kLmParts + part] = the largest
+  // finite d(landmark k, c) over the cells c of a 64 x 64 column (all layers):
+  // part 0 the whole column, 1..4 its west, east, south and north edge
+  // lines; lm_count 0: off.
   const uint32_t* lm_pin;
   const uint32_t* lm_colmax;
   uint32_t lm_count;
@@ -157,6 +159,7 @@ struct SearchArgs {
 };
 
 constexpr uint32_t kLmColumn = 64;  // landmark column side (cells)
+constexpr uint32_t kLmParts = 5;    // per column: whole, west, east, south, north edge
 constexpr uint32_t kLmMinTiles = 4;  // smallest window (in tiles) that uses landmarks
 
 template <typename CostT>
@@ -210,47 +213,71 @@ __device__ __forceinline__ void PushWave(const SearchArgs& a, bool want, uint32_
   }
 }
 
-// Lower bound on the distance from any cell of the window rectangle
-// [x0, x1] x [y0, y1] (lattice coordinates, every layer) to the nearest
-// unconnected pin of the net, the minimum over those pins of two per-pin
-// bounds:
+// Lower bound on the distance to the nearest unconnected pin of the net from
+// the cells a queued change can enter first: with side < 0 any cell of the
+// window rectangle [x0, x1] x [y0, y1] (lattice coordinates, every layer);
+// with side 0..3 only its west, east, south or north edge line, the one a
+// change from the neighbouring tile crosses.  (A change entering at cell c'
+// creates labels whose priority is at least its exit distance + 1 + the
+// bound at c', and the consistency of the per-cell bound carries that to
+// every later cell, so the entry cells are all the key has to cover.)  It is
+// the minimum over those pins of two per-pin bounds:
 //  - goal_cost * Manhattan distance: each in-layer step costs at least
 //    goal_cost and vias cost nothing in the bound;
 //  - ALT: for a landmark l with d(l, p) finite, d(c, p) >= d(l, p) - d(l, c)
 //    (triangle inequality of the metric the fields were built on, which no
 //    search metric undercuts), and d(l, c) <= the largest finite d(l, .) over
-//    the 64 x 64 columns the rectangle touches.  Cells unreachable from l
-//    cannot reach p (adjacency is symmetric), so leaving them out of the
-//    maxima is sound.
+//    a set of cells holding the entry cells: the entry edge line of the one
+//    64 x 64 column that holds the rectangle when that edge lies on the
+//    column's own edge, else every column the rectangle touches (at most
+//    2 x 2).  Cells unreachable from l cannot reach p (adjacency is
+//    symmetric), so leaving them out of the maxima is sound.
 // Each per-cell bound is consistent (it drops by at most the weight of an
 // edge), and so are their maximum over landmarks, the minimum over pins and
 // the clamp below kInfD; adding the bound to tile keys therefore orders tiles
 // by A* priority without changing any label: a change queued with key g can
 // only create labels whose own priority exceeds g + bound.
 __device__ __forceinline__ uint32_t GoalBound(const SearchArgs& a, const NetStatic& ns, uint32_t connected,
-                                              uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1) {
+                                              uint32_t x0, uint32_t x1, uint32_t y0, uint32_t y1, int side) {
   if (a.goal_cost == 0 && a.lm_count == 0) return 0;
   // Windows of at most kLmMinTiles tiles search too little for the landmark
   // loop to pay; the choice is fixed per net, so its keys stay consistent.
   const uint32_t lm_count = ns.tiles_x * ns.tiles_y > kLmMinTiles ? a.lm_count : 0u;
   const uint32_t cx0 = x0 / kLmColumn, cx1 = x1 / kLmColumn;
   const uint32_t cy0 = y0 / kLmColumn, cy1 = y1 / kLmColumn;
+  // The entry edge's own maximum applies when the rectangle lies in one
+  // column and its entry edge is on that column's edge.
+  uint32_t part = 0;
+  if (side >= 0 && cx0 == cx1 && cy0 == cy1) {
+    const bool on_edge = side == 0   ? x0 == cx0 * kLmColumn
+                         : side == 1 ? x1 == min(cx0 * kLmColumn + kLmColumn, a.W) - 1
+                         : side == 2 ? y0 == cy0 * kLmColumn
+                                     : y1 == min(cy0 * kLmColumn + kLmColumn, a.H) - 1;
+    if (on_edge) part = 1u + static_cast<uint32_t>(side);
+  }
+  // Entry cells for the Manhattan part.
+  const uint32_t ex0 = side == 1 ? x1 : x0, ex1 = side == 0 ? x0 : x1;
+  const uint32_t ey0 = side == 3 ? y1 : y0, ey1 = side == 2 ? y0 : y1;
   uint64_t best = ~0ull;
   for (uint32_t p = 0; p < ns.pin_count; ++p) {
     if (connected & (1u << p)) continue;
     const uint32_t cell = a.pins[ns.pin_offset + p];
     const uint32_t x = cell % a.W, y = (cell / a.W) % a.H;
-    const uint32_t dx = x < x0 ? x0 - x : (x > x1 ? x - x1 : 0u);
-    const uint32_t dy = y < y0 ? y0 - y : (y > y1 ? y - y1 : 0u);
+    const uint32_t dx = x < ex0 ? ex0 - x : (x > ex1 ? x - ex1 : 0u);
+    const uint32_t dy = y < ey0 ? ey0 - y : (y > ey1 ? y - ey1 : 0u);
     uint64_t b = static_cast<uint64_t>(dx + dy) * a.goal_cost;
     const uint32_t* dp = a.lm_pin + static_cast<uint64_t>(ns.pin_offset + p) * a.lm_count;
     for (uint32_t k = 0; k < lm_count; ++k) {
       const uint32_t to_pin = dp[k];
       if (to_pin >= kInfD) continue;
-      const uint32_t* col = a.lm_colmax + static_cast<uint64_t>(k) * a.lm_columns;
+      const uint32_t* col = a.lm_colmax + static_cast<uint64_t>(k) * a.lm_columns * kLmParts;
       uint32_t far = 0;
-      for (uint32_t cy = cy0; cy <= cy1; ++cy)
-        for (uint32_t cx = cx0; cx <= cx1; ++cx) far = max(far, col[cy * a.lm_columns_x + cx]);
+      if (part != 0) {
+        far = col[(cy0 * a.lm_columns_x + cx0) * kLmParts + part];
+      } else {
+        for (uint32_t cy = cy0; cy <= cy1; ++cy)
+          for (uint32_t cx = cx0; cx <= cx1; ++cx) far = max(far, col[(cy * a.lm_columns_x + cx) * kLmParts]);
+      }
       if (to_pin > far) b = max(b, static_cast<uint64_t>(to_pin - far));
     }
     best = min(best, b);
@@ -265,10 +292,11 @@ __device__ __forceinline__ uint32_t GoalBound(const SearchArgs& a, const NetStat
 // key is also a valid lower bound; weighting it keeps the tiles along a route
 // in distance order where the full ALT bound would put them all in one bucket.
 __device__ __forceinline__ uint2 TileKeys(const SearchArgs& a, const NetStatic& ns, uint32_t connected,
-                                          uint32_t tx, uint32_t ty, uint32_t tw, uint32_t th, uint32_t dist) {
+                                          uint32_t tx, uint32_t ty, uint32_t tw, uint32_t th, uint32_t dist,
+                                          int side) {
   const uint32_t x0 = ns.x0 + tx * tw, y0 = ns.y0 + ty * th;
   const uint32_t x1 = min(x0 + tw, ns.x0 + ns.w) - 1, y1 = min(y0 + th, ns.y0 + ns.h) - 1;
-  const uint64_t b = GoalBound(a, ns, connected, x0, x1, y0, y1);
+  const uint64_t b = GoalBound(a, ns, connected, x0, x1, y0, y1, side);
   const uint64_t cap = kInfD - 1;
   const uint32_t f = static_cast<uint32_t>(min(static_cast<uint64_t>(dist) + b, cap));
   if (a.lm_count == 0) return make_uint2(f, f);
@@ -512,19 +540,19 @@ __device__ void ProcessTile(const SearchArgs& a, uint32_t r, uint32_t g, uint32_
     const uint32_t next = r + 1;
     const uint32_t conn = a.state[n].connected;
     if (s_flags[0] < kInfD && tx > 0) {
-      const uint2 k = TileKeys(a, ns, conn, tx - 1, ty, TX, TY, s_flags[0]);
+      const uint2 k = TileKeys(a, ns, conn, tx - 1, ty, TX, TY, s_flags[0], 1);
       Push(a, g - 1, n, next, k.x, k.y);
     }
     if (s_flags[1] < kInfD && tx + 1 < ns.tiles_x) {
-      const uint2 k = TileKeys(a, ns, conn, tx + 1, ty, TX, TY, s_flags[1]);
+      const uint2 k = TileKeys(a, ns, conn, tx + 1, ty, TX, TY, s_flags[1], 0);
       Push(a, g + 1, n, next, k.x, k.y);
     }
     if (s_flags[2] < kInfD && ty > 0) {
-      const uint2 k = TileKeys(a, ns, conn, tx, ty - 1, TX, TY, s_flags[2]);
+      const uint2 k = TileKeys(a, ns, conn, tx, ty - 1, TX, TY, s_flags[2], 3);
       Push(a, g - ns.tiles_x, n, next, k.x, k.y);
     }
     if (s_flags[3] < kInfD && ty + 1 < ns.tiles_y) {
-      const uint2 k = TileKeys(a, ns, conn, tx, ty + 1, TX, TY, s_flags[3]);
+      const uint2 k = TileKeys(a, ns, conn, tx, ty + 1, TX, TY, s_flags[3], 2);
       Push(a, g + ns.tiles_x, n, next, k.x, k.y);
     }
     if (!converged) Push(a, g, n, next, 0, 0);
@@ -928,16 +956,17 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
       if (k < 5) {
         bool exists;
         uint32_t ttx = tx, tty = ty;
+        int side = -1;  // the target's edge a change from this tile enters through
         switch (k) {
-          case 0: exists = tx > 0; target = g - 1; ttx = tx - 1; break;
-          case 1: exists = tx + 1 < ns.tiles_x; target = g + 1; ttx = tx + 1; break;
-          case 2: exists = ty > 0; target = g - ns.tiles_x; tty = ty - 1; break;
-          case 3: exists = ty + 1 < ns.tiles_y; target = g + ns.tiles_x; tty = ty + 1; break;
+          case 0: exists = tx > 0; target = g - 1; ttx = tx - 1; side = 1; break;
+          case 1: exists = tx + 1 < ns.tiles_x; target = g + 1; ttx = tx + 1; side = 0; break;
+          case 2: exists = ty > 0; target = g - ns.tiles_x; tty = ty - 1; side = 3; break;
+          case 3: exists = ty + 1 < ns.tiles_y; target = g + ns.tiles_x; tty = ty + 1; side = 2; break;
           default: exists = !converged; target = g; break;
         }
         want = s_side[k] < kInfD && exists;
         if (want) {
-          const uint2 kk = TileKeys(a, ns, a.state[n].connected, ttx, tty, kRT, kRT, s_side[k]);
+          const uint2 kk = TileKeys(a, ns, a.state[n].connected, ttx, tty, kRT, kRT, s_side[k], side);
           key = kk.x;
           fkey = kk.y;
         }
@@ -1283,20 +1312,21 @@ __global__ void LandmarkPinKernel(const uint32_t* __restrict__ pins, uint64_t pi
 // lattice, so label index = cell index) started at landmark k.  One workgroup
 // per (64 x 64 column, landmark): writes d(landmark, cell) for the column's
 // cells (kInfD where the cell is blocked, unreached or its tile never
-// initialised) and the largest finite distance (0 if none) to colmax.
+// initialised) and the largest finite distance (0 if none) over the whole
+// column and over each of its four edge lines to colmax.
 __global__ __launch_bounds__(256) void LandmarkExtractKernel(const uint32_t* __restrict__ labels,
                                                              const uint32_t* __restrict__ tile_init,
                                                              const NetStatic* __restrict__ nets, uint32_t W,
                                                              uint32_t H, uint32_t L, uint32_t tx, uint32_t ty,
                                                              uint32_t columns_x, uint32_t* __restrict__ dist,
                                                              uint32_t* __restrict__ colmax) {
-  __shared__ uint32_t s_max[4];
+  __shared__ uint32_t s_max[kLmParts][4];
   const uint32_t k = blockIdx.y, column = blockIdx.x;
   const NetStatic ns = nets[k];
   const uint32_t x0 = (column % columns_x) * kLmColumn, y0 = (column / columns_x) * kLmColumn;
   const uint32_t nx = min(kLmColumn, W - x0), ny = min(kLmColumn, H - y0);
   const uint64_t cells = static_cast<uint64_t>(W) * H * L;
-  uint32_t m = 0;
+  uint32_t m[kLmParts] = {0, 0, 0, 0, 0};
   for (uint32_t i = threadIdx.x; i < kLmColumn * kLmColumn * L; i += blockDim.x) {
     const uint32_t x = i % kLmColumn, y = (i / kLmColumn) % kLmColumn, l = i / (kLmColumn * kLmColumn);
     if (x >= nx || y >= ny) continue;
@@ -1306,15 +1336,25 @@ __global__ __launch_bounds__(256) void LandmarkExtractKernel(const uint32_t* __r
     uint32_t d = kInfD;
     if (tile_init[g] != kNever) d = min(labels[ns.label_offset + c] >> 3, kInfD);
     dist[k * cells + c] = d;
-    if (d < kInfD) m = max(m, d);
+    if (d < kInfD) {
+      m[0] = max(m[0], d);
+      if (x == 0) m[1] = max(m[1], d);
+      if (x == nx - 1) m[2] = max(m[2], d);
+      if (y == 0) m[3] = max(m[3], d);
+      if (y == ny - 1) m[4] = max(m[4], d);
+    }
   }
-  for (int off = 32; off > 0; off >>= 1) m = max(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), off, 64)));
-  if ((threadIdx.x & 63u) == 0) s_max[threadIdx.x >> 6] = m;
+#pragma unroll
+  for (uint32_t part = 0; part < kLmParts; ++part) {
+    uint32_t v = m[part];
+    for (int off = 32; off > 0; off >>= 1) v = max(v, static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), off, 64)));
+    if ((threadIdx.x & 63u) == 0) s_max[part][threadIdx.x >> 6] = v;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x < kLmParts) {
     uint32_t v = 0;
-    for (uint32_t w = 0; w < blockDim.x / 64; ++w) v = max(v, s_max[w]);
-    colmax[static_cast<uint64_t>(k) * gridDim.x + column] = v;
+    for (uint32_t w = 0; w < blockDim.x / 64; ++w) v = max(v, s_max[threadIdx.x][w]);
+    colmax[(static_cast<uint64_t>(k) * gridDim.x + column) * kLmParts + threadIdx.x] = v;
   }
 }
 
@@ -2233,7 +2273,7 @@ extern "C" int apgx_lattice_build_landmarks(apgx_lattice* lat, uint32_t count, d
   uint32_t* dist = nullptr;
   uint32_t* colmax = nullptr;
   hipError_t e = hipMalloc(&dist, 4 * cells * n);
-  if (e == hipSuccess) e = hipMalloc(&colmax, 4ull * columns * n);
+  if (e == hipSuccess) e = hipMalloc(&colmax, 4ull * columns * n * kLmParts);
   if (e != hipSuccess) {
     if (dist) (void)hipFree(dist);
     apgx_search_release(s);
