@@ -60,12 +60,14 @@ constexpr uint32_t kDefaultSweepSchedule = 2;
 // cell costs give 12.0k queries/s against 11.3k for (64, 64 mean costs).
 constexpr uint32_t kRegDefaultSweeps = 10;
 constexpr double kDefaultDeltaCells = 48.0;  // apgx_route_options.frontier_delta 0
-// With landmarks (four_layer_1024, 1024 pairs, 8 fields): tiles ordered by
-// distance + 15/16 of the bound with 32 sweeps per register-tile visit give
-// 19.5k queries/s against 16.0k for 12/16 with 10 sweeps and 13.1k without
-// landmarks; the full bound (16/16) puts a route's tiles in one bucket (19.0k).
-constexpr uint32_t kDefaultLandmarkWeight = 15;
-constexpr uint32_t kRegLandmarkSweeps = 32;
+// With landmarks (four_layer_1024, 1024 pairs, 16 fields, neighbour keys
+// bounded at their entry edge): tiles ordered by distance + the whole bound,
+// register-tile visits up to 64 sweeps: 26.6k queries/s against 23.9k for
+// 15/16 of the bound and 32 sweeps, 14.0k without landmarks.  (With the
+// coarser whole-tile bound, 15/16 was best: the whole bound put a route's
+// tiles in one bucket.)
+constexpr uint32_t kDefaultLandmarkWeight = 16;
+constexpr uint32_t kRegLandmarkSweeps = 64;
 constexpr int kMaxTileCells = 16384;
 constexpr int kKeep = kMaxTileCells / kThreads;  // labels kept in registers per thread
 
@@ -2126,7 +2128,7 @@ extern "C" int apgx_autotune(apgx_lattice* lat, const apgx_net_query* nets, uint
       if (t.tx == 64 && t.ty == 64 && reg) {
         cands.push_back(apgx_tune_candidate{64, 64, dl, 16, 0.0});
         cands.push_back(apgx_tune_candidate{64, 64, dl, kRegDefaultSweeps, 0.0});
-        if (opt.landmarks) cands.push_back(apgx_tune_candidate{64, 64, dl, kRegLandmarkSweeps, 0.0});
+        if (opt.landmarks) cands.push_back(apgx_tune_candidate{64, 64, dl, 32, 0.0});
       }
     }
   }

@@ -234,7 +234,7 @@ typedef struct apgx_route_options {
   uint32_t frontier_delta;    /* distance bucket width of tile scheduling; 0: 48 x mean cell
                                  cost; 0xFFFFFFFF: unordered (every queued tile every round) */
   uint32_t max_sweeps;        /* sweeps per tile visit before re-queueing; 0: 10 on register
-                                 tiles (32 with landmarks), 64 on LDS tiles */
+                                 tiles (64 with landmarks), 64 on LDS tiles */
   uint32_t no_goal_bound;     /* 1: order tiles by distance only; 0: distance plus the
                                  consistent lower bound to the nearest unconnected pin */
   uint32_t sweep_schedule;    /* register tiles: k = one full sweep (via exchange and
@@ -246,7 +246,7 @@ typedef struct apgx_route_options {
                                  with no_goal_bound.
                                  Distances and paths are the same either way. */
   uint32_t landmark_weight;   /* with landmarks: tiles are ordered by distance plus
-                                 landmark_weight / 16 of the bound (1 .. 16; 0: 15), pins
+                                 landmark_weight / 16 of the bound (1 .. 16; 0: 16), pins
                                  settle on distance plus the whole bound */
   uint32_t reserved;
 } apgx_route_options;
