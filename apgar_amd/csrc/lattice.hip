@@ -260,6 +260,22 @@ __device__ __forceinline__ uint32_t GoalBound(const SearchArgs& a, const NetStat
   // Entry cells for the Manhattan part.
   const uint32_t ex0 = side == 1 ? x1 : x0, ex1 = side == 0 ? x0 : x1;
   const uint32_t ey0 = side == 3 ? y1 : y0, ey1 = side == 2 ? y0 : y1;
+  // Per landmark, the largest distance over the entry cells: loaded once, then
+  // shared by every pin (kept in registers: the loop bound is a constant).
+  uint32_t far[APGX_MAX_LANDMARKS];
+#pragma unroll
+  for (uint32_t k = 0; k < APGX_MAX_LANDMARKS; ++k) {
+    far[k] = 0;
+    if (k < lm_count) {
+      const uint32_t* col = a.lm_colmax + static_cast<uint64_t>(k) * a.lm_columns * kLmParts;
+      if (part != 0) {
+        far[k] = col[(cy0 * a.lm_columns_x + cx0) * kLmParts + part];
+      } else {
+        for (uint32_t cy = cy0; cy <= cy1; ++cy)
+          for (uint32_t cx = cx0; cx <= cx1; ++cx) far[k] = max(far[k], col[(cy * a.lm_columns_x + cx) * kLmParts]);
+      }
+    }
+  }
   uint64_t best = ~0ull;
   for (uint32_t p = 0; p < ns.pin_count; ++p) {
     if (connected & (1u << p)) continue;
@@ -269,18 +285,11 @@ __device__ __forceinline__ uint32_t GoalBound(const SearchArgs& a, const NetStat
     const uint32_t dy = y < ey0 ? ey0 - y : (y > ey1 ? y - ey1 : 0u);
     uint64_t b = static_cast<uint64_t>(dx + dy) * a.goal_cost;
     const uint32_t* dp = a.lm_pin + static_cast<uint64_t>(ns.pin_offset + p) * a.lm_count;
-    for (uint32_t k = 0; k < lm_count; ++k) {
+#pragma unroll
+    for (uint32_t k = 0; k < APGX_MAX_LANDMARKS; ++k) {
+      if (k >= lm_count) break;
       const uint32_t to_pin = dp[k];
-      if (to_pin >= kInfD) continue;
-      const uint32_t* col = a.lm_colmax + static_cast<uint64_t>(k) * a.lm_columns * kLmParts;
-      uint32_t far = 0;
-      if (part != 0) {
-        far = col[(cy0 * a.lm_columns_x + cx0) * kLmParts + part];
-      } else {
-        for (uint32_t cy = cy0; cy <= cy1; ++cy)
-          for (uint32_t cx = cx0; cx <= cx1; ++cx) far = max(far, col[(cy * a.lm_columns_x + cx) * kLmParts]);
-      }
-      if (to_pin > far) b = max(b, static_cast<uint64_t>(to_pin - far));
+      if (to_pin < kInfD && to_pin > far[k]) b = max(b, static_cast<uint64_t>(to_pin - far[k]));
     }
     best = min(best, b);
   }
