@@ -80,7 +80,8 @@ class RouteStats(C.Structure):
                 ("workspace_bytes", C.c_uint64), ("rounds", C.c_uint32), ("phases", C.c_uint32),
                 ("tile_w", C.c_uint32), ("tile_h", C.c_uint32), ("host_syncs", C.c_uint32),
                 ("graph_launches", C.c_uint32), ("round_kernel_ms", C.c_double),
-                ("tiles_touched", C.c_uint64)]
+                ("tiles_touched", C.c_uint64), ("layer_sweeps", C.c_uint64), ("full_sweeps", C.c_uint64),
+                ("changing_full_sweeps", C.c_uint64), ("free_cell_visits", C.c_uint64)]
 
 
 class TuneCandidate(C.Structure):

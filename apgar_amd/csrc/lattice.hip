@@ -355,6 +355,30 @@ __device__ __forceinline__ uint32_t Step(uint32_t carry_d, uint32_t cost, uint32
   return (min(carry_d + cost, kInfD) << 3) | code;
 }
 
+// Search statistics, slots of SearchArgs::stats (u64):
+//   0 tile visits, 1 window cells of the visited tiles, 2 sweep iterations,
+//   3 relaxations (edges relaxed into free cells by the sweeps executed:
+//     4 in-layer per free cell and sweep, plus one per adjacent layer on full
+//     sweeps), 4 distinct tiles visited, 5 layer sweeps executed (summed over
+//     the tile's layers; register tiles skip sweeps of quiet layers),
+//   6 full sweeps (via exchange and stability test), 7 cell-buffer overflow
+//   flag, 8 full sweeps whose test found a change, 9 free cells of the visited
+//   tiles.
+constexpr int kStatSlots = 16;
+__device__ __forceinline__ void AddVisitStats(const SearchArgs& a, uint64_t cells, uint32_t slots,
+                                              uint64_t layer_sweeps, uint32_t full, uint32_t changing,
+                                              uint32_t free_cells, uint64_t relaxations, bool first) {
+  atomicAdd(&a.stats[0], 1ull);
+  atomicAdd(&a.stats[1], static_cast<unsigned long long>(cells));
+  atomicAdd(&a.stats[2], static_cast<unsigned long long>(slots));
+  atomicAdd(&a.stats[3], static_cast<unsigned long long>(relaxations));
+  if (first) atomicAdd(&a.stats[4], 1ull);
+  atomicAdd(&a.stats[5], static_cast<unsigned long long>(layer_sweeps));
+  atomicAdd(&a.stats[6], static_cast<unsigned long long>(full));
+  atomicAdd(&a.stats[8], static_cast<unsigned long long>(changing));
+  atomicAdd(&a.stats[9], static_cast<unsigned long long>(free_cells));
+}
+
 // One tile visit: stage, sweep to local convergence, write back, activate.
 template <int TX, int TY>
 __device__ void ProcessTile(const SearchArgs& a, uint32_t r, uint32_t g, uint32_t* s_lab, uint32_t* s_cost,
@@ -384,6 +408,8 @@ __device__ void ProcessTile(const SearchArgs& a, uint32_t r, uint32_t g, uint32_
 
   const int cells = L * TY * TX;
   uint32_t orig[kKeep];
+  uint32_t free_cells = 0, edges = 0;  // this thread's free cells and the edges a sweep relaxes into them
+  const bool vias = L > 1 && a.via != 0;
 #pragma unroll
   for (int k = 0; k < kKeep; ++k) {
     const int i = tid + k * kThreads;
@@ -396,6 +422,10 @@ __device__ void ProcessTile(const SearchArgs& a, uint32_t r, uint32_t g, uint32_
             (static_cast<uint64_t>(l) * a.H + ns.y0 + by + y) * a.W + ns.x0 + bx + x;
         c = cost_bytes == 2 ? LoadCost<uint16_t>(a.cost, gidx) : LoadCost<uint32_t>(a.cost, gidx);
         if (initd) v = lab[(l * h + by + y) * w + bx + x];
+        if (c != 0) {
+          ++free_cells;
+          edges += 4u + (vias ? (l > 0 ? 1u : 0u) + (l + 1 < L ? 1u : 0u) : 0u);
+        }
       }
       s_cost[(l * TY + y) * CPX + x] = c ? c : kInfD;
       s_lab[(l * (TY + 2) + y + 1) * PX + x + 1] = v;
@@ -545,6 +575,10 @@ __device__ void ProcessTile(const SearchArgs& a, uint32_t r, uint32_t g, uint32_
       }
     }
   }
+  if (free_cells) {
+    atomicAdd(&s_flags[4], free_cells);
+    atomicAdd(&s_flags[5], edges);
+  }
   __syncthreads();
   if (tid == 0) {
     if (!initd) a.tile_init[g] = r;
@@ -568,11 +602,10 @@ __device__ void ProcessTile(const SearchArgs& a, uint32_t r, uint32_t g, uint32_
     }
     if (!converged) Push(a, g, n, next, 0, 0);
     if (atomicMax(&a.state[n].last_counted, r) < r) atomicAdd(&a.state[n].rounds, 1u);
-    atomicAdd(&a.stats[0], 1ull);
-    atomicAdd(&a.stats[1], static_cast<unsigned long long>(nx) * ny * L);
-    atomicAdd(&a.stats[2], static_cast<unsigned long long>(sweeps));
-    atomicAdd(&a.stats[3], static_cast<unsigned long long>(sweeps) * nx * ny * L);
-    if (!initd) atomicAdd(&a.stats[4], 1ull);
+    // Every sweep of an LDS tile is full: all layers, vias, one stability test.
+    AddVisitStats(a, static_cast<uint64_t>(nx) * ny * L, sweeps, static_cast<uint64_t>(sweeps) * L,
+                  sweeps, converged ? sweeps - 1 : sweeps, s_flags[4],
+                  static_cast<uint64_t>(sweeps) * s_flags[5], !initd);
   }
 }
 
@@ -583,13 +616,15 @@ __global__ __launch_bounds__(kThreads) void RoundKernel(SearchArgs a, uint32_t c
   uint32_t* s_lab = smem;
   uint32_t* s_cost = smem + static_cast<int>(a.L) * (TY + 2) * PX;
   __shared__ uint32_t s_item;
-  __shared__ uint32_t s_flags[4];
+  __shared__ uint32_t s_flags[6];  // smallest changed border distance per side; free cells, sweep edges
   const uint32_t r = __builtin_amdgcn_readfirstlane(*a.round_ptr);
   const uint32_t count = a.counters[2 * r];
   for (;;) {
     if (threadIdx.x == 0) {
       s_item = PopAdmitted(a, r, count);
       for (int k = 0; k < 4; ++k) s_flags[k] = kInfD;
+      s_flags[4] = 0;
+      s_flags[5] = 0;
     }
     __syncthreads();
     const uint32_t g = s_item;
@@ -660,6 +695,7 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
   __shared__ uint32_t s_item[2];  // current / prefetched item (parity of the iteration)
   __shared__ uint32_t s_side[5];   // smallest changed border distance per side (W, E, S, N), tile min
   __shared__ uint32_t s_changed[2 * L];  // stability votes, double-buffered by vote parity
+  __shared__ uint32_t s_wstat[3 * L];    // per layer: free cells, sweeps executed, relaxations
   const int l = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));  // wave-uniform layer
   const uint32_t r = __builtin_amdgcn_readfirstlane(*a.round_ptr);
   const uint32_t count = a.counters[2 * r];
@@ -787,7 +823,7 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
     const uint32_t* below = s_t + (l > 0 ? l - 1 : l) * kRT * kRP;
     const uint32_t* above = s_t + (l + 1 < L ? l + 1 : l) * kRT * kRP;
 
-    uint32_t sweeps = 0, votes = 0;
+    uint32_t sweeps = 0, votes = 0, executed = 0, changing = 0;
     bool converged = false;
     // A quiet layer: unchanged since the last vote, as are the layers next to
     // it.  Its labels are then a fixed point of the in-layer passes (halos are
@@ -849,6 +885,7 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
       // full sweep and so covers this sweep's changes too.
       if (sweeps % period != 0) {
         ++sweeps;
+        ++executed;
         continue;
       }
       if (a.via != 0) {  // the host passes via = 0 for single-layer lattices
@@ -907,10 +944,12 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
         ++votes;
       }
       ++sweeps;
+      ++executed;
       if (!any) {
         converged = true;
         break;
       }
+      ++changing;
       if (quiet) sweeps += period - 1;  // skip the in-layer-only sweeps up to the next full one
     }
 
@@ -919,6 +958,16 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
 #pragma unroll
     for (int y = 0; y < kRT; ++y) T[y * kRP + lane] = lab[y];
     WaveSync();
+    // Free cells of this layer of the tile, for the statistics (blocked and
+    // out-of-window cells hold the blocked cost half).  Counted here, after
+    // the sweeps: counting at staging time costs the sweep loop registers.
+    uint32_t free_l = 0;
+#pragma unroll
+    for (int i = 0; i < kRT / 2; ++i)
+      free_l += ((cst[i] & 0xFFFFu) != kBlockedHalf ? 1u : 0u) + ((cst[i] >> 16) != kBlockedHalf ? 1u : 0u);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) free_l += __shfl_xor(free_l, off, 64);
+    free_l = __builtin_amdgcn_readfirstlane(free_l);
     uint32_t col_min = kInf;
 #pragma unroll
     for (int y = 0; y < kRT; ++y) col_min = min(col_min, lab[y]);
@@ -954,6 +1003,12 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
 #pragma unroll 8
       for (int y = 0; y < ny; ++y) lbase[y * w + lane] = T[y * kRP + lane];
     }
+    if (lane == 0) {
+      const uint32_t nvia = a.via != 0 ? (l > 0 ? 1u : 0u) + (l + 1 < L ? 1u : 0u) : 0u;
+      s_wstat[l] = free_l;
+      s_wstat[L + l] = executed;
+      s_wstat[2 * L + l] = free_l * (4u * executed + nvia * votes);
+    }
     __syncthreads();
     // Epilogue: lanes 0..4 of the first wave issue the four neighbour pushes
     // and the self push together; lane 5 counts the net's round, lane 6 the
@@ -986,11 +1041,16 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
       if (k == 5) {
         if (atomicMax(&a.state[n].last_counted, r) < r) atomicAdd(&a.state[n].rounds, 1u);
       } else if (k == 6) {
-        atomicAdd(&a.stats[0], 1ull);
-        atomicAdd(&a.stats[1], static_cast<unsigned long long>(nx) * ny * L);
-        atomicAdd(&a.stats[2], static_cast<unsigned long long>(sweeps));
-        atomicAdd(&a.stats[3], static_cast<unsigned long long>(sweeps) * nx * ny * L);
-        if (!initd) atomicAdd(&a.stats[4], 1ull);
+        uint32_t free_cells = 0, layer_sweeps = 0;
+        uint64_t relax = 0;
+#pragma unroll
+        for (int q = 0; q < L; ++q) {
+          free_cells += s_wstat[q];
+          layer_sweeps += s_wstat[L + q];
+          relax += s_wstat[2 * L + q];
+        }
+        AddVisitStats(a, static_cast<uint64_t>(nx) * ny * L, min(sweeps, a.max_sweeps), layer_sweeps, votes,
+                      changing, free_cells, relax, !initd);
       }
     }
     if (popper) s_item[(iter + 1) & 1] = PopAdmitted(a, r, count);
@@ -1308,6 +1368,59 @@ __global__ void FinalWalkKernel(SearchArgs a) {
   WalkConnection<TX, TY>(a, 0u, ns, st, c, distance, a.pins[ns.pin_offset + pin], false, n);
 }
 
+// Label-range check of a finished search.  Distances saturate at kInfD, which
+// reads as unreached, so a true distance of 2^29 - 1 or more would surface as
+// an unreached cell.  A reached pin's distance is always exact (every label
+// on its shortest path is no larger than its own), so the one result that can
+// be wrong is "disconnected".  When a net ends disconnected its queue is
+// empty and its labels are the fixed point of the saturated relaxation over
+// the window; there a free cell is left unreached next to a reached free
+// neighbour exactly when that edge's relaxation saturated.  One block row per
+// listed net (grid.y), grid-stride over the window's cells; flags[i] gets 1 if
+// net list[i] saturated, 2 if a relaxable edge is left (not a fixed point).
+__device__ __forceinline__ uint32_t LabelDistanceAt(const SearchArgs& a, const NetStatic& ns, uint32_t tw,
+                                                    uint32_t th, uint32_t x, uint32_t y, uint32_t l) {
+  const uint32_t g = ns.tile_offset + (y / th) * ns.tiles_x + x / tw;
+  if (a.tile_init[g] == kNever) return kInfD;
+  return min(a.labels[ns.label_offset + (static_cast<uint64_t>(l) * ns.h + y) * ns.w + x] >> 3, kInfD);
+}
+
+__global__ __launch_bounds__(256) void SaturationCheckKernel(SearchArgs a, uint32_t cost_bytes, uint32_t tw,
+                                                             uint32_t th, const uint32_t* __restrict__ list,
+                                                             uint32_t* __restrict__ flags) {
+  const NetStatic ns = a.nets[list[blockIdx.y]];
+  const uint64_t cells = static_cast<uint64_t>(ns.w) * ns.h * a.L;
+  auto cost_at = [&](uint32_t x, uint32_t y, uint32_t l) -> uint32_t {
+    const uint64_t gi = (static_cast<uint64_t>(l) * a.H + ns.y0 + y) * a.W + ns.x0 + x;
+    return cost_bytes == 2 ? LoadCost<uint16_t>(a.cost, gi) : LoadCost<uint32_t>(a.cost, gi);
+  };
+  uint32_t found = 0;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < cells;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t x = static_cast<uint32_t>(i % ns.w), y = static_cast<uint32_t>((i / ns.w) % ns.h);
+    const uint32_t l = static_cast<uint32_t>(i / (static_cast<uint64_t>(ns.w) * ns.h));
+    const uint32_t c = cost_at(x, y, l);
+    if (c == 0 || LabelDistanceAt(a, ns, tw, th, x, y, l) < kInfD) continue;
+    // Every free in-window neighbour u with a finite label relaxes into this cell.
+    auto probe = [&](uint32_t ux, uint32_t uy, uint32_t ul, uint32_t weight) {
+      if (cost_at(ux, uy, ul) == 0) return;
+      const uint32_t du = LabelDistanceAt(a, ns, tw, th, ux, uy, ul);
+      if (du >= kInfD) return;
+      found |= static_cast<uint64_t>(du) + weight >= kInfD ? 1u : 2u;
+    };
+    if (x > 0) probe(x - 1, y, l, c);
+    if (x + 1 < ns.w) probe(x + 1, y, l, c);
+    if (y > 0) probe(x, y - 1, l, c);
+    if (y + 1 < ns.h) probe(x, y + 1, l, c);
+    if (a.via != 0) {
+      const uint32_t vw = a.via + (a.via_pays_cell ? c : 0u);
+      if (l > 0) probe(x, y, l - 1, vw);
+      if (l + 1 < a.L) probe(x, y, l + 1, vw);
+    }
+  }
+  if (found) atomicOr(&flags[blockIdx.y], found);
+}
+
 // Per pin slot and landmark: d(landmark k, pin) from the landmark fields
 // (kInfD for a slot whose cell lies outside the lattice: its net is invalid).
 __global__ void LandmarkPinKernel(const uint32_t* __restrict__ pins, uint64_t pin_count,
@@ -1600,6 +1713,63 @@ int Launch(const void* kernel, dim3 grid, dim3 block, size_t lds, hipStream_t st
 }  // namespace
 }  // namespace apgx
 
+namespace apgx {
+namespace {
+
+// Runs SaturationCheckKernel over every at-risk net that ended disconnected.
+// A saturated relaxation makes the verdict unrepresentable here (the distance
+// is 2^29 - 1 or more): the search fails with APGX_E_UNSUPPORTED rather than
+// report a reachable pin as disconnected.  (The reference's u64 labels reject
+// such sums with checked adds, gpu/cuda/planar_router.cu:332-340.)
+int CheckLabelRange(const SearchArgs& a, uint32_t cost_bytes, TileShape shape, const std::vector<apgx_net_query>& nets,
+                    const std::vector<NetState>& state, const std::vector<uint8_t>& at_risk, hipStream_t stream) {
+  std::vector<uint32_t> list;
+  uint64_t largest = 0;
+  for (uint32_t i = 0; i < state.size(); ++i) {
+    if (!at_risk[i] || state[i].status != APGX_NET_DISCONNECTED) continue;
+    list.push_back(i);
+  }
+  if (list.empty()) return APGX_OK;
+  for (uint32_t i : list) {
+    apgx_window w = nets[i].window;
+    if (w.width == 0 && w.height == 0) w = apgx_window{0, 0, a.W, a.H};
+    largest = std::max<uint64_t>(largest, static_cast<uint64_t>(w.width) * w.height * a.L);
+  }
+  uint32_t* d_list = nullptr;
+  uint32_t* d_flags = nullptr;
+  const size_t n = list.size();
+  APGX_HIP(hipMalloc(&d_list, 8 * n), "label range check allocation");
+  d_flags = d_list + n;
+  std::vector<uint32_t> flags(n, 0);
+  hipError_t e = hipMemcpyAsync(d_list, list.data(), 4 * n, hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) e = hipMemsetAsync(d_flags, 0, 4 * n, stream);
+  const uint32_t bx = static_cast<uint32_t>(std::min<uint64_t>((largest + 255) / 256, 2048));
+  for (size_t off = 0; e == hipSuccess && off < n; off += 65535) {
+    const uint32_t rows = static_cast<uint32_t>(std::min<size_t>(n - off, 65535));
+    hipLaunchKernelGGL(SaturationCheckKernel, dim3(bx, rows), dim3(256), 0, stream, a, cost_bytes,
+                       static_cast<uint32_t>(shape.tx), static_cast<uint32_t>(shape.ty), d_list + off,
+                       d_flags + off);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(flags.data(), d_flags, 4 * n, hipMemcpyDeviceToHost, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  (void)hipFree(d_list);
+  if (e != hipSuccess) return HipFail(e, "label range check");
+  for (size_t k = 0; k < n; ++k) {
+    const std::string id = std::to_string(nets[list[k]].net_id);
+    if (flags[k] & 1u)
+      return Fail(APGX_E_UNSUPPORTED, "apgx_route_nets: net " + id +
+                                          ": a distance reaches the 29-bit label range (2^29 - 1)");
+    if (flags[k] & 2u)
+      return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_route_nets: net " + id +
+                                                 ": disconnected labels are not a fixed point");
+  }
+  return APGX_OK;
+}
+
+}  // namespace
+}  // namespace apgx
+
 extern "C" int apgx_route_nets(apgx_lattice* lat, const apgx_net_query* nets, uint32_t net_count,
                                const uint32_t* pins, uint64_t pin_count,
                                const apgx_route_options* options, apgx_search** out) {
@@ -1656,6 +1826,14 @@ int apgx::RouteNets(apgx_lattice* lat, const CostOverride* ov, const apgx_net_qu
   std::vector<uint32_t> tile_net;
   uint64_t label_words = 0, tiles = 0, conns = 0, window_cells_total = 0;
   const uint64_t plane = static_cast<uint64_t>(d.width) * d.height;
+  // Largest edge weight: a simple path through a window of c cells costs at
+  // most (c - 1) times it.  Nets whose bound reaches kInfD can saturate a
+  // label, and a "disconnected" verdict of theirs is checked after the search.
+  const uint64_t via_weight = (d.layers > 1 && d.via_cost)
+                                  ? d.via_cost + ((ov && ov->via_pays_cell) ? static_cast<uint64_t>(max_cost) : 0u)
+                                  : 0u;
+  const uint64_t max_weight = std::max<uint64_t>(max_cost, via_weight);
+  std::vector<uint8_t> at_risk(net_count, 0);
   for (uint32_t i = 0; i < net_count; ++i) {
     const apgx_net_query& q = nets[i];
     NetStatic& n = ns[i];
@@ -1693,6 +1871,9 @@ int apgx::RouteNets(apgx_lattice* lat, const CostOverride* ov, const apgx_net_qu
     tiles += nt;
     label_words += static_cast<uint64_t>(n.w) * n.h * d.layers;
     window_cells_total += static_cast<uint64_t>(n.w) * n.h * d.layers * (q.pin_count - 1);
+    const unsigned __int128 path_bound =
+        static_cast<unsigned __int128>(static_cast<uint64_t>(n.w) * n.h * d.layers - 1) * max_weight;
+    at_risk[i] = path_bound >= kInfD ? 1u : 0u;
     conns += q.pin_count - 1;
   }
   if (tiles >= 0xFFFFFFF0ull) {
@@ -1715,7 +1896,7 @@ int apgx::RouteNets(apgx_lattice* lat, const CostOverride* ov, const apgx_net_qu
   Part p_wl0{4 * std::max<uint64_t>(tiles, 1), 0};
   Part p_wl1{4 * std::max<uint64_t>(tiles, 1), 0};
   Part p_counters{4 * counter_words, 0};
-  Part p_stats{8 * 8, 0};
+  Part p_stats{8 * kStatSlots, 0};
   Part p_pins{4 * std::max<uint64_t>(pin_count, 1), 0};
   Part p_conns{sizeof(DevConnection) * std::max<uint64_t>(conns, 1), 0};
   Part p_cursor{8, 0};
@@ -1848,7 +2029,7 @@ int apgx::RouteNets(apgx_lattice* lat, const CostOverride* ov, const apgx_net_qu
   check(hipMemsetAsync(a.tile_stamp, 0, 4 * std::max<uint64_t>(tiles, 1), stream), "stamp clear");
   check(hipMemsetAsync(a.tile_init, 0xFF, 4 * std::max<uint64_t>(tiles, 1), stream), "init clear");
   check(hipMemsetAsync(a.counters, 0, 4 * counter_words, stream), "counter clear");
-  check(hipMemsetAsync(a.stats, 0, 64, stream), "stats clear");
+  check(hipMemsetAsync(a.stats, 0, 8 * kStatSlots, stream), "stats clear");
   check(hipMemsetAsync(a.cell_cursor, 0, 8, stream), "cursor clear");
   check(hipMemsetAsync(a.tile_key, 0xFF, 8 * std::max<uint64_t>(tiles, 1), stream), "key clear");
   check(hipMemsetAsync(a.net_minkey, 0xFF, 8 * std::max<uint32_t>(net_count, 1), stream), "key clear");
@@ -1957,10 +2138,25 @@ int apgx::RouteNets(apgx_lattice* lat, const CostOverride* ov, const apgx_net_qu
   s->conns.resize(conns);
   if (status == APGX_OK && conns)
     check(hipMemcpy(s->conns.data(), a.conns, sizeof(DevConnection) * conns, hipMemcpyDeviceToHost), "connection readback");
-  unsigned long long used = 0, stats[8] = {0};
+  if (status == APGX_OK) status = CheckLabelRange(a, cost_bytes, shape, s->nets, s->state, at_risk, stream);
+  // Net cost: the sum of the connection distances, checked in 64 bits (the
+  // device's running u32 sum is only a hint).
+  for (uint32_t i = 0; status == APGX_OK && i < net_count; ++i) {
+    NetState& x = s->state[i];
+    if (x.status != APGX_NET_REACHED) continue;
+    uint64_t sum = 0;
+    for (uint32_t c = 0; c < x.n_connected; ++c) sum += s->conns[ns[i].conn_offset + c].distance;
+    if (sum >= APGX_UNREACHABLE) {
+      status = Fail(APGX_E_UNSUPPORTED, "apgx_route_nets: net " + std::to_string(s->nets[i].net_id) +
+                                            ": the sum of its connection distances exceeds the uint32 net cost");
+      break;
+    }
+    x.cost = static_cast<uint32_t>(sum);
+  }
+  unsigned long long used = 0, stats[kStatSlots] = {0};
   if (status == APGX_OK) {
     check(hipMemcpy(&used, a.cell_cursor, 8, hipMemcpyDeviceToHost), "cursor readback");
-    check(hipMemcpy(stats, a.stats, 64, hipMemcpyDeviceToHost), "stats readback");
+    check(hipMemcpy(stats, a.stats, 8 * kStatSlots, hipMemcpyDeviceToHost), "stats readback");
   }
   if (status == APGX_OK) {
     used = std::min<unsigned long long>(used, cell_capacity);
@@ -1991,7 +2187,11 @@ int apgx::RouteNets(apgx_lattice* lat, const CostOverride* ov, const apgx_net_qu
   rs.cell_visits = stats[1];
   rs.inner_sweeps = stats[2];
   rs.tiles_touched = stats[4];
-  rs.relaxations = stats[3] * (d.layers > 1 && d.via_cost ? 6ull : 4ull);
+  rs.relaxations = stats[3];
+  rs.layer_sweeps = stats[5];
+  rs.full_sweeps = stats[6];
+  rs.changing_full_sweeps = stats[8];
+  rs.free_cell_visits = stats[9];
   rs.algorithmic_bytes = stats[1] * (static_cast<uint64_t>(cost_bytes) + 8ull);
   rs.workspace_bytes = total + s->label_bytes;
   rs.rounds = r - 1;

@@ -14,10 +14,17 @@ By default the searches add the ALT bound of 16 landmark distance fields
 upload, before the warmup; their build time and the rate with that build
 charged to a single step are reported in "landmarks".
 
---workload eight_layer_2048: an 8 x 2048 x 2048 lattice with the same model,
-4096 nets of 2-8 pins and log-uniform windows of side 32-2048.  Tile shape and
-schedule come from the autotuner on a 256-net sample (reported, not timed), and
-round chunks replay as captured hipGraphs.  The value is nets/s.
+eight_layer_2048: an 8 x 2048 x 2048 lattice with the same model, 4096 nets
+of 2-8 pins and log-uniform windows of side 32-2048.  Tile shape and schedule
+come from the autotuner on a 256-net sample (reported, not timed), and round
+chunks replay as captured hipGraphs.  Its rate is nets/s.  The default run
+times it after four_layer_1024 and reports it as the line's "eight_layer_2048"
+object (its own roofline and CPU baseline; --no-sub skips it);
+--workload eight_layer_2048 makes it the line's main value.
+
+The CPU baseline is the oracle's restatement of the reference CPU A* on the
+host's cores (at most 16 threads, the CPU share of one GPU) and on one thread,
+each on a bounded sample; the CPU model and core counts are reported with it.
 
 Prints one JSON line.  --gpus above 1 is rejected: this build runs on one GPU.
 """
@@ -66,6 +73,10 @@ def parse_args():
     p.add_argument("--graph", type=int, default=-1, help="1/0: replay round chunks as hipGraphs "
                    "(default: on for eight_layer_2048)")
     p.add_argument("--no-tune", action="store_true", help="eight_layer_2048: skip the autotuner")
+    p.add_argument("--no-sub", action="store_true",
+                   help="four_layer_1024: skip the eight_layer_2048 object timed in the same run")
+    p.add_argument("--sub-steps", type=int, default=3, help="timed steps of the eight_layer_2048 object")
+    p.add_argument("--sub-warmup", type=int, default=1, help="warmup steps of the eight_layer_2048 object")
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run a device-to-device copy of MIB MiB (HBM counter scale reference)")
     p.add_argument("--train-seeds", type=int, default=200, help="learned_prior: training boards")
@@ -84,59 +95,100 @@ def parse_args():
     return p.parse_args()
 
 
-def read_traffic(workload, landmarks, launches):
-    """HBM bytes per round-kernel launch from the committed rocprofv3 PMC summary
-    of the same configuration (None when there is none).  A summary may give
-    bytes per search instead, which are spread over this run's launches."""
-    f = ROOT / "profiles" / f"round_kernel_{workload}{'' if landmarks else '_no_landmarks'}_traffic.txt"
-    if not f.exists():
-        return None
-    for line in f.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-        if line.startswith("traffic_bytes_per_search:"):
-            return float(line.split(":", 1)[1]) / launches
-    return None
+def run_config(workload, opts, landmarks):
+    """The search configuration a profile summary must match to be quoted."""
+    return {"workload": workload, "landmarks": int(landmarks),
+            "landmark_weight": int(opts.get("landmark_weight", 0)),
+            "tile": "%dx%d" % tuple(opts.get("tile", (0, 0))),
+            "frontier_delta": int(opts.get("frontier_delta", 0)),
+            "max_sweeps": int(opts.get("max_sweeps", 0)),
+            "sweep_schedule": int(opts.get("sweep_schedule", 0)),
+            "goal_bound": bool(opts.get("goal_bound", True))}
 
 
-def read_issue(workload, landmarks):
-    """VALU and LDS issue utilisation of the round kernel from the committed
-    rocprofv3 SQ-counter summary of the same configuration (None when there is
-    none): the kernel's practical bound, beside the HBM roofline."""
-    f = ROOT / "profiles" / f"round_kernel_{workload}{'' if landmarks else '_no_landmarks'}_sq_counters.txt"
+def read_profile(kind, config):
+    """Figures of the committed rocprofv3 summary of `kind` ("traffic" or
+    "sq_counters") for the round kernel under exactly this configuration: the
+    summary's `config:` line must equal it.  Returns (fields, source) or
+    (None, reason)."""
+    f = ROOT / "profiles" / f"round_kernel_{config['workload']}_{kind}.txt"
     if not f.exists():
-        return None
-    out = {}
+        return None, f"no profiles/{f.name}"
+    fields, cfg = {}, None
     for line in f.read_text().splitlines():
-        for key in ("valu_busy", "lds_busy"):
-            if line.startswith(key + ":"):
-                out[key] = float(line.split(":", 1)[1])
-    return dict(out, source=f"profiles/{f.name}") if out else None
+        key, sep, val = line.partition(":")
+        if not sep:
+            continue
+        if key == "config":
+            cfg = json.loads(val)
+        elif key in ("traffic_bytes_per_launch", "traffic_bytes_per_search", "valu_busy", "lds_busy"):
+            fields[key] = float(val)
+    if cfg != config:
+        return None, f"profiles/{f.name} was measured under another configuration ({cfg})"
+    return fields, f"profiles/{f.name}"
+
+
+def host_cpu():
+    """CPU model and the cores this process may use (its affinity set)."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"model": model, "visible_cpus": os.cpu_count(), "usable_cpus": usable}
+
+
+HEURISTIC = ("oracle A*: key (f, h, g, y, x, layer, seq), heuristic (|dx|+|dy|) x smallest free cell cost "
+             "+ |dl| x via (the reference's geometric heuristic restated); the GPU search also uses "
+             "landmark (ALT) bounds, so the two do different amounts of work")
+
+
+def _timed_sample(fn, items, threads, budget_s):
+    """Runs fn over items on `threads` threads; with budget_s, stops starting
+    new items once that much time has passed (a bounded sample)."""
+    t0 = time.perf_counter()
+    done = 0
+    if threads == 1:
+        for it in items:
+            fn(it)
+            done += 1
+            if budget_s and time.perf_counter() - t0 >= budget_s:
+                break
+    else:
+        with ThreadPoolExecutor(max_workers=threads) as pool:
+            list(pool.map(fn, items))
+        done = len(items)
+    return done, time.perf_counter() - t0
 
 
 def cpu_pairs_baseline(cost, via, pairs, sample, threads):
-    """CPU A* oracle (restated reference algorithm) on a bounded sample of pairs."""
+    """CPU A* oracle (restated reference algorithm) on bounded samples of the
+    pairs: one A* per thread on `threads` threads, and one thread alone."""
     from oracle import py as oracle_py
 
     o = oracle_py.Dense(cost, via)
-    idx = list(range(min(sample, len(pairs))))
 
     def one(i):
         s, t = (int(v) for v in pairs[i])
         return o.astar(s, t)[0]
 
-    t0 = time.perf_counter()
-    with ThreadPoolExecutor(max_workers=threads) as pool:
-        list(pool.map(one, idx))
-    dt = time.perf_counter() - t0
-    return {"value": len(idx) / dt, "unit": "queries/s", "cores": threads, "kind": "port",
-            "sample": f"first {len(idx)} of the {len(pairs)} four_layer_1024 queries, "
-                      f"one A* per thread, {threads} threads, {dt:.1f} s"}
+    n, dt = _timed_sample(one, list(range(min(sample, len(pairs)))), threads, 0)
+    n1, dt1 = _timed_sample(one, list(range(len(pairs))), 1, 15.0)
+    return {"value": n / dt, "unit": "queries/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} of the {len(pairs)} four_layer_1024 queries, one A* per thread, "
+                      f"{threads} threads, {dt:.1f} s",
+            "single_thread": {"value": n1 / dt1, "cores": 1,
+                              "sample": f"first {n1} queries on one thread, {dt1:.1f} s"},
+            "heuristic": HEURISTIC, "host": host_cpu()}
 
 
 def cpu_nets_baseline(cost, via, nets, pins, sample, threads):
     """CPU oracle multi-pin routing (repeated Dijkstra from the tree) on an evenly
-    strided sample of nets, one net per thread."""
+    strided sample of nets, one net per thread, and on one thread alone."""
     from oracle import py as oracle_py
 
     idx = [i * len(nets) // sample for i in range(sample)]
@@ -146,13 +198,15 @@ def cpu_nets_baseline(cost, via, nets, pins, sample, threads):
         win = (n.window.x0, n.window.y0, n.window.width, n.window.height)
         return oracle_py.Dense(cost, via, win).route_net(pins[n.pin_offset:n.pin_offset + n.pin_count])
 
-    t0 = time.perf_counter()
-    with ThreadPoolExecutor(max_workers=threads) as pool:
-        list(pool.map(one, idx))
-    dt = time.perf_counter() - t0
-    return {"value": len(idx) / dt, "unit": "nets/s", "cores": threads, "kind": "port",
-            "sample": f"{len(idx)} of the {len(nets)} eight_layer_2048 nets (every "
-                      f"{len(nets) // sample}th), one net per thread, {threads} threads, {dt:.1f} s"}
+    n, dt = _timed_sample(one, idx, threads, 0)
+    n1, dt1 = _timed_sample(one, idx, 1, 15.0)
+    return {"value": n / dt, "unit": "nets/s", "cores": threads, "kind": "port",
+            "sample": f"{n} of the {len(nets)} eight_layer_2048 nets (every "
+                      f"{len(nets) // sample}th), one net per thread, {threads} threads, {dt:.1f} s",
+            "single_thread": {"value": n1 / dt1, "cores": 1,
+                              "sample": f"the first {n1} of those nets on one thread, {dt1:.1f} s"},
+            "heuristic": "oracle multi-source Dijkstra from the growing tree (no goal bound)",
+            "host": host_cpu()}
 
 
 def copy_calibration(mib):
@@ -284,20 +338,9 @@ def learned_prior(args):
     net.close()
 
 
-def main():
-    args = parse_args()
-    if args.copy_calibration:
-        copy_calibration(args.copy_calibration)
-        return
-    if args.gpus != 1 and args.workload != "learned_prior":
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); queries are "
-              "independent and would shard by query list, which this build does not launch.",
-              file=sys.stderr)
-        sys.exit(2)
-    if args.workload == "learned_prior":
-        learned_prior(args)
-        return
-    big = args.workload == "eight_layer_2048"
+def search_workload(args, workload, steps, warmup, cpu):
+    """Times one search workload and returns its JSON object."""
+    big = workload == "eight_layer_2048"
     spec = EIGHT_LAYER_2048 if big else FOUR_LAYER_1024
     cost = make_lattice(apgx, spec)
     lat = apgx.Lattice(cost, via_cost=spec["via"], cost_bytes=2)
@@ -310,7 +353,7 @@ def main():
         landmark_ms = lat.build_landmarks(args.landmarks)
         opts["landmarks"] = True
         opts["landmark_weight"] = args.landmark_weight
-        log(f"[bench] {args.landmarks} landmark fields built in {landmark_ms:.2f} ms (device)")
+        log(f"[bench] {workload}: {args.landmarks} landmark fields built in {landmark_ms:.2f} ms (device)")
     tune = None
     if big:
         count = args.queries or spec["nets"]
@@ -335,24 +378,24 @@ def main():
 
         def step(profile=False):
             return lat.route_pairs(pairs, profile_rounds=1 if profile else 0, **opts)
-    log(f"[bench] {args.workload}: lattice {cost.shape} uploaded; {count} {what}")
+    log(f"[bench] {workload}: lattice {cost.shape} uploaded; {count} {what}")
 
-    for i in range(args.warmup):
+    for i in range(warmup):
         _, st = step()
-        log(f"[bench] warmup {i}: device {st.device_ms:.2f} ms rounds {st.rounds}")
+        log(f"[bench] {workload} warmup {i}: device {st.device_ms:.2f} ms rounds {st.rounds}")
     apgx.synchronize()
     t0 = time.perf_counter()
     dev_ms, all_stats = [], []
     reached = 0
-    for i in range(args.steps):
+    for i in range(steps):
         routes, st = step()
         dev_ms.append(st.device_ms)
         all_stats.append(st)
         reached = sum(r.status == apgx.NET_REACHED for r in routes)
-        log(f"[bench] step {i}: device {st.device_ms:.2f} ms wall {st.wall_ms:.2f} ms "
+        log(f"[bench] {workload} step {i}: device {st.device_ms:.2f} ms wall {st.wall_ms:.2f} ms "
             f"rounds {st.rounds} tiles {st.tile_visits}")
     apgx.synchronize()
-    wall = (time.perf_counter() - t0) / args.steps
+    wall = (time.perf_counter() - t0) / steps
     ms = float(np.mean(dev_ms))
 
     # Roofline of the dominant kernel (the round kernel), each launch timed with
@@ -364,13 +407,21 @@ def main():
     achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
     st = all_stats[-1]
     L = cost.shape[0]
+    cfg = run_config(workload, opts, args.landmarks)
+    traffic, tsrc = read_profile("traffic", cfg)
+    traffic_bytes = None
+    if traffic:
+        traffic_bytes = traffic.get("traffic_bytes_per_launch")
+        if traffic_bytes is None and "traffic_bytes_per_search" in traffic:
+            traffic_bytes = traffic["traffic_bytes_per_search"] / launches
+    visits = max(st.tile_visits, 1)
     line = {
         "metric": "queries/s & % HBM roofline",
         "value": count / (ms / 1e3),
         "unit": unit,
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": ms,
         "higher_is_better": True,
         "scaling": "weak",
@@ -378,7 +429,7 @@ def main():
         "dtype": "u32",
         "data": "synthetic",
         "config": {
-            "workload": args.workload,
+            "workload": workload,
             "lattice": ("8x2048x2048" if big else "4x1024x1024")
                        + ", uint16 cost U[1,8], 15% keepouts (sides 4-64), vias 20",
             what: count,
@@ -392,7 +443,11 @@ def main():
         "rounds": st.rounds,
         "tile_visits": st.tile_visits,
         "tiles_touched": st.tiles_touched,
-        "inner_sweeps_per_visit": st.inner_sweeps / max(st.tile_visits, 1),
+        "sweep_iterations_per_visit": st.inner_sweeps / visits,
+        "sweeps_executed_per_visit": st.layer_sweeps / (visits * L),
+        "full_sweeps_per_visit": st.full_sweeps / visits,
+        "changing_full_sweep_fraction": st.changing_full_sweeps / max(st.full_sweeps, 1),
+        "free_cell_fraction": st.free_cell_visits / max(st.cell_visits, 1),
         "relaxations_per_s": st.relaxations / (ms / 1e3),
         "roofline": {
             "bound": "hbm",
@@ -400,7 +455,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": read_traffic(args.workload, args.landmarks, launches),
+            "traffic": traffic_bytes,
+            "traffic_source": tsrc,
             "kernel": (f"RegRoundKernel<{L}>" if (st.tile_w, st.tile_h) == (64, 64)
                        else f"RoundKernel<{st.tile_w},{st.tile_h}>"),
             "launches": launches,
@@ -408,9 +464,10 @@ def main():
             "algorithmic_bytes_per_launch": bytes_per_launch,
         },
     }
-    issue = read_issue(args.workload, args.landmarks)
+    issue, isrc = read_profile("sq_counters", cfg)
     if issue:
-        line["issue"] = issue
+        line["issue"] = {k: issue[k] for k in ("valu_busy", "lds_busy") if k in issue}
+        line["issue"]["source"] = isrc
     if landmark_ms is not None:
         # Per-lattice preparation (like the upload): built once, reused by every
         # batch on the lattice.  The second rate charges it to a single step.
@@ -419,16 +476,37 @@ def main():
         line["config"]["landmarks"] = args.landmarks
     if tune:
         line["autotune"] = tune
-    if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    if cpu:
+        threads = args.cpu_threads or min(16, host_cpu()["usable_cpus"])
         if big:
             line["cpu_baseline"] = cpu_nets_baseline(cost, spec["via"], nets, pins,
-                                                     args.cpu_sample or 64, threads)
+                                                     args.cpu_sample or 32, threads)
         else:
             line["cpu_baseline"] = cpu_pairs_baseline(cost, spec["via"], pairs,
                                                       args.cpu_sample or 512, threads)
-    print(json.dumps(line))
     lat.close()
+    return line
+
+
+def main():
+    args = parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    if args.gpus != 1 and args.workload != "learned_prior":
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); queries are "
+              "independent and would shard by query list, which this build does not launch.",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.workload == "learned_prior":
+        learned_prior(args)
+        return
+    line = search_workload(args, args.workload, args.steps, args.warmup, not args.no_cpu)
+    if args.workload == "four_layer_1024" and not args.no_sub:
+        # The batched uneven persistent launch, timed in the same run.
+        line["eight_layer_2048"] = search_workload(args, "eight_layer_2048", args.sub_steps, args.sub_warmup,
+                                                   not args.no_cpu)
+    print(json.dumps(line))
 
 
 if __name__ == "__main__":

@@ -257,8 +257,11 @@ typedef struct apgx_route_stats {
   uint64_t tile_visits;          /* tiles processed over all rounds */
   uint64_t cell_visits;          /* tile_visits * cells per tile (window clipped) */
   uint64_t algorithmic_bytes;    /* cell_visits * (cost bytes + 4 label read + 4 label write) */
-  uint64_t relaxations;          /* directed edges examined, summed over inner sweeps */
-  uint64_t inner_sweeps;         /* LDS sweeps summed over tile visits */
+  uint64_t relaxations;          /* directed edges relaxed into free cells by the sweeps executed:
+                                    4 in-layer per free cell and sweep, plus one per adjacent
+                                    layer on full sweeps */
+  uint64_t inner_sweeps;         /* sweep iterations summed over tile visits (register tiles: including
+                                    the in-layer sweeps that quiet layers skip) */
   uint64_t workspace_bytes;      /* device bytes owned by the search */
   uint32_t rounds;               /* global rounds executed */
   uint32_t phases;               /* connection phases executed */
@@ -268,6 +271,10 @@ typedef struct apgx_route_stats {
   uint32_t graph_launches;
   double round_kernel_ms;        /* summed hipEvent time of the round kernels */
   uint64_t tiles_touched;        /* distinct tiles visited (tile_visits counts revisits) */
+  uint64_t layer_sweeps;         /* sweeps executed, summed over each visited tile's layers */
+  uint64_t full_sweeps;          /* sweeps with the via exchange and the stability test */
+  uint64_t changing_full_sweeps; /* full sweeps whose stability test found a changed label */
+  uint64_t free_cell_visits;     /* free cells of the visited tiles, summed over visits */
 } apgx_route_stats;
 
 typedef struct apgx_search apgx_search;

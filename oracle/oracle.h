@@ -65,8 +65,9 @@ uint32_t orc_dense_astar(const orc_lattice* lat, uint32_t source, uint32_t targe
  * canonical path, add the path.  conn_pin/conn_dist/conn_len get npins-1
  * entries, cells gets the concatenated connection paths (pin..tree cell).
  * Returns 0 reached, 1 disconnected, -1 error.  Pinned by the two-pin case
- * (equals Dijkstra + canonical path) and by brute force over pin orders on
- * tiny lattices. */
+ * (equals Dijkstra + canonical path), hand-worked trees and ties, and an
+ * independent brute-force restatement on tiny lattices (Bellman-Ford from the
+ * tree over the explicit edge list, predecessor walk by lowest index). */
 int orc_dense_route_net(const orc_lattice* lat, const uint32_t* pins, uint32_t npins,
                         uint32_t* conn_pin, uint32_t* conn_dist, uint32_t* conn_len,
                         uint32_t* cells, uint64_t cell_cap, uint64_t* cells_used);

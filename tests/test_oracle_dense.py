@@ -206,6 +206,58 @@ def test_route_net_tie_takes_lowest_pin(oracle):
     assert rc == 0 and conns[0][0] == 1 and conns[1][0] == 2
 
 
+def brute_force_route_net(cost, via, window, pins):
+    """Independent restatement of the multi-pin rule on the explicit edge list:
+    Bellman-Ford from every tree cell, the nearest unconnected pin (lowest pin
+    index on a tie), its path walked back through the lowest-index in-window
+    free predecessor u with d[u] + w(u, v) == d[v], the path added to the tree."""
+    es = edges(cost, via, window)
+    n = cost.size
+    into = {}
+    for u, v, w in es:
+        into.setdefault(v, []).append((u, w))
+    tree = [int(pins[0])]
+    connected = {0}
+    out = []
+    while len(connected) < len(pins):
+        d = bellman_ford(n, es, tree)
+        best = min((int(d[int(p)]), k) for k, p in enumerate(pins) if k not in connected)
+        if best[0] == UNR:
+            return 1, []
+        dist, k = best
+        cell, path = int(pins[k]), [int(pins[k])]
+        while d[cell] != 0:
+            cell = min(u for u, w in into[cell] if d[u] != UNR and int(d[u]) + w == int(d[cell]))
+            path.append(cell)
+        out.append((k, dist, path))
+        connected.add(k)
+        tree.extend(path)
+    return 0, out
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_route_net_matches_brute_force(oracle, seed):
+    rng = np.random.default_rng(300 + seed)
+    L, H, W = [(1, 7, 9), (2, 6, 6), (3, 5, 7)][seed % 3]
+    cost = random_lattice(rng, L, H, W, blocked=0.15, cmax=[3, 9][seed % 2])
+    via = 0 if L == 1 else [2, 7, 20][seed % 3]
+    window = None if seed < 3 else (1, 0, W - 1, H - 1)
+    if window is None:
+        cells = np.flatnonzero(cost.reshape(-1))
+    else:
+        x0, y0, ww, wh = window
+        ll, yy, xx = np.meshgrid(np.arange(L), np.arange(y0, y0 + wh), np.arange(x0, x0 + ww), indexing="ij")
+        cells = ((ll * H + yy) * W + xx).reshape(-1)
+        cells = cells[cost.reshape(-1)[cells] > 0]
+    for trial in range(3):
+        pins = rng.choice(cells, size=min(len(cells), 2 + (seed + trial) % 5), replace=False)
+        rc, conns = oracle.Dense(cost, via, window).route_net(pins)
+        brc, bconns = brute_force_route_net(cost, via, window, pins)
+        assert rc == brc
+        assert [(p, d) for p, d, _ in conns] == [(p, d) for p, d, _ in bconns]
+        assert all(list(c) == b for (_, _, c), (_, _, b) in zip(conns, bconns))
+
+
 def test_single_layer_generator_statistics(apgx):
     cost = make_lattice(apgx, SINGLE_LAYER_256)
     assert cost.shape == (1, 256, 256)
