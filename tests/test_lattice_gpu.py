@@ -286,8 +286,12 @@ def test_eight_layer_2048_full_size(apgx, oracle, gpu):
     sample = [order[q * len(order) // 12] for q in range(6)]
     top = order[int(0.9 * len(order)):]
     multi = [i for i in top if counts[i] >= 4]
-    picks = [top[-1], min(multi, key=lambda i: sides[i]), top[len(top) // 2], top[0]]
-    assert len(set(picks)) == 4 and all(sides[i] >= sides[top[0]] for i in picks)
+    picks = []
+    for i in (top[-1], min(multi, key=lambda i: sides[i]), top[len(top) // 2], top[0], top[len(top) // 4]):
+        if i not in picks:
+            picks.append(i)
+    picks = picks[:4]
+    assert len(picks) == 4 and all(sides[i] >= sides[top[0]] for i in picks)
     sample += picks
     _say(f"[info] oracle nets: sides {[int(sides[i]) for i in sample]}, pins {[int(counts[i]) for i in sample]}")
 
