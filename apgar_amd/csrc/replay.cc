@@ -143,8 +143,10 @@ bool ParseUnsigned(std::string_view text, uint64_t max, uint64_t* out) {
   return true;
 }
 
-// ParseCanonicalReplayEnvelope (src/tooling/replay.cc:11-80).
-bool ParseEnvelope(std::string_view contents, std::vector<std::string_view>* values, std::string* error) {
+// ParseCanonicalReplayEnvelope (src/tooling/replay.cc:11-80) over one
+// format's keys in canonical order.
+bool ParseEnvelope(std::string_view contents, const std::string_view* keys, size_t key_count,
+                   std::vector<std::string_view>* values, std::string* error) {
   if (!ValidUtf8(contents)) {
     *error = "canonical replay must be valid UTF-8";
     return false;
@@ -172,7 +174,8 @@ bool ParseEnvelope(std::string_view contents, std::vector<std::string_view>* val
   }
   size_t offset = 0;
   values->clear();
-  for (std::string_view key : kKeys) {
+  for (size_t k = 0; k < key_count; ++k) {
+    const std::string_view key = keys[k];
     const size_t end = payload.find('\n', offset);
     if (end == std::string_view::npos) {
       *error = "replay payload is truncated";
@@ -202,7 +205,7 @@ void Copy(char* dst, size_t cap, std::string_view src) {
 // ParseArtifact (tools/gpu_batch_replay.cc:129-226).
 bool Parse(std::string_view contents, apgx_batch_replay* a, std::string* error) {
   std::vector<std::string_view> v;
-  if (!ParseEnvelope(contents, &v, error)) return false;
+  if (!ParseEnvelope(contents, kKeys, kKeyCount, &v, error)) return false;
   const FaultContract* contract = nullptr;
   for (const FaultContract& c : kContracts)
     if (c.name == v[6]) contract = &c;
@@ -250,6 +253,115 @@ bool Parse(std::string_view contents, apgx_batch_replay* a, std::string* error) 
   return true;
 }
 
+// The 14 keys of the single-route artifact in canonical order
+// (tools/gpu_replay_parser.cc:13-29).
+constexpr std::string_view kRouteKeys[] = {
+    "format=", "schema_version=", "fixture=", "fixture_checksum_fnv1a64=", "generator=", "fault=", "layer=",
+    "maximum_rounds=", "expected_board_content_hash=", "expected_profile_fingerprint=",
+    "expected_routing_profile_fingerprint=", "expected_device_view_fingerprint=", "expected_failure=",
+    "expected_invariant=",
+};
+
+// ParseGpuReplayArtifactV1 (tools/gpu_replay_parser.cc:11-56): v1 has one
+// contract, the frontier with the goal-predecessor self cycle reported as an
+// internal invariant.
+bool ParseRoute(std::string_view contents, apgx_route_replay* a, std::string* error) {
+  std::vector<std::string_view> v;
+  if (!ParseEnvelope(contents, kRouteKeys, sizeof(kRouteKeys) / sizeof(kRouteKeys[0]), &v, error)) return false;
+  if (v[0] != "apgar_gpu_invariant_replay" || v[1] != "1" || v[4] != "bucketed_frontier" ||
+      v[5] != "goal_predecessor_self_cycle" || v[12] != "internal_invariant" ||
+      v[13] != "gpu.predecessor.self_reference.v1") {
+    *error = "replay format, schema, generator, fault, or expected outcome is unsupported";
+    return false;
+  }
+  std::memset(a, 0, sizeof(*a));
+  Copy(a->fixture, sizeof(a->fixture), v[2]);
+  Copy(a->expected_invariant, sizeof(a->expected_invariant), v[13]);
+  a->generator = APGX_V1_GENERATOR_BUCKETED_FRONTIER;
+  a->fault = APGX_V1_ROUTE_FAULT_GOAL_PREDECESSOR_SELF_CYCLE;
+  uint64_t layer = 0, rounds = 0;
+  constexpr uint64_t k32 = 0xFFFFFFFFull, k64 = ~0ull;
+  if (!ParseUnsigned(v[3], k64, &a->fixture_checksum) || !ParseUnsigned(v[6], k32, &layer) ||
+      !ParseUnsigned(v[7], k32, &rounds) || rounds == 0 || !ParseUnsigned(v[8], k64, &a->expected_board_content_hash) ||
+      !ParseUnsigned(v[9], k64, &a->expected_profile_fingerprint) ||
+      !ParseUnsigned(v[10], k64, &a->expected_routing_profile_fingerprint) ||
+      !ParseUnsigned(v[11], k64, &a->expected_device_view_fingerprint)) {
+    *error = "replay contains an invalid integer field";
+    return false;
+  }
+  a->layer = static_cast<uint32_t>(layer);
+  a->maximum_rounds = static_cast<uint32_t>(rounds);
+  return true;
+}
+
+// A fixture compiled into its device view, with the request's endpoints.
+struct Fixture {
+  apgx_replay_fixture_header fh;
+  apgx_v1_header header;
+  std::vector<apgx_v1_layer_range> layers;
+  std::vector<apgx_v1_node> nodes;
+  std::vector<apgx_v1_run> runs;
+  std::vector<uint32_t> run_nodes;
+  uint32_t start = APGX_V1_INVALID_INDEX, goal = APGX_V1_INVALID_INDEX;
+};
+
+// Checksum, record structure and layer of a fixture, its compilation by
+// apgx_v1_build_board and its endpoints.  Returns 0, or the stage at which it
+// was rejected (the runner's exit code is then 2) with the message.
+int CompileFixture(const uint8_t* fixture, uint64_t fixture_length, uint64_t checksum, uint32_t layer, Fixture* fx,
+                   std::string* why) {
+  const std::string_view bytes(reinterpret_cast<const char*>(fixture ? fixture : reinterpret_cast<const uint8_t*>("")),
+                               fixture_length);
+  if (fixture == nullptr || Fnv1a(bytes) != checksum) {
+    *why = "fixture is missing or its checksum differs";
+    return APGX_REPLAY_STAGE_FIXTURE;
+  }
+  apgx_replay_fixture_header& fh = fx->fh;
+  if (fixture_length < sizeof(fh)) {
+    *why = "fixture is truncated";
+    return APGX_REPLAY_STAGE_FIXTURE;
+  }
+  std::memcpy(&fh, fixture, sizeof(fh));
+  if (std::memcmp(fh.magic, APGX_REPLAY_FIXTURE_MAGIC, sizeof(fh.magic)) != 0 || fh.schema_version != 1 ||
+      fh.node_count > (fixture_length - sizeof(fh)) / sizeof(apgx_v1_compiled_node) ||
+      fixture_length != sizeof(fh) + fh.node_count * sizeof(apgx_v1_compiled_node)) {
+    *why = "fixture is not an apgx V1 replay fixture";
+    return APGX_REPLAY_STAGE_FIXTURE;
+  }
+  std::vector<apgx_v1_compiled_node> cnodes(fh.node_count);
+  if (fh.node_count)
+    std::memcpy(cnodes.data(), fixture + sizeof(fh), fh.node_count * sizeof(apgx_v1_compiled_node));
+  if (fh.layer != layer) {
+    *why = "replay layer differs from the versioned fixture case";
+    return APGX_REPLAY_STAGE_FIXTURE;
+  }
+  // Device-view compilation (BuildDeviceCompiledBoardV1).
+  apgx_v1_board* board = nullptr;
+  if (apgx_v1_build_board(&fh.profile, cnodes.data(), fh.node_count, &board) != APGX_OK) {
+    *why = std::string("device-view compilation failed: ") + apgx_last_error();
+    return APGX_REPLAY_STAGE_FIXTURE;
+  }
+  uint64_t nl = 0, nn = 0, nr = 0, nrn = 0;
+  apgx_v1_board_sizes(board, &nl, &nn, &nr, &nrn);
+  fx->layers.resize(nl);
+  fx->nodes.resize(nn);
+  fx->runs.resize(nr);
+  fx->run_nodes.resize(nrn);
+  apgx_v1_board_copy(board, &fx->header, fx->layers.data(), fx->nodes.data(), fx->runs.data(), fx->run_nodes.data());
+  apgx_v1_board_release(board);
+  for (uint64_t i = 0; i < nn; ++i) {
+    const apgx_v1_node& n = fx->nodes[i];
+    if (n.layer != fh.layer) continue;
+    if (n.lattice_x == fh.start_x && n.lattice_y == fh.start_y) fx->start = static_cast<uint32_t>(i);
+    if (n.lattice_x == fh.goal_x && n.lattice_y == fh.goal_y) fx->goal = static_cast<uint32_t>(i);
+  }
+  if (fx->start == APGX_V1_INVALID_INDEX || fx->goal == APGX_V1_INVALID_INDEX || fx->start == fx->goal) {
+    *why = "fixture endpoints are not distinct represented nodes";
+    return APGX_REPLAY_STAGE_FIXTURE;
+  }
+  return 0;
+}
+
 }  // namespace
 
 // FingerprintCandidateGenerationPolicy (src/routing/candidate_policy.cc:139-160)
@@ -279,6 +391,17 @@ extern "C" int apgx_batch_replay_parse(const char* text, uint64_t length, apgx_b
   if (out == nullptr || (length && text == nullptr)) return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_batch_replay_parse: null argument");
   std::string why;
   const bool ok = replay::Parse(std::string_view(text ? text : "", length), out, &why);
+  if (error && error_capacity) replay::Copy(error, error_capacity, ok ? std::string_view() : std::string_view(why));
+  ClearError();
+  return ok ? APGX_OK : APGX_E_UNSUPPORTED;
+}
+
+extern "C" int apgx_route_replay_parse(const char* text, uint64_t length, apgx_route_replay* out, char* error,
+                                       uint64_t error_capacity) {
+  using namespace apgx;
+  if (out == nullptr || (length && text == nullptr)) return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_route_replay_parse: null argument");
+  std::string why;
+  const bool ok = replay::ParseRoute(std::string_view(text ? text : "", length), out, &why);
   if (error && error_capacity) replay::Copy(error, error_capacity, ok ? std::string_view() : std::string_view(why));
   ClearError();
   return ok ? APGX_OK : APGX_E_UNSUPPORTED;
@@ -315,36 +438,16 @@ extern "C" int apgx_batch_replay_run(const char* text, uint64_t text_length, con
   std::string why;
   if (!replay::Parse(std::string_view(text ? text : "", text_length), &a, &why))
     return o.finish(2, APGX_REPLAY_STAGE_PARSE, why);
-  // Fixture: checksum first, then its record structure.
-  const std::string_view bytes(reinterpret_cast<const char*>(fixture ? fixture : reinterpret_cast<const uint8_t*>("")),
-                               fixture_length);
-  if (fixture == nullptr || replay::Fnv1a(bytes) != a.fixture_checksum)
-    return o.finish(2, APGX_REPLAY_STAGE_FIXTURE, "fixture is missing or its checksum differs");
-  apgx_replay_fixture_header fh;
-  if (fixture_length < sizeof(fh)) return o.finish(2, APGX_REPLAY_STAGE_FIXTURE, "fixture is truncated");
-  std::memcpy(&fh, fixture, sizeof(fh));
-  if (std::memcmp(fh.magic, APGX_REPLAY_FIXTURE_MAGIC, sizeof(fh.magic)) != 0 || fh.schema_version != 1 ||
-      fh.node_count > (fixture_length - sizeof(fh)) / sizeof(apgx_v1_compiled_node) ||
-      fixture_length != sizeof(fh) + fh.node_count * sizeof(apgx_v1_compiled_node))
-    return o.finish(2, APGX_REPLAY_STAGE_FIXTURE, "fixture is not an apgx V1 replay fixture");
-  std::vector<apgx_v1_compiled_node> cnodes(fh.node_count);
-  if (fh.node_count)
-    std::memcpy(cnodes.data(), fixture + sizeof(fh), fh.node_count * sizeof(apgx_v1_compiled_node));
-  if (fh.layer != a.layer)
-    return o.finish(2, APGX_REPLAY_STAGE_FIXTURE, "replay layer differs from the versioned fixture case");
-  // Device-view compilation (BuildDeviceCompiledBoardV1).
-  apgx_v1_board* board = nullptr;
-  if (apgx_v1_build_board(&fh.profile, cnodes.data(), fh.node_count, &board) != APGX_OK)
-    return o.finish(2, APGX_REPLAY_STAGE_FIXTURE, std::string("device-view compilation failed: ") + apgx_last_error());
-  uint64_t nl = 0, nn = 0, nr = 0, nrn = 0;
-  apgx_v1_board_sizes(board, &nl, &nn, &nr, &nrn);
-  apgx_v1_header header;
-  std::vector<apgx_v1_layer_range> layers(nl);
-  std::vector<apgx_v1_node> nodes(nn);
-  std::vector<apgx_v1_run> runs(nr);
-  std::vector<uint32_t> run_nodes(nrn);
-  apgx_v1_board_copy(board, &header, layers.data(), nodes.data(), runs.data(), run_nodes.data());
-  apgx_v1_board_release(board);
+  replay::Fixture fx;
+  if (int code = replay::CompileFixture(fixture, fixture_length, a.fixture_checksum, a.layer, &fx, &why))
+    return o.finish(2, static_cast<uint32_t>(code), why);
+  const apgx_replay_fixture_header& fh = fx.fh;
+  const apgx_v1_header& header = fx.header;
+  auto& layers = fx.layers;
+  auto& nodes = fx.nodes;
+  auto& runs = fx.runs;
+  auto& run_nodes = fx.run_nodes;
+  const uint64_t nl = layers.size(), nn = nodes.size(), nr = runs.size(), nrn = run_nodes.size();
   // Association gate, before any device work (tools/gpu_batch_replay.cc:271-287).
   const uint64_t policy_identity = replay::PolicyIdentity(a);
   if (header.source_board_content_hash != a.expected_board_content_hash ||
@@ -366,15 +469,7 @@ extern "C" int apgx_batch_replay_run(const char* text, uint64_t text_length, con
                   static_cast<unsigned long long>(policy_identity));
     return o.finish(2, APGX_REPLAY_STAGE_ASSOCIATION, msg);
   }
-  // Endpoints of the fixture's request.
-  uint32_t start = APGX_V1_INVALID_INDEX, goal = APGX_V1_INVALID_INDEX;
-  for (uint64_t i = 0; i < nn; ++i) {
-    if (nodes[i].layer != fh.layer) continue;
-    if (nodes[i].lattice_x == fh.start_x && nodes[i].lattice_y == fh.start_y) start = static_cast<uint32_t>(i);
-    if (nodes[i].lattice_x == fh.goal_x && nodes[i].lattice_y == fh.goal_y) goal = static_cast<uint32_t>(i);
-  }
-  if (start == APGX_V1_INVALID_INDEX || goal == APGX_V1_INVALID_INDEX || start == goal)
-    return o.finish(2, APGX_REPLAY_STAGE_FIXTURE, "fixture endpoints are not distinct represented nodes");
+  const uint32_t start = fx.start, goal = fx.goal;
   if (a.fault == APGX_REPLAY_FAULT_UNAUTHENTICATED_PRODUCER)
     return o.finish(2, APGX_REPLAY_STAGE_ASSOCIATION,
                     "candidate construction and producer authentication are outside this library");
@@ -444,6 +539,74 @@ extern "C" int apgx_batch_replay_run(const char* text, uint64_t text_length, con
                                   : std::string("expected GPU batch invariant was not reproduced: ") + check.detail;
     return o.finish(1, APGX_REPLAY_STAGE_REPRODUCE, msg, check.invariant);
   }
+  apgx_device_info info;
+  std::memset(&info, 0, sizeof(info));
+  apgx_device_info_query(&info);
+  char msg[256];
+  std::snprintf(msg, sizeof(msg),
+                "reproduced=internal_invariant invariant=%s artifact_schema=1 backend=%s device=\"%s\" arch=%s",
+                a.expected_invariant, info.backend, info.device_name, info.gcn_arch_name);
+  return o.finish(0, APGX_REPLAY_STAGE_REPRODUCE, msg, check.invariant);
+}
+
+extern "C" int apgx_route_replay_run(const char* text, uint64_t text_length, const uint8_t* fixture,
+                                     uint64_t fixture_length, uint32_t flags, apgx_replay_outcome* out) {
+  using namespace apgx;
+  if (out == nullptr || (text_length && text == nullptr) || (fixture_length && fixture == nullptr))
+    return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_route_replay_run: null argument");
+  std::memset(out, 0, sizeof(*out));
+  Outcome o{out};
+  apgx_route_replay a;
+  std::string why;
+  if (!replay::ParseRoute(std::string_view(text ? text : "", text_length), &a, &why))
+    return o.finish(2, APGX_REPLAY_STAGE_PARSE, why);
+  replay::Fixture fx;
+  if (int code = replay::CompileFixture(fixture, fixture_length, a.fixture_checksum, a.layer, &fx, &why))
+    return o.finish(2, static_cast<uint32_t>(code), why);
+  // Association gate (tools/gpu_replay_runner.cc:78-89), before any device work.
+  const bool board = fx.header.source_board_content_hash != a.expected_board_content_hash;
+  const bool profile = fx.header.compiler_profile_fingerprint != a.expected_profile_fingerprint;
+  const bool routing = fx.fh.routing_profile_fingerprint != a.expected_routing_profile_fingerprint;
+  const bool device = fx.header.device_view_fingerprint != a.expected_device_view_fingerprint;
+  if (board || profile || routing || device) {
+    std::string msg = "replay semantic association mismatch:";
+    if (board) msg += " board";
+    if (profile) msg += " compiler_profile";
+    if (routing) msg += " routing_profile";
+    if (device) msg += " device_view";
+    return o.finish(2, APGX_REPLAY_STAGE_ASSOCIATION, msg);
+  }
+  if (flags & APGX_REPLAY_DRY_RUN) return o.finish(0, APGX_REPLAY_STAGE_ASSOCIATION, "associations verified");
+  apgx_v1_view* view = nullptr;
+  if (apgx_v1_view_upload(&fx.header, fx.layers.data(), fx.layers.size(), fx.nodes.data(), fx.nodes.size(),
+                          fx.runs.data(), fx.runs.size(), fx.run_nodes.data(), fx.run_nodes.size(), &view) != APGX_OK)
+    return o.finish(1, APGX_REPLAY_STAGE_EXECUTE, std::string("view upload failed: ") + apgx_last_error());
+  const apgx_v1_route_request req{a.generator, fx.start, fx.goal, a.maximum_rounds, fx.fh.routing_profile_fingerprint,
+                                  ~0ull};
+  apgx_v1_route_pending* pending = nullptr;
+  if (apgx_v1_route_execute(view, &req, nullptr, &pending) != APGX_OK) {
+    const std::string err = apgx_last_error();
+    apgx_v1_view_release(view);
+    return o.finish(1, APGX_REPLAY_STAGE_EXECUTE, "route failed before validation: " + err);
+  }
+  apgx_v1_route_pending_inject_fault(pending, a.fault);
+  const uint64_t S = fx.nodes.size() * 9;
+  std::vector<uint64_t> labels(S);
+  std::vector<uint32_t> preds(S);
+  apgx_v1_route_result result;
+  const int rc = apgx_v1_route_readback(pending, &result, labels.data(), preds.data(), S);
+  apgx_v1_route_pending_release(pending);
+  apgx_v1_view_release(view);
+  if (rc != APGX_OK)
+    return o.finish(1, APGX_REPLAY_STAGE_EXECUTE, std::string("route readback failed: ") + apgx_last_error());
+  apgx_v1_validation check;
+  uint64_t cost = 0;
+  apgx_v1_validate_route(&fx.header, fx.nodes.data(), fx.nodes.size(), &req, &result, labels.data(), preds.data(),
+                         &cost, &check);
+  const bool reproduced = !check.valid && check.failure == APGX_V1_CHECK_INTERNAL_INVARIANT &&
+                          std::string_view(check.invariant) == a.expected_invariant;
+  if (!reproduced) return o.finish(1, APGX_REPLAY_STAGE_REPRODUCE, "expected GPU invariant failure was not reproduced",
+                                   check.invariant);
   apgx_device_info info;
   std::memset(&info, 0, sizeof(info));
   apgx_device_info_query(&info);

@@ -24,6 +24,7 @@
 #include "apgx.h"
 #include "common.h"
 #include "v1_common.h"
+#include "v1_view.h"
 
 namespace apgx {
 namespace v1 {
@@ -445,15 +446,6 @@ __global__ void V1ScatterKernel(V1Args a) {
 }  // namespace v1
 }  // namespace apgx
 
-struct apgx_v1_view {
-  apgx_v1_header header{};
-  int device = 0;
-  uint64_t layer_count = 0, node_count = 0, run_count = 0, run_node_count = 0, legal_edges = 0;
-  apgx_v1_node* d_nodes = nullptr;
-  apgx_v1_run* d_runs = nullptr;
-  uint32_t* d_run_nodes = nullptr;
-  uint64_t persistent_bytes = 0;
-};
 
 struct apgx_v1_pending {
   uint32_t query_count = 0;
@@ -581,6 +573,7 @@ extern "C" int apgx_v1_view_upload(const apgx_v1_header* header, const apgx_v1_l
 
 extern "C" void apgx_v1_view_release(apgx_v1_view* v) {
   if (v == nullptr) return;
+  if (v->cancel_word) (void)hipHostFree(v->cancel_word);
   if (v->d_nodes) (void)hipFree(v->d_nodes);
   if (v->d_runs) (void)hipFree(v->d_runs);
   if (v->d_run_nodes) (void)hipFree(v->d_run_nodes);

@@ -453,6 +453,136 @@ extern "C" int apgx_v1_validate_batch(const apgx_v1_header* header, const apgx_v
   return APGX_OK;
 }
 
+namespace apgx {
+namespace v1route {
+uint64_t RouteBytes(bool frontier, uint64_t node_count);
+}  // namespace v1route
+}  // namespace apgx
+
+extern "C" int apgx_v1_validate_route(const apgx_v1_header* header, const apgx_v1_node* nodes, uint64_t node_count,
+                                      const apgx_v1_route_request* request, const apgx_v1_route_result* result,
+                                      const uint64_t* labels, const uint32_t* predecessors, uint64_t* path_cost,
+                                      apgx_v1_validation* out) {
+  using namespace apgx;
+  using namespace apgx::v1;
+  using namespace apgx::v1check;
+  if (out == nullptr || header == nullptr || (node_count && nodes == nullptr) || request == nullptr ||
+      result == nullptr)
+    return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_v1_validate_route: null argument");
+  std::memset(out, 0, sizeof(*out));
+  out->valid = 1;
+  if (path_cost) *path_cost = kInf;
+  auto reject = [&](uint32_t failure, const char* invariant, const std::string& detail) {
+    out->valid = 0;
+    out->failure = failure;
+    std::strncpy(out->invariant, invariant, sizeof(out->invariant) - 1);
+    std::strncpy(out->detail, detail.c_str(), sizeof(out->detail) - 1);
+    ClearError();
+    return APGX_OK;
+  };
+  auto invariant = [&](const char* id, const std::string& detail) {
+    return reject(APGX_V1_CHECK_INTERNAL_INVARIANT, id, detail);
+  };
+  auto failed = [&](const std::string& detail) { return reject(APGX_V1_CHECK_VALIDATION_FAILED, "", detail); };
+  const uint64_t S = node_count * 9;
+  const apgx_v1_kernel_telemetry& t = result->telemetry;
+  const apgx_v1_route_header& rh = result->header;
+
+  // ValidateKernelTelemetry (src/gpu/planar_router.cc:411-454).
+  GeneratorShape shape{};
+  if (!DescribeGenerator(request->generator, &shape))
+    return reject(APGX_V1_CHECK_VALIDATION_FAILED, "gpu.generator.unknown.v1",
+                  "GPU telemetry validation names an unknown planar generator");
+  if (t.persistent_device_bytes != header->estimated_persistent_device_bytes)
+    return invariant("", "GPU persistent-memory telemetry disagrees with deterministic accounting");
+  if (t.batch_device_bytes != v1route::RouteBytes(request->generator == APGX_V1_GENERATOR_BUCKETED_FRONTIER, node_count))
+    return invariant("", "GPU batch-memory telemetry disagrees with deterministic accounting");
+  uint64_t peak = 0;
+  if (!CheckedAdd(t.persistent_device_bytes, t.batch_device_bytes, &peak) || t.peak_device_bytes != peak)
+    return invariant("", "GPU peak-memory telemetry disagrees with simultaneous owned tiers");
+  if (!std::isfinite(t.kernel_milliseconds) || t.kernel_milliseconds < 0.0)
+    return invariant("", "GPU kernel-time telemetry is not finite and nonnegative");
+
+  // ValidateRawAssociations (:382-409).
+  if (rh.schema_version != APGX_V1_ROUTE_RESULT_SCHEMA_VERSION)
+    return failed("GPU result schema does not match the route evidence contract");
+  if (rh.source_board_content_hash != header->source_board_content_hash ||
+      rh.compiler_profile_fingerprint != header->compiler_profile_fingerprint ||
+      rh.compiler_version != header->compiler_version || rh.rule_bucket_identity != header->rule_bucket_identity ||
+      rh.device_view_fingerprint != header->device_view_fingerprint)
+    return failed("GPU result associations do not match the immutable device view");
+  if (rh.routing_profile_fingerprint != request->routing_profile_fingerprint)
+    return failed("GPU result prepared routing context is invalid or mismatched");
+  if (rh.start_node != request->start_node || rh.goal_node != request->goal_node || rh.generator != request->generator)
+    return failed("GPU result endpoint or generator associations do not match the route request");
+
+  // ValidateCompletionTelemetry (:456-484).
+  const uint32_t completion = result->completion;
+  if (completion > APGX_V1_CANCELLED)
+    return invariant("gpu.completion.unknown.v1", "GPU result contains an unknown completion code");
+  if (t.rounds > request->maximum_rounds)
+    return invariant("", "GPU round telemetry exceeds the configured execution budget");
+  if (completion == APGX_V1_BUDGET_EXHAUSTED && t.rounds != request->maximum_rounds)
+    return invariant("", "GPU reported budget exhaustion before consuming its round budget");
+  if ((completion == APGX_V1_REACHED || completion == APGX_V1_DISCONNECTED) && t.rounds == 0)
+    return invariant("", "GPU reported a converged completion without an execution round");
+
+  // ValidateAllPredecessors (:492-612): exact when converged, a lower bound
+  // otherwise; the reached path re-runs it exactly below, as the reference does.
+  if (result->state_count != S || labels == nullptr || predecessors == nullptr)
+    return invariant("", "GPU label or predecessor array has an invalid bound");
+  if (request->start_node >= node_count)
+    return invariant("", "GPU start node maps outside the flattened state array");
+  if (request->goal_node >= node_count) return invariant("", "GPU goal node maps outside the flattened state array");
+  const Context c{header, nodes, node_count, nullptr};
+  apgx_v1_query q;
+  std::memset(&q, 0, sizeof(q));
+  q.start_node = request->start_node;
+  q.goal_node = request->goal_node;
+  std::string why;
+  const char* id = "";
+  const bool exact = completion == APGX_V1_REACHED || completion == APGX_V1_DISCONNECTED;
+  if (!AllPredecessorsValid(c, q, labels, predecessors, exact, &why, &id)) return invariant(id, why);
+
+  if (completion == APGX_V1_REACHED) {
+    // ValidateAndReconstructGpuRouteWithPolicy (:829-930): the goal state and
+    // the predecessor walk from it to the start.
+    uint64_t best = kInf;
+    uint32_t best_state = kNone;
+    for (uint32_t h = 0; h < 8; ++h) {
+      const uint32_t st = request->goal_node * 9u + h;
+      if (labels[st] < best || (labels[st] == best && st < best_state)) {
+        best = labels[st];
+        best_state = st;
+      }
+    }
+    if (best == kInf || result->goal_state != best_state)
+      return invariant("", "GPU goal state is not the stable minimum finite heading state");
+    const uint32_t start_state = request->start_node * 9u + 8u;
+    uint32_t cursor = best_state;
+    uint64_t points = 0;
+    for (;;) {
+      if (cursor >= S) return invariant("", "GPU reconstruction escaped the allocated state array");
+      ++points;
+      if (cursor == start_state) break;
+      cursor = predecessors[cursor];
+      if (points > S) return invariant("", "GPU predecessor reconstruction exceeded its acyclic state bound");
+    }
+    if (path_cost) *path_cost = best;
+  } else if (completion == APGX_V1_DISCONNECTED) {
+    // ValidateDisconnectedResult (:613-627) and the CPU confirmation (:629-667).
+    for (uint32_t h = 0; h < 8; ++h)
+      if (labels[request->goal_node * 9ull + h] != kInf)
+        return invariant("", "GPU reported disconnected with a finite goal label");
+    if (GoalReachable(c, q))
+      return invariant("gpu.disconnected.cpu_reachable.v1",
+                       "GPU reported disconnected for a query the CPU oracle reached under the identical normalized "
+                       "policy");
+  }
+  ClearError();
+  return APGX_OK;
+}
+
 extern "C" int apgx_v1_generator_shape(uint32_t generator, apgx_v1_generator_shape_info* out) {
   using namespace apgx;
   using namespace apgx::v1;

@@ -1,6 +1,7 @@
-"""Candidate-batch invariant replays (gpu_batch_replay v1) over libapgx.so.
+"""Invariant replays over libapgx.so: candidate batches (gpu_batch_replay v1)
+and single routes (gpu_replay v1).
 
-Mirrors tools/gpu_batch_replay.cc: `python -m apgar_amd.replay` with no
+Mirrors tools/gpu_batch_replay.cc and tools/gpu_replay.cc: `python -m apgar_amd.replay` with no
 arguments reproduces every default artifact and checks that every rejected
 artifact fails at the generator/fault association rule; with arguments it
 reproduces the named artifacts.  Exit status 0 all reproduced, 1 a failure was
@@ -38,6 +39,13 @@ class BatchReplay(C.Structure):
                 ("expected_invariant", C.c_char * 64)]
 
 
+class RouteReplay(C.Structure):
+    _fields_ = [("fixture", C.c_char * 256), ("fixture_checksum", U64), ("generator", U32), ("fault", U32),
+                ("layer", U32), ("maximum_rounds", U32), ("expected_board_content_hash", U64),
+                ("expected_profile_fingerprint", U64), ("expected_routing_profile_fingerprint", U64),
+                ("expected_device_view_fingerprint", U64), ("expected_invariant", C.c_char * 64)]
+
+
 class FixtureHeader(C.Structure):
     _fields_ = [("magic", C.c_char * 16), ("schema_version", U32), ("layer", U32), ("start_x", I64),
                 ("start_y", I64), ("goal_x", I64), ("goal_y", I64), ("routing_profile_fingerprint", U64),
@@ -56,7 +64,10 @@ _SIGS = {
     "apgx_batch_replay_parse": (C.c_int, [C.c_char_p, U64, P(BatchReplay), C.c_char_p, U64]),
     "apgx_batch_replay_policy_identity": (U64, [P(BatchReplay)]),
     "apgx_batch_replay_run": (C.c_int, [C.c_char_p, U64, C.c_char_p, U64, U32, P(Outcome)]),
+    "apgx_route_replay_parse": (C.c_int, [C.c_char_p, U64, P(RouteReplay), C.c_char_p, U64]),
+    "apgx_route_replay_run": (C.c_int, [C.c_char_p, U64, C.c_char_p, U64, U32, P(Outcome)]),
 }
+ROUTE_FORMAT = b"format=apgar_gpu_invariant_replay\n"
 
 
 def _lib():
@@ -80,6 +91,18 @@ def parse(text: bytes):
     return (out, "") if rc == 0 else (None, err.value.decode())
 
 
+def is_route_artifact(text: bytes) -> bool:
+    """Single-route (gpu_replay v1) artifacts open with their own format line."""
+    return text.startswith(ROUTE_FORMAT)
+
+
+def parse_route(text: bytes):
+    """Single-route artifact: (artifact, "") or (None, the reference's message)."""
+    out, err = RouteReplay(), C.create_string_buffer(256)
+    rc = _lib().apgx_route_replay_parse(text, len(text), C.byref(out), err, len(err))
+    return (out, "") if rc == 0 else (None, err.value.decode())
+
+
 def policy_identity(artifact: BatchReplay) -> int:
     return _lib().apgx_batch_replay_policy_identity(C.byref(artifact))
 
@@ -100,10 +123,11 @@ def fixture_bytes(profile: v1.CompiledProfile, nodes, start, goal, *, layer=0, r
 
 
 def run_bytes(text: bytes, fixture: bytes | None, dry_run=False) -> Outcome:
+    """Runs a batch (gpu_batch_replay) or single-route (gpu_replay) artifact."""
     out = Outcome()
     fx = fixture if fixture is not None else None
-    check(_lib().apgx_batch_replay_run(text, len(text), fx, len(fx) if fx is not None else 0,
-                                       DRY_RUN if dry_run else 0, C.byref(out)))
+    runner = _lib().apgx_route_replay_run if is_route_artifact(text) else _lib().apgx_batch_replay_run
+    check(runner(text, len(text), fx, len(fx) if fx is not None else 0, DRY_RUN if dry_run else 0, C.byref(out)))
     return out
 
 
@@ -111,7 +135,7 @@ def run(path, dry_run=False) -> Outcome:
     """Reproduces one artifact; its fixture path is resolved against the
     repository root (a missing fixture is reported, not raised)."""
     text = Path(path).read_bytes()
-    art, _ = parse(text)
+    art, _ = parse_route(text) if is_route_artifact(text) else parse(text)
     fixture = None
     if art is not None:
         fp = ROOT / art.fixture.decode()

@@ -120,6 +120,38 @@ class Validation(C.Structure):
                 ("invariant", C.c_char * 64), ("detail", C.c_char * 128)]
 
 
+class RouteRequest(C.Structure):
+    _fields_ = [("generator", U32), ("start_node", U32), ("goal_node", U32), ("maximum_rounds", U32),
+                ("routing_profile_fingerprint", U64), ("maximum_device_bytes", U64)]
+
+
+class RouteHeader(C.Structure):
+    _fields_ = [("schema_version", U32), ("compiler_version", U32), ("start_node", U32), ("goal_node", U32),
+                ("source_board_content_hash", U64), ("compiler_profile_fingerprint", U64),
+                ("routing_profile_fingerprint", U64), ("rule_bucket_identity", U64),
+                ("device_view_fingerprint", U64), ("generator", U8), ("reserved", U8 * 7)]
+
+
+class KernelTelemetry(C.Structure):
+    _fields_ = [("persistent_device_bytes", U64), ("batch_device_bytes", U64), ("peak_device_bytes", U64),
+                ("examined_work", U64), ("heading_turn_relaxations", U64), ("rounds", U32), ("reserved", U32),
+                ("kernel_milliseconds", C.c_double)]
+
+
+class RouteResult(C.Structure):
+    _fields_ = [("header", RouteHeader), ("completion", U32), ("goal_state", U32), ("state_count", U64),
+                ("telemetry", KernelTelemetry)]
+
+
+class RouteReadback(NamedTuple):
+    result: RouteResult
+    labels: np.ndarray
+    predecessors: np.ndarray
+
+
+ROUTE_FAULT_NONE, ROUTE_FAULT_GOAL_PREDECESSOR_SELF_CYCLE = 0, 1
+
+
 class GeneratorShape(C.Structure):
     _fields_ = [("chunk_rounds", U32), ("fixed_launches", U32), ("fixed_examined_work_per_round", U32),
                 ("examined_work_per_node_per_round", U32), ("heading_turn_relaxations_per_examined_work", U32),
@@ -188,6 +220,13 @@ _SIGS = {
     "apgx_v1_validate_batch": (C.c_int, [P(Header), P(Node), U64, P(BatchRequest), P(Query), U32, P(PolicyEdge),
                                          U64, P(Result), P(CompactPath), P(U32), U64, P(WorkspaceView),
                                          P(BatchTelemetry), P(U64), P(Validation)]),
+    "apgx_v1_route_execute": (C.c_int, [C.c_void_p, P(RouteRequest), P(U32), P(C.c_void_p)]),
+    "apgx_v1_route_readback": (C.c_int, [C.c_void_p, P(RouteResult), P(U64), P(U32), U64]),
+    "apgx_v1_route_pending_inject_fault": (C.c_int, [C.c_void_p, U32]),
+    "apgx_v1_route_pending_release": (None, [C.c_void_p]),
+    "apgx_v1_expected_route_bytes": (C.c_int, [U32, U64, P(U64)]),
+    "apgx_v1_validate_route": (C.c_int, [P(Header), P(Node), U64, P(RouteRequest), P(RouteResult), P(U64), P(U32),
+                                         P(U64), P(Validation)]),
 }
 
 
@@ -308,6 +347,60 @@ class View:
             _lib().apgx_v1_pending_release(pending)
         n = len(queries)
         return Readback(list(results)[:n], list(paths)[:n], list(states)[: used.value], tel, ws)
+
+
+    def route(self, start, goal, *, generator=GENERATOR_FRONTIER, maximum_rounds=100_000,
+              routing_profile_fingerprint=0, maximum_device_bytes=(1 << 64) - 1, cancel=None,
+              fault=ROUTE_FAULT_NONE):
+        """Executes one single route (apgx_v1_route_execute) and reads it back:
+        RouteReadback(result, labels, predecessors).  `cancel` (a one-element
+        ctypes uint32 array, or True for an already set flag) is the caller's
+        cancellation word; `fault` (replay/test only) applies the readback
+        decorator."""
+        req = RouteRequest(generator, start, goal, maximum_rounds, routing_profile_fingerprint,
+                           maximum_device_bytes)
+        flag = None
+        if cancel is True:
+            flag = (U32 * 1)(1)
+        elif cancel is not None:
+            flag = cancel
+        pending = C.c_void_p()
+        check(_lib().apgx_v1_route_execute(self.handle, C.byref(req), flag, C.byref(pending)))
+        try:
+            check(_lib().apgx_v1_route_pending_inject_fault(pending, fault))
+            S = len(self.board.nodes) * 9
+            labels = np.zeros(S, np.uint64)
+            preds = np.zeros(S, np.uint32)
+            res = RouteResult()
+            check(_lib().apgx_v1_route_readback(pending, C.byref(res), labels.ctypes.data_as(P(U64)),
+                                                preds.ctypes.data_as(P(U32)), S))
+        finally:
+            _lib().apgx_v1_route_pending_release(pending)
+        return RouteReadback(res, labels, preds)
+
+
+def route_request_for(start, goal, *, generator=GENERATOR_FRONTIER, maximum_rounds=100_000,
+                      routing_profile_fingerprint=0, maximum_device_bytes=(1 << 64) - 1):
+    return RouteRequest(generator, start, goal, maximum_rounds, routing_profile_fingerprint, maximum_device_bytes)
+
+
+def validate_route(board: Board, request: RouteRequest, readback: RouteReadback):
+    """Host trust-boundary check of a single route (apgx_v1_validate_route):
+    (valid, invariant id, detail, path cost, failure class)."""
+    labels = np.ascontiguousarray(readback.labels, dtype=np.uint64)
+    preds = np.ascontiguousarray(readback.predecessors, dtype=np.uint32)
+    cost = U64()
+    out = Validation()
+    check(_lib().apgx_v1_validate_route(C.byref(board.header), board.nodes, len(board.nodes), C.byref(request),
+                                        C.byref(readback.result), labels.ctypes.data_as(P(U64)),
+                                        preds.ctypes.data_as(P(U32)), C.byref(cost), C.byref(out)))
+    return bool(out.valid), out.invariant.decode(), out.detail.decode(), cost.value, out.failure
+
+
+def expected_route_bytes(generator, node_count):
+    out = U64()
+    check(_lib().apgx_v1_expected_route_bytes(generator, node_count, C.byref(out)))
+    return out.value
 
 
 def _read_workspace(pending, n):

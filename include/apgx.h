@@ -984,6 +984,120 @@ int apgx_v1_expected_batch_bytes(uint32_t generator, uint64_t query_count, uint6
                                  uint64_t* host_bytes, uint64_t* readback_bytes);
 
 /* ------------------------------------------------------------------------ */
+/* V1 single route                                                          */
+/* ------------------------------------------------------------------------ */
+/* One start / goal node pair on an uploaded view under the profile's step and
+ * bend costs (no candidate policy): replaces IPlanarRouteBackend::ExecuteRoute
+ * and ReadbackRoute (include/apgar/gpu/planar_router.h:549-551; the CUDA
+ * backend's gpu/cuda/planar_router.cu:2049-2181 and 2794-2853), which
+ * RouteWithPreparedPlanarGpuBackend (src/gpu/planar_router.cc:1206-1372)
+ * calls for default-policy routes.  Two generators, each with the reference's
+ * round semantics, so rounds and examined work equal the reference's:
+ *  - bucketed frontier: delta-stepping over heading states (FrontierRelax /
+ *    RunFrontier, :363-412, :1552-1640).  Each round relaxes every active
+ *    state whose label lies in the current bucket (width min(orthogonal,
+ *    diagonal step)) and defers the others; a round with nothing left to do
+ *    converges.  It explores the whole reachable state space, it does not stop
+ *    at the goal.
+ *  - heading-aware sweep: departures then segmented run scans each round
+ *    (ComputeDepartureLabels / SweepRuns / RunSweep, :414-484, :1642-1722),
+ *    converged when a round lowers no label.
+ * Then every state's lowest tight predecessor (SelectStablePredecessors,
+ * :486-524).  The whole round loop runs in one launch (one workgroup), with
+ * cancellation read once per round from a host-mapped word that the executing
+ * thread mirrors from `cancel`; the call returns when the route has finished.
+ * Errors as the reference's: foreign or null handles, endpoints outside the
+ * view, equal endpoints, a zero round budget or an unknown generator are
+ * internal invariants; a current device other than the upload device is a
+ * backend failure; a device budget below the deterministic route bytes is
+ * resource exhaustion.  The pending route holds the view's execution lease
+ * until it is released (a second route on the same view waits). */
+#define APGX_V1_ROUTE_RESULT_SCHEMA_VERSION 1u
+
+typedef struct apgx_v1_route_request {     /* BackendExecutionRequest (planar_router.h:326-334) */
+  uint32_t generator;                      /* APGX_V1_GENERATOR_* */
+  uint32_t start_node;
+  uint32_t goal_node;
+  uint32_t maximum_rounds;                 /* > 0 */
+  uint64_t routing_profile_fingerprint;
+  uint64_t maximum_device_bytes;
+} apgx_v1_route_request;
+
+typedef struct apgx_v1_route_header {      /* DeviceRouteResultHeaderV1, 64 bytes (planar_router.h:225-241) */
+  uint32_t schema_version;
+  uint32_t compiler_version;
+  uint32_t start_node;
+  uint32_t goal_node;
+  uint64_t source_board_content_hash;
+  uint64_t compiler_profile_fingerprint;
+  uint64_t routing_profile_fingerprint;
+  uint64_t rule_bucket_identity;
+  uint64_t device_view_fingerprint;
+  uint8_t generator;
+  uint8_t reserved[7];
+} apgx_v1_route_header;
+
+typedef struct apgx_v1_kernel_telemetry {  /* KernelTelemetry (planar_router.h:114-124) */
+  uint64_t persistent_device_bytes;
+  uint64_t batch_device_bytes;
+  uint64_t peak_device_bytes;
+  uint64_t examined_work;
+  uint64_t heading_turn_relaxations;
+  uint32_t rounds;
+  uint32_t reserved;
+  double kernel_milliseconds;
+} apgx_v1_kernel_telemetry;
+
+/* UntrustedKernelResult (planar_router.h:301-318) less its two arrays. */
+typedef struct apgx_v1_route_result {
+  apgx_v1_route_header header;             /* as the device wrote it */
+  uint32_t completion;                     /* APGX_V1_REACHED .. APGX_V1_CANCELLED */
+  uint32_t goal_state;                     /* reached: the lowest-index minimal goal state; else invalid */
+  uint64_t state_count;                    /* entries of the label and predecessor arrays */
+  apgx_v1_kernel_telemetry telemetry;
+} apgx_v1_route_result;
+
+typedef struct apgx_v1_route_pending apgx_v1_route_pending;
+
+int apgx_v1_route_execute(apgx_v1_view* view, const apgx_v1_route_request* request,
+                          const volatile uint32_t* cancel, apgx_v1_route_pending** out);
+/* Result, then every state's label (UINT64_MAX: unreachable) and predecessor
+ * (APGX_V1_INVALID_INDEX: none) when the arrays are non-NULL; they need
+ * `capacity` >= the view's represented states. */
+int apgx_v1_route_readback(apgx_v1_route_pending* pending, apgx_v1_route_result* result, uint64_t* labels,
+                           uint32_t* predecessors, uint64_t capacity);
+/* Replay/test-only readback decorator (FaultInjectingPlanarRouteBackend,
+ * src/gpu/fault_injecting_backend.cc:47-79): with the goal-predecessor fault a
+ * reached readback names its goal state as its own predecessor. */
+#define APGX_V1_ROUTE_FAULT_NONE 0u
+#define APGX_V1_ROUTE_FAULT_GOAL_PREDECESSOR_SELF_CYCLE 1u
+int apgx_v1_route_pending_inject_fault(apgx_v1_route_pending* pending, uint32_t fault);
+void apgx_v1_route_pending_release(apgx_v1_route_pending* pending);
+
+/* Deterministic device bytes of one route (ValidateKernelTelemetry's formula,
+ * src/gpu/planar_router.cc:411-454): the 64-byte result header, a u64 label
+ * and a u32 predecessor per state, then two u32 round marks per state and two
+ * u64 words (frontier) or eight u64 departures per node and two u64 words
+ * (sweep). */
+int apgx_v1_expected_route_bytes(uint32_t generator, uint64_t node_count, uint64_t* device_bytes);
+
+/* Host trust boundary of a single route, in RouteWithPreparedPlanarGpuBackend's
+ * order (src/gpu/planar_router.cc:1306-1364): kernel telemetry (:411-454), raw
+ * associations (:382-409), completion telemetry (:456-484), every predecessor
+ * (ValidateAllPredecessors, exact for reached and disconnected, a lower bound
+ * otherwise), then for a reached route the goal state (least finite goal
+ * label, lowest index on a tie) and the predecessor walk from it to the start
+ * (:829-930), for a disconnected one finite goal labels (:613-627) and an exact
+ * CPU search confirming the verdict (:629-667).  `out` as for
+ * apgx_v1_validate_batch (query_index 0); path_cost (may be NULL) gets the
+ * validated cost of a reached route (UINT64_MAX otherwise).  Returns APGX_OK
+ * unless an argument is null. */
+int apgx_v1_validate_route(const apgx_v1_header* header, const apgx_v1_node* nodes, uint64_t node_count,
+                           const apgx_v1_route_request* request, const apgx_v1_route_result* result,
+                           const uint64_t* labels, const uint32_t* predecessors, uint64_t* path_cost,
+                           apgx_v1_validation* out);
+
+/* ------------------------------------------------------------------------ */
 /* Candidate-batch invariant replays (gpu_batch_replay v1)                   */
 /* ------------------------------------------------------------------------ */
 
@@ -1080,6 +1194,42 @@ typedef struct apgx_replay_outcome {
  * APGX_OK unless an argument is null; the verdict is in `out`. */
 int apgx_batch_replay_run(const char* text, uint64_t text_length, const uint8_t* fixture, uint64_t fixture_length,
                           uint32_t flags, apgx_replay_outcome* out);
+
+/* ------------------------------------------------------------------------ */
+/* Single-route invariant replays (gpu_replay v1)                            */
+/* ------------------------------------------------------------------------ */
+/* GpuReplayArtifactV1 (tools/gpu_replay_parser.h:16-28): fixture, forced
+ * generator (bucketed frontier), the goal-predecessor fault, layer, round
+ * budget and the four expected associations; the only v1 contract is
+ * internal_invariant / gpu.predecessor.self_reference.v1. */
+typedef struct apgx_route_replay {
+  char fixture[256];
+  uint64_t fixture_checksum;
+  uint32_t generator;                      /* APGX_V1_GENERATOR_BUCKETED_FRONTIER */
+  uint32_t fault;                          /* APGX_V1_ROUTE_FAULT_GOAL_PREDECESSOR_SELF_CYCLE */
+  uint32_t layer;
+  uint32_t maximum_rounds;
+  uint64_t expected_board_content_hash;
+  uint64_t expected_profile_fingerprint;
+  uint64_t expected_routing_profile_fingerprint;
+  uint64_t expected_device_view_fingerprint;
+  char expected_invariant[64];
+} apgx_route_replay;
+
+/* ParseGpuReplayArtifactV1 (tools/gpu_replay_parser.cc:11-56): the canonical
+ * envelope of the batch artifacts with this format's 14 keys; APGX_E_UNSUPPORTED
+ * with the reference's message in `error` otherwise. */
+int apgx_route_replay_parse(const char* text, uint64_t length, apgx_route_replay* out, char* error,
+                            uint64_t error_capacity);
+/* RunGpuReplayArtifactV1 (tools/gpu_replay_runner.cc:40-124): parse, fixture
+ * checksum and layer, device-view compilation, the association gate (board,
+ * compiler profile, routing profile, device view; any mismatch is exit code 2
+ * before GPU work), then the route on the GPU with the fault decorator,
+ * validated by apgx_v1_validate_route; reproduced (exit 0) when it fails as an
+ * internal invariant with the expected identifier, else exit 1.  The fixture
+ * is the batch replays' (apgx_replay_fixture_header + nodes). */
+int apgx_route_replay_run(const char* text, uint64_t text_length, const uint8_t* fixture,
+                          uint64_t fixture_length, uint32_t flags, apgx_replay_outcome* out);
 
 /* ------------------------------------------------------------------------ */
 /* Synthetic lattice generators (deterministic xoshiro256** streams)         */

@@ -78,6 +78,20 @@ int orc_v1_frontier_workspace(const orc_v1_node* nodes, uint64_t n, const orc_v1
                               int64_t gy, uint32_t max_rounds, uint32_t* goal_state_out, uint32_t* rounds_out,
                               uint64_t* labels_out, uint32_t* preds_out);
 
+/* Single-route generators (generator 0 bucketed frontier, 1 heading-aware
+ * sweep) of the reference's ExecuteRoute on the profile's costs, then every
+ * state's lowest tight predecessor, flattened numbering.  completion: 0
+ * reached, 1 disconnected, 2 budget exhausted.  Pinned by the reference's
+ * published single-route rounds and examined work for both generators on the
+ * corpus (benchmarks/results/phase2_planar_bakeoff_a2b73bd.json), labels equal
+ * to an independent Dijkstra over the heading states at convergence, and a
+ * hand-worked corridor (rounds, examined work, turn relaxations, labels,
+ * predecessors). */
+int orc_v1_route(const orc_v1_node* nodes, uint64_t n, const orc_v1_profile* pr, uint32_t generator, uint32_t layer,
+                 int64_t sx, int64_t sy, int64_t gx, int64_t gy, uint32_t max_rounds, uint32_t* completion_out,
+                 uint32_t* rounds_out, uint64_t* examined_out, uint64_t* turns_out, uint32_t* goal_state_out,
+                 uint64_t* labels_out, uint32_t* preds_out);
+
 #ifdef __cplusplus
 }
 #endif

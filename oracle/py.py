@@ -231,6 +231,10 @@ def _v1_setup():
     h.orc_v1_frontier_workspace.argtypes = [P(V1Node), C.c_uint64, P(V1Profile), P(V1Policy), C.c_uint32,
                                             C.c_int64, C.c_int64, C.c_int64, C.c_int64, C.c_uint32,
                                             P(C.c_uint32), P(C.c_uint32), P(C.c_uint64), P(C.c_uint32)]
+    h.orc_v1_route.restype = C.c_int
+    h.orc_v1_route.argtypes = [P(V1Node), C.c_uint64, P(V1Profile), C.c_uint32, C.c_uint32, C.c_int64, C.c_int64,
+                               C.c_int64, C.c_int64, C.c_uint32, P(C.c_uint32), P(C.c_uint32), P(C.c_uint64),
+                               P(C.c_uint64), P(C.c_uint32), P(C.c_uint64), P(C.c_uint32)]
     h._v1_ready = True
     return h
 
@@ -324,3 +328,23 @@ def v1_frontier_workspace(o: "V1", start, goal, policy=None, layer=0, max_rounds
         raise RuntimeError("oracle v1 frontier workspace error")
     return {"status": rc, "goal_state": goal_state.value, "rounds": rounds.value, "labels": labels,
             "predecessors": preds}
+
+
+def v1_route(o: "V1", start, goal, generator, layer=0, max_rounds=100_000):
+    """Single-route generator restatement (generator 0 frontier, 1 sweep):
+    dict(completion 0 reached / 1 disconnected / 2 budget, rounds,
+    examined_work, turns, goal_state, labels, predecessors), flattened
+    numbering."""
+    import numpy as np
+    h = _v1_setup()
+    comp, rounds, gs = C.c_uint32(0), C.c_uint32(0), C.c_uint32(0)
+    examined, turns = C.c_uint64(0), C.c_uint64(0)
+    labels = np.zeros(9 * o.n, np.uint64)
+    preds = np.zeros(9 * o.n, np.uint32)
+    rc = h.orc_v1_route(o.nodes, o.n, C.byref(o.profile), generator, layer, start[0], start[1], goal[0], goal[1],
+                        max_rounds, C.byref(comp), C.byref(rounds), C.byref(examined), C.byref(turns), C.byref(gs),
+                        labels.ctypes.data_as(C.POINTER(C.c_uint64)), preds.ctypes.data_as(C.POINTER(C.c_uint32)))
+    if rc < 0:
+        raise RuntimeError("oracle v1 route error")
+    return {"completion": comp.value, "rounds": rounds.value, "examined_work": examined.value,
+            "turns": turns.value, "goal_state": gs.value, "labels": labels, "predecessors": preds}

@@ -575,3 +575,185 @@ int orc_v1_frontier_workspace(const orc_v1_node* nodes, uint64_t n, const orc_v1
   free(states);
   return rc;
 }
+
+/* Single-route generators of the reference's ExecuteRoute in flattened
+ * numbering, with the profile's step and bend costs only (no candidate
+ * policy).  Frontier: RunFrontier / FrontierRelax
+ * (gpu/cuda/planar_router.cu:363-412, 1552-1640), delta-stepping over heading
+ * states with bucket width min(orth, diag); a state is active in a round when
+ * the previous round deferred it or lowered its label; active states in the
+ * bucket relax every legal direction (each counted as examined work), the
+ * others are deferred; converged when nothing is deferred or lowered.  Sweep:
+ * RunSweep / ComputeDepartureLabels / SweepRuns (:414-484, :1642-1722),
+ * departures then segmented scans along every maximal directed run; examined
+ * work is the legal directed edges per round, a turn relaxation is counted for
+ * each departure whose final minimum came from another heading; converged
+ * when a round lowers no label.  Then SelectStablePredecessors (:486-524).
+ * Sequential emulation of a round equals the parallel one: every candidate
+ * made inside a bucket lies above it, and every run writes its own labels. */
+static uint64_t route_step(const orc_v1_profile* pr, int dir, int incoming) {
+  uint64_t c = (dir & 1) ? pr->diag : pr->orth;
+  if (incoming != 8 && incoming != dir) c += pr->bend;
+  return c;
+}
+
+int orc_v1_route(const orc_v1_node* nodes, uint64_t n, const orc_v1_profile* pr, uint32_t generator, uint32_t layer,
+                 int64_t sx, int64_t sy, int64_t gx, int64_t gy, uint32_t max_rounds, uint32_t* completion_out,
+                 uint32_t* rounds_out, uint64_t* examined_out, uint64_t* turns_out, uint32_t* goal_state_out,
+                 uint64_t* labels_out, uint32_t* preds_out) {
+  uint32_t* order = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+  orc_v1_node* flat = (orc_v1_node*)malloc(sizeof(orc_v1_node) * (n ? n : 1));
+  if (!order || !flat || orc_v1_flatten_order(nodes, n, pr->tile_w, pr->tile_h, order)) {
+    free(order); free(flat); return -1;
+  }
+  for (uint64_t i = 0; i < n; ++i) flat[i] = nodes[order[i]];
+  lookup_t lk;
+  if (lookup_init(&lk, flat, n)) { free(order); free(flat); return -1; }
+  const int64_t s = lookup(&lk, layer, sx, sy), g = lookup(&lk, layer, gx, gy);
+  const uint64_t S = n * 9;
+  uint64_t* L = (uint64_t*)malloc(sizeof(uint64_t) * (S ? S : 1));
+  uint64_t* dep = (uint64_t*)malloc(sizeof(uint64_t) * (n ? n * 8 : 1));
+  int64_t* nb = (int64_t*)malloc(sizeof(int64_t) * (n ? n * 8 : 1));
+  uint8_t* cur = (uint8_t*)calloc(S ? S : 1, 1);
+  uint8_t* nxt = (uint8_t*)calloc(S ? S : 1, 1);
+  int rc = 0;
+  uint32_t rounds = 0, completion = 2;
+  uint64_t examined = 0, turns = 0, legal = 0;
+  if (s < 0 || g < 0 || s == g || !L || !dep || !nb || !cur || !nxt) { rc = -1; goto done; }
+  for (uint64_t u = 0; u < n; ++u)
+    for (int d = 0; d < 8; ++d) {
+      nb[u * 8 + d] = -1;
+      if (!(flat[u].mask & (1u << d))) continue;
+      nb[u * 8 + d] = lookup(&lk, layer, flat[u].x + DX[d], flat[u].y + DY[d]);
+      if (nb[u * 8 + d] < 0) { rc = -1; goto done; }
+      ++legal;
+    }
+  for (uint64_t i = 0; i < S; ++i) L[i] = INF64;
+  const uint64_t start_state = (uint64_t)s * 9 + 8;
+  L[start_state] = 0;
+  int converged = 0;
+  if (generator == 0) {
+    const uint64_t delta = pr->orth < pr->diag ? pr->orth : pr->diag;
+    if (delta == 0) { rc = -1; goto done; }
+    uint64_t bucket_upper = 0;
+    cur[start_state] = 1;
+    while (rounds < max_rounds) {
+      uint64_t next_min = INF64;
+      memset(nxt, 0, S);
+      for (uint64_t st = 0; st < S; ++st) {
+        if (!cur[st] || L[st] == INF64) continue;
+        const uint64_t label = L[st];
+        if (label > bucket_upper) {
+          nxt[st] = 1;
+          if (label < next_min) next_min = label;
+          continue;
+        }
+        const uint64_t u = st / 9;
+        const int inc = (int)(st % 9);
+        for (int d = 0; d < 8; ++d) {
+          if (nb[u * 8 + d] < 0) continue;
+          ++examined;
+          uint64_t c;
+          if (checked_add(label, route_step(pr, d, inc), &c)) continue;
+          const uint64_t t = (uint64_t)nb[u * 8 + d] * 9 + d;
+          if (c < L[t]) {
+            L[t] = c;
+            nxt[t] = 1;
+            if (c < next_min) next_min = c;
+          }
+        }
+      }
+      ++rounds;
+      if (next_min == INF64) { converged = 1; break; }
+      const uint64_t base = (next_min / delta) * delta;
+      bucket_upper = base > INF64 - (delta - 1) ? INF64 : base + delta - 1;
+      uint8_t* tmp = cur; cur = nxt; nxt = tmp;
+    }
+  } else {
+    while (rounds < max_rounds) {
+      for (uint64_t u = 0; u < n; ++u)
+        for (int d = 0; d < 8; ++d) {
+          uint64_t best = (int64_t)u == s ? 0 : L[u * 9 + d];
+          int used_turn = 0;
+          for (int inc = 0; inc < 8; ++inc) {
+            const uint64_t lab = L[u * 9 + inc];
+            uint64_t c;
+            if (lab == INF64 || checked_add(lab, inc == d ? 0 : pr->bend, &c)) continue;
+            if (c < best) { best = c; used_turn = inc != d; }
+          }
+          dep[u * 8 + d] = best;
+          turns += used_turn ? 1 : 0;
+        }
+      int changed = 0;
+      for (int d = 0; d < 8; ++d) {
+        const int opp = (d + 4) % 8;
+        for (uint64_t u = 0; u < n; ++u) {
+          if (nb[u * 8 + d] < 0) continue;
+          const int64_t before = nb[u * 8 + opp] >= 0 ? nb[u * 8 + opp] : lookup(&lk, layer, flat[u].x - DX[d], flat[u].y - DY[d]);
+          if (before >= 0 && nb[(uint64_t)before * 8 + d] == (int64_t)u) continue; /* not a run start */
+          uint64_t carry = INF64, v = u;
+          while (nb[v * 8 + d] >= 0) {
+            const uint64_t t = (uint64_t)nb[v * 8 + d];
+            if (dep[v * 8 + d] < carry) carry = dep[v * 8 + d];
+            uint64_t c;
+            if (!checked_add(carry, (d & 1) ? pr->diag : pr->orth, &c)) {
+              if (c < L[t * 9 + d]) { L[t * 9 + d] = c; changed = 1; }
+              carry = c < dep[t * 8 + d] ? c : dep[t * 8 + d];
+            } else {
+              carry = dep[t * 8 + d];
+            }
+            v = t;
+          }
+        }
+      }
+      ++rounds;
+      examined += legal;
+      if (!changed) { converged = 1; break; }
+    }
+  }
+  if (converged) {
+    completion = 1;
+    for (int h = 0; h < 8; ++h)
+      if (L[(uint64_t)g * 9 + h] != INF64) completion = 0;
+  }
+  if (goal_state_out) {
+    *goal_state_out = UINT32_MAX;
+    if (completion == 0) {
+      uint64_t best = INF64;
+      for (int h = 0; h < 8; ++h) {
+        const uint64_t st = (uint64_t)g * 9 + h;
+        if (L[st] < best) { best = L[st]; *goal_state_out = (uint32_t)st; }
+      }
+    }
+  }
+  for (uint64_t st = 0; st < S; ++st) {
+    if (labels_out) labels_out[st] = L[st];
+    if (!preds_out) continue;
+    preds_out[st] = UINT32_MAX;
+    const int in = (int)(st % 9);
+    if (st == start_state || in == 8 || L[st] == INF64) continue;
+    const uint64_t v = st / 9;
+    const int opp = (in + 4) % 8;
+    const int64_t p = lookup(&lk, layer, flat[v].x + DX[opp], flat[v].y + DY[opp]);
+    if (p < 0) continue;
+    uint64_t chosen = INF64;
+    for (int ph = 0; ph < 9; ++ph) {
+      if (ph == 8 && p != s) continue;
+      const uint64_t ps = (uint64_t)p * 9 + ph;
+      uint64_t c;
+      if (L[ps] == INF64 || checked_add(L[ps], route_step(pr, in, ph), &c)) continue;
+      if (c == L[st] && ps < chosen) chosen = ps;
+    }
+    if (chosen != INF64) preds_out[st] = (uint32_t)chosen;
+  }
+done:
+  if (completion_out) *completion_out = completion;
+  if (rounds_out) *rounds_out = rounds;
+  if (examined_out) *examined_out = examined;
+  if (turns_out) *turns_out = turns;
+  free(L); free(dep); free(nb); free(cur); free(nxt);
+  free(lk.keys);
+  free(order);
+  free(flat);
+  return rc;
+}

@@ -1,4 +1,4 @@
-"""Writes the candidate-batch replay artifacts under tests/golden/replays/.
+"""Writes the replay artifacts under tests/golden/replays/.
 
 One fixture -- the compiled high_turn_maze corpus case (profile scalars,
 request endpoints, represented nodes with legal masks; see
@@ -7,7 +7,9 @@ schemas/gpu_batch_replay/v1.md that this library can reproduce, one canonical
 artifact naming it, plus two negatives: a checksum-valid artifact that pairs a
 compact-path fault with the frontier (rejected by the parser's association
 rule) and one whose routing-profile fingerprint differs from the fixture's
-(rejected by the association gate before any GPU work).
+(rejected by the association gate before any GPU work).  The single-route
+format (gpu_replay v1) gets its one contract, the frontier's goal-predecessor
+self cycle, and the same routing-profile negative.
 
 The corpus case has no Board IR, so its board content hash, rule-bucket
 identity and routed-net profile (net 1, generation 1, width 200000, clearance
@@ -90,6 +92,23 @@ def artifact(fixture_path, fixture_sum, generator, fault, invariant, k, header, 
     return payload + f"checksum_fnv1a64={vc.stable_hash_string(payload)}\n".encode()
 
 
+def route_artifact(fixture_path, fixture_sum, header, routing_fp):
+    """A single-route gpu_replay v1 artifact (tools/gpu_replay_parser.cc:13-29):
+    the frontier with the goal-predecessor self cycle."""
+    fields = [
+        ("format", "apgar_gpu_invariant_replay"), ("schema_version", 1), ("fixture", fixture_path),
+        ("fixture_checksum_fnv1a64", fixture_sum), ("generator", "bucketed_frontier"),
+        ("fault", "goal_predecessor_self_cycle"), ("layer", 0), ("maximum_rounds", 100000),
+        ("expected_board_content_hash", header.source_board_content_hash),
+        ("expected_profile_fingerprint", header.compiler_profile_fingerprint),
+        ("expected_routing_profile_fingerprint", routing_fp),
+        ("expected_device_view_fingerprint", header.device_view_fingerprint),
+        ("expected_failure", "internal_invariant"), ("expected_invariant", "gpu.predecessor.self_reference.v1"),
+    ]
+    payload = "".join(f"{key}={value}\n" for key, value in fields).encode()
+    return payload + f"checksum_fnv1a64={vc.stable_hash_string(payload)}\n".encode()
+
+
 def policy_identity():
     h = vc.Fnv1a()
     h.string("APGAR-CANDIDATE-POLICY-V1")
@@ -120,6 +139,9 @@ def main():
     (OUT / "batch_query_identity_wrong_routing_profile_v1.replay").write_bytes(
         artifact(fx_path, fx_sum, v1.GENERATOR_FRONTIER, "query_identity", CONTRACT["query_identity"][1],
                  len(CONTRACT) + 1, header, routing_fp + 1, pid))
+    (OUT / "goal_predecessor_self_cycle_v1.replay").write_bytes(route_artifact(fx_path, fx_sum, header, routing_fp))
+    (OUT / "goal_predecessor_self_cycle_wrong_routing_profile_v1.replay").write_bytes(
+        route_artifact(fx_path, fx_sum, header, routing_fp + 1))
 
 
 if __name__ == "__main__":

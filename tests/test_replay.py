@@ -1,4 +1,5 @@
-"""Candidate-batch invariant replays (apgx_batch_replay_*).
+"""Invariant replays: candidate batches (apgx_batch_replay_*) and single
+routes (apgx_route_replay_*).
 
 CPU: the parser against the reference's own sixteen checked-in artifacts
 (tests/golden/reference_replays: checksums, grammar, the generator/fault
@@ -135,12 +136,88 @@ def test_committed_artifacts_are_current(apgx):
 
 def test_our_artifacts_pass_the_association_gate(apgx):
     todo, parse_rejected, gate_rejected = replay.default_artifacts()
-    assert len(todo) == 12 and len(parse_rejected) == 1 and len(gate_rejected) == 1
+    assert len(todo) == 13 and len(parse_rejected) == 1 and len(gate_rejected) == 2
     for p in todo:
         o = replay.run(p, dry_run=True)
         assert (o.exit_code, o.stage_name) == (0, "association"), (p.name, o.message)
     assert replay.reject(parse_rejected[0]) == 0
-    assert replay.reject_at_gate(gate_rejected[0]) == 0
+    for p in gate_rejected:
+        assert replay.reject_at_gate(p) == 0
+
+
+# ---------------------------------------------------------- gpu_replay v1 (single route)
+ROUTE_BASE = (GOLDEN / "reference_replays" / "goal_predecessor_self_cycle_v1.replay").read_bytes()
+ROUTE_ASSOC = "replay format, schema, generator, fault, or expected outcome is unsupported"
+
+
+def test_reference_route_artifacts_parse(apgx):
+    """The reference's two gpu_replay artifacts: every field as it wrote it."""
+    art, err = replay.parse_route(ROUTE_BASE)
+    assert art is not None, err
+    assert art.fixture.decode() == "tests/fixtures/m1_exactness.kicad_pcb"
+    assert art.fixture_checksum == 15781378684595989985
+    assert (art.generator, art.fault, art.layer, art.maximum_rounds) == (0, 1, 0, 100000)
+    assert art.expected_board_content_hash == 1860510029311497781
+    assert art.expected_profile_fingerprint == 8851933838860818274
+    assert art.expected_routing_profile_fingerprint == 1820138087830637135
+    assert art.expected_device_view_fingerprint == 2764614999745307367
+    assert art.expected_invariant.decode() == "gpu.predecessor.self_reference.v1"
+    wrong, err = replay.parse_route((GOLDEN / "reference_replays" /
+                                     "goal_predecessor_self_cycle_wrong_routing_profile_v1.replay").read_bytes())
+    assert wrong is not None, err
+    assert wrong.expected_routing_profile_fingerprint != art.expected_routing_profile_fingerprint
+    # Batch artifacts are another format for this parser.
+    batch, err = replay.parse_route((GOLDEN / "reference_replays" / "batch_workspace_owner_v1.replay").read_bytes())
+    assert batch is None and err == "replay fields are missing or out of canonical order"
+
+
+@pytest.mark.parametrize("mutate, message", [
+    (lambda t: t.replace(b"generator=bucketed_frontier\n", b"generator=heading_aware_sweep\n"), ROUTE_ASSOC),
+    (lambda t: t.replace(b"fault=goal_predecessor_self_cycle\n", b"fault=none\n"), ROUTE_ASSOC),
+    (lambda t: t.replace(b"schema_version=1\n", b"schema_version=2\n"), ROUTE_ASSOC),
+    (lambda t: t.replace(b"expected_failure=internal_invariant\n", b"expected_failure=unsupported\n"), ROUTE_ASSOC),
+    (lambda t: t.replace(b"maximum_rounds=100000\n", b"maximum_rounds=0\n"), "replay contains an invalid integer field"),
+    (lambda t: t.replace(b"layer=0\n", b"layer=01\n"), "replay contains an invalid integer field"),
+    (lambda t: t.replace(b"layer=0\n", b"layer=4294967296\n"), "replay contains an invalid integer field"),
+    (lambda t: t.replace(b"layer=0\nmaximum_rounds", b"maximum_rounds"), "replay fields are missing or out of canonical order"),
+    (lambda t: t + b"extra=1\n", "replay payload has unknown fields"),
+])
+def test_route_grammar_rejections(apgx, mutate, message):
+    art, err = replay.parse_route(resign(mutate(payload_of(ROUTE_BASE))))
+    assert art is None and err == message
+
+
+def test_route_checksum_and_fixture(apgx):
+    art, err = replay.parse_route(ROUTE_BASE.replace(b"layer=0", b"layer=1"))
+    assert art is None and err == "replay payload checksum mismatch"
+    o = replay.run(GOLDEN / "reference_replays" / "goal_predecessor_self_cycle_v1.replay", dry_run=True)
+    assert (o.exit_code, o.stage_name) == (2, "fixture")
+    assert o.message.decode() == "fixture is missing or its checksum differs"
+
+
+def test_route_association_gate_names_each_mismatch(apgx):
+    text = (GOLDEN / "replays" / "goal_predecessor_self_cycle_v1.replay").read_bytes()
+    fixture = (GOLDEN / "replays" / "high_turn_maze_v1.fixture").read_bytes()
+    o = replay.run_bytes(text, fixture, dry_run=True)
+    assert (o.exit_code, o.stage_name) == (0, "association")
+    art, _ = replay.parse_route(text)
+    for field, name in (("expected_board_content_hash", "board"), ("expected_profile_fingerprint", "compiler_profile"),
+                        ("expected_routing_profile_fingerprint", "routing_profile"),
+                        ("expected_device_view_fingerprint", "device_view")):
+        value = getattr(art, field)
+        drifted = payload_of(text).replace(f"{field}={value}\n".encode(), f"{field}={value ^ 1}\n".encode())
+        o = replay.run_bytes(resign(drifted), fixture, dry_run=True)
+        assert (o.exit_code, o.stage_name) == (2, "association")
+        assert o.message.decode() == f"replay semantic association mismatch: {name}"
+
+
+def test_route_artifacts_are_current(apgx):
+    from tests.golden import make_replays as mk
+
+    data, header, routing_fp = mk.fixture()
+    want = mk.route_artifact("tests/golden/replays/high_turn_maze_v1.fixture", vc.stable_hash_string(data), header,
+                             routing_fp)
+    assert (GOLDEN / "replays" / "goal_predecessor_self_cycle_v1.replay").read_bytes() == want
 
 
 def test_fixture_tampering_is_rejected(apgx):
@@ -163,7 +240,9 @@ def test_every_artifact_reproduces_on_the_gpu(apgx, gpu):
     print("[case] candidate-batch replays", flush=True)
     todo, _, _ = replay.default_artifacts()
     for p in todo:
-        art, _ = replay.parse(p.read_bytes())
+        text = p.read_bytes()
+        art, _ = replay.parse_route(text) if replay.is_route_artifact(text) else replay.parse(text)
+        print(f"[case] replay {p.name}", flush=True)
         o = replay.run(p)
         assert o.exit_code == 0 and o.stage_name == "reproduce", (p.name, o.message)
         assert o.invariant == art.expected_invariant
