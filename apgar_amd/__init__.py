@@ -245,6 +245,8 @@ _SIGNATURES = {
     "apgx_price_update": (C.c_int, [C.c_uint64, _P(PricePolicy), _P(PriceLimits), C.c_uint64, _P(C.c_uint32),
                                     _P(C.c_uint8), _P(C.c_uint32), _P(C.c_uint32), _P(C.c_uint32),
                                     _P(PriceEpoch)]),
+    "apgx_price_update_no_alloc": (C.c_int, [C.c_uint64, _P(PricePolicy), _P(PriceLimits), C.c_uint64, C.c_void_p,
+                                             C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, _P(PriceEpoch)]),
     "apgx_prior_create": (C.c_int, [_P(PriorConfig), _P(C.c_void_p)]),
     "apgx_prior_release": (None, [C.c_void_p]),
     "apgx_prior_param_count": (C.c_uint64, [C.c_void_p]),
@@ -743,6 +745,18 @@ def price_update(epoch: int, usage: np.ndarray, history: np.ndarray, capacity=No
                                   u32p(h), u32p(present), u32p(total), C.byref(out)))
     summary = {k: getattr(out, k) for k, _ in PriceEpoch._fields_ if k != "reserved"}
     return summary, present, h, total
+
+
+def price_update_no_alloc(epoch: int, cells: int, usage, history, capacity=None, present=None, total=None,
+                          **settings):
+    """The no-allocation price epoch (apgx_price_update_no_alloc) on DEVICE
+    planes given as integer device addresses (history updated in place).
+    Returns the epoch summary dict."""
+    pol, lim = _price_policy(**settings)
+    out = PriceEpoch()
+    check(lib().apgx_price_update_no_alloc(epoch, C.byref(pol), C.byref(lim), cells, usage, capacity, history,
+                                           present, total, C.byref(out)))
+    return {k: getattr(out, k) for k, _ in PriceEpoch._fields_ if k != "reserved"}
 
 
 def _price_policy(**settings):

@@ -18,14 +18,17 @@
 // of a snapshot.
 //
 // Kernels: one HBM pass computes per-cell values and per-block partials
-// (counts, clamped sums, first failing cells); the host walks the block
-// partials in order to find where a running count or sum first crosses its
-// bound, a one-block kernel pins the exact cell inside that block, and a
-// second pass writes the planes when every check passes.
+// (counts, clamped sums, first failing cells); one workgroup then walks the
+// block partials in the reference's check order (totals, block-wide scans to
+// find where a running count or sum first crosses its bound, the exact cell
+// inside that block) and records the outcome; a second pass writes the planes
+// when every check passed.  The three launches run back to back and the host
+// reads one 64-byte outcome record at the end.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "apgx.h"
@@ -133,49 +136,217 @@ __global__ __launch_bounds__(kThreads) void PriceScan(Params p, const uint32_t* 
   }
 }
 
-// One thread walks one block's cells in order: result[0] = the cell where the
-// running count of new entries reaches `new_target` (kNone if never), result[1]
-// = the cell where `agg_start` plus the running total exceeds `agg_bound` or
-// overflows, result[2] = 1 if that was an overflow, result[3] = the arithmetic
-// outcome of cell `arith_cell` (if inside this block).
-__global__ void PriceLocate(Params p, const uint32_t* __restrict__ usage, const uint8_t* __restrict__ cap,
-                            const uint32_t* __restrict__ hist, uint64_t block, uint64_t new_target,
-                            uint64_t agg_start, uint64_t agg_bound, uint64_t arith_cell,
-                            unsigned long long* __restrict__ result) {
-  if (threadIdx.x != 0) return;
-  uint64_t seen_new = 0, agg = agg_start;
-  unsigned long long r0 = kNone, r1 = kNone, r2 = 0, r3 = 0;
+struct Limits {
+  uint64_t max_entries, max_hot, max_agg;
+};
+
+// Outcome of an epoch, written by PriceDecide (64 bytes).
+struct Decision {
+  unsigned long long outcome;    // APGX_PRICE_*
+  unsigned long long cell;       // failing cell or kNone
+  unsigned long long hot;        // overused cells
+  unsigned long long entries;    // price entries after the update
+  unsigned long long aggregate;  // sum of totals (passing epochs)
+  unsigned long long pass;       // 1: every check passed and PriceWrite writes the planes
+  unsigned long long pad[2];
+};
+static_assert(sizeof(Decision) == 64, "outcome record");
+
+// The cell of the `target`-th (0-based) new entry of one block, in cell order.
+__device__ uint64_t LocateNewEntry(const Params& p, const uint32_t* usage, const uint8_t* cap, const uint32_t* hist,
+                                   uint64_t block, uint64_t target) {
+  uint64_t seen = 0;
+  const uint64_t begin = block * kBlockCells, end = min(p.cells, begin + kBlockCells);
+  for (uint64_t c = begin; c < end; ++c) {
+    if (!Evaluate(p, usage[c], CapAt(cap, c), hist[c]).new_entry) continue;
+    if (seen == target) return c;
+    ++seen;
+  }
+  return kNone;
+}
+
+// The entry of one block, in cell order, at which `agg` plus the running
+// total first overflows (*overflow) or exceeds `bound`.
+__device__ uint64_t LocateAggregate(const Params& p, const uint32_t* usage, const uint8_t* cap, const uint32_t* hist,
+                                    uint64_t block, uint64_t agg, uint64_t bound, bool* overflow) {
   const uint64_t begin = block * kBlockCells, end = min(p.cells, begin + kBlockCells);
   for (uint64_t c = begin; c < end; ++c) {
     const CellEval e = Evaluate(p, usage[c], CapAt(cap, c), hist[c]);
-    if (c == arith_cell) r3 = e.arith;
-    if (e.new_entry) {
-      if (r0 == kNone && seen_new == new_target) r0 = c;
-      ++seen_new;
+    if (!e.entry) continue;
+    uint64_t next = 0;
+    if (__builtin_add_overflow(agg, e.total, &next)) {
+      *overflow = true;
+      return c;
     }
-    if (e.entry && r1 == kNone) {
-      uint64_t next = 0;
-      if (__builtin_add_overflow(agg, e.total, &next)) {
-        r1 = c;
-        r2 = 1;
-      } else {
-        agg = next;
-        if (agg > agg_bound) r1 = c;
-      }
-    }
+    agg = next;
+    if (agg > bound) return c;
   }
-  result[0] = r0;
-  result[1] = r1;
-  result[2] = r2;
-  result[3] = r3;
+  return kNone;
 }
 
+constexpr int kDecideThreads = 1024;
+
+// Exclusive prefix sums of s[0 .. kDecideThreads) in place; returns the
+// total (thread 0; the callers synchronise around it).
+__device__ unsigned __int128 ExclusiveScan(unsigned __int128* s) {
+  unsigned __int128 run = 0;
+  for (int i = 0; i < kDecideThreads; ++i) {
+    const unsigned __int128 v = s[i];
+    s[i] = run;
+    run += v;
+  }
+  return run;
+}
+
+// The epoch's checks in the reference's order (negotiated_regeneration_plan.cc
+// :690-790), over the block partials of PriceScan: the hot-resource bound;
+// then, over the overused cells in cell order, the entry bound (the cell where
+// prior entries + new entries so far reach the bound) against the first
+// arithmetic overflow; then, over the entries in cell order, the first value
+// bound against the first crossing of the aggregate bound.  Each thread owns
+// a contiguous run of blocks; a running count or sum crosses its bound in
+// exactly one run, found from a scan of the per-thread sums, whose owner finds
+// the block, and thread 0 walks that block for the cell.
+__global__ __launch_bounds__(kDecideThreads) void PriceDecide(Params p, Limits lim, const uint32_t* __restrict__ usage,
+                                                              const uint8_t* __restrict__ cap,
+                                                              const uint32_t* __restrict__ hist,
+                                                              const BlockPartial* __restrict__ part, uint64_t blocks,
+                                                              Decision* __restrict__ out) {
+  __shared__ unsigned long long s_over, s_entries, s_prior, s_fa, s_fv;
+  __shared__ unsigned __int128 s_scan[kDecideThreads];
+  __shared__ unsigned long long s_hit, s_hit_before, s_entry_cell, s_done;
+  const int t = threadIdx.x;
+  const uint64_t chunk = (blocks + kDecideThreads - 1) / kDecideThreads;
+  const uint64_t b0 = min(blocks, static_cast<uint64_t>(t) * chunk), b1 = min(blocks, b0 + chunk);
+  if (t == 0) {
+    s_over = s_entries = s_prior = 0;
+    s_fa = s_fv = kNone;
+    s_entry_cell = kNone;
+    s_done = 0;
+  }
+  __syncthreads();
+  unsigned long long over = 0, ent = 0, prior = 0, fa = kNone, fv = kNone;
+  for (uint64_t i = b0; i < b1; ++i) {
+    const BlockPartial q = part[i];
+    over += q.over_count;
+    ent += q.entry_count;
+    prior += q.entry_count - q.new_count;
+    fa = min(fa, q.first_arith);
+    fv = min(fv, q.first_value);
+  }
+  atomicAdd(&s_over, over);
+  atomicAdd(&s_entries, ent);
+  atomicAdd(&s_prior, prior);
+  atomicMin(&s_fa, fa);
+  atomicMin(&s_fv, fv);
+  __syncthreads();
+  const uint64_t total_over = s_over, entries = s_entries, prior_entries = s_prior;
+  const uint64_t first_arith = s_fa, first_value = s_fv;
+  Decision d{};
+  d.cell = kNone;
+  d.hot = total_over;
+  d.entries = entries;
+  // 1. Hot-resource bound, before any price.
+  if (total_over > lim.max_hot) {
+    if (t == 0) {
+      d.outcome = APGX_PRICE_HOT_RESOURCE_BOUND;
+      *out = d;
+    }
+    return;
+  }
+  // 2. Entry bound: the k-th new entry with prior + k reaching the bound.
+  const uint64_t new_entries = entries - prior_entries;
+  if (new_entries > 0 && prior_entries + new_entries > lim.max_entries) {
+    const uint64_t k = lim.max_entries > prior_entries ? lim.max_entries - prior_entries : 0;
+    unsigned long long mine = 0;
+    for (uint64_t i = b0; i < b1; ++i) mine += part[i].new_count;
+    s_scan[t] = mine;
+    if (t == 0) s_hit = kNone;
+    __syncthreads();
+    if (t == 0) ExclusiveScan(s_scan);
+    __syncthreads();
+    const unsigned __int128 before = s_scan[t];
+    if (before <= k && k < before + mine) {
+      uint64_t seen = static_cast<uint64_t>(before);
+      for (uint64_t i = b0; i < b1; ++i) {
+        if (seen + part[i].new_count > k) {
+          s_hit = i;
+          s_hit_before = seen;
+          break;
+        }
+        seen += part[i].new_count;
+      }
+    }
+    __syncthreads();
+    if (t == 0 && s_hit != kNone) s_entry_cell = LocateNewEntry(p, usage, cap, hist, s_hit, k - s_hit_before);
+    __syncthreads();
+  }
+  // 3. The entry bound or the first overflowing overused cell, whichever comes first.
+  if (t == 0) {
+    const uint64_t entry_cell = s_entry_cell;
+    if (entry_cell != kNone && entry_cell <= first_arith) {
+      d.outcome = APGX_PRICE_ENTRY_BOUND;
+      d.cell = entry_cell;
+      s_done = 1;
+    } else if (first_arith != kNone) {
+      d.outcome = Evaluate(p, usage[first_arith], CapAt(cap, first_arith), hist[first_arith]).arith;
+      d.cell = first_arith;
+      s_done = 1;
+    }
+  }
+  __syncthreads();
+  if (s_done) {
+    if (t == 0) *out = d;
+    return;
+  }
+  // 4. Entries in cell order: the running aggregate's first crossing.
+  unsigned __int128 mine = 0;
+  for (uint64_t i = b0; i < b1; ++i) mine += part[i].sum_total;
+  s_scan[t] = mine;
+  if (t == 0) s_hit = kNone;
+  __syncthreads();
+  unsigned __int128 sum = 0;  // the sum of every entry's total (thread 0)
+  if (t == 0) sum = ExclusiveScan(s_scan);
+  __syncthreads();
+  const unsigned __int128 before = s_scan[t];
+  if (before <= lim.max_agg && before + mine > lim.max_agg) {
+    unsigned __int128 seen = before;
+    for (uint64_t i = b0; i < b1; ++i) {
+      if (seen + part[i].sum_total > lim.max_agg) {
+        s_hit = i;
+        s_hit_before = static_cast<uint64_t>(seen);
+        break;
+      }
+      seen += part[i].sum_total;
+    }
+  }
+  __syncthreads();
+  if (t != 0) return;
+  uint64_t agg_cell = kNone;
+  bool overflow = false;
+  if (s_hit != kNone) agg_cell = LocateAggregate(p, usage, cap, hist, s_hit, s_hit_before, lim.max_agg, &overflow);
+  if (first_value != kNone && first_value <= agg_cell) {
+    d.outcome = APGX_PRICE_VALUE_BOUND;
+    d.cell = first_value;
+  } else if (agg_cell != kNone) {
+    d.outcome = overflow ? APGX_PRICE_AGGREGATE_OVERFLOW : APGX_PRICE_AGGREGATE_BOUND;
+    d.cell = agg_cell;
+  } else {
+    d.outcome = APGX_PRICE_OK;
+    d.aggregate = static_cast<uint64_t>(sum);
+    d.pass = 1;
+  }
+  *out = d;
+}
+
+// Writes the planes of an epoch whose checks passed (a failing epoch leaves
+// them untouched).
 __global__ __launch_bounds__(kThreads) void PriceWrite(Params p, const uint32_t* __restrict__ usage,
                                                        const uint8_t* __restrict__ cap,
                                                        uint32_t* __restrict__ hist, uint32_t* __restrict__ present,
-                                                       uint32_t* __restrict__ total) {
+                                                       uint32_t* __restrict__ total, const Decision* __restrict__ d) {
   const uint64_t c = blockIdx.x * static_cast<uint64_t>(kThreads) + threadIdx.x;
-  if (c >= p.cells) return;
+  if (c >= p.cells || d->pass == 0) return;
   const CellEval e = Evaluate(p, usage[c], CapAt(cap, c), hist[c]);
   hist[c] = static_cast<uint32_t>(e.history);
   if (present) present[c] = static_cast<uint32_t>(e.present);
@@ -189,14 +360,65 @@ struct Buffers {
   uint32_t* present = nullptr;
   uint32_t* total = nullptr;
   BlockPartial* part = nullptr;
-  unsigned long long* result = nullptr;
+  Decision* decision = nullptr;
   ~Buffers() {
     for (void* q : {static_cast<void*>(usage), static_cast<void*>(cap), static_cast<void*>(hist),
                     static_cast<void*>(present), static_cast<void*>(total), static_cast<void*>(part),
-                    static_cast<void*>(result)})
+                    static_cast<void*>(decision)})
       if (q) (void)hipFree(q);
   }
 };
+
+// Policy scalars of an epoch: the present factor (checked) and the bounds.
+// Returns false (with out->outcome set) when the factor overflows.
+bool EpochParams(uint64_t epoch_index, const apgx_price_policy* policy, const apgx_price_limits* limits,
+                 uint64_t cells, Params* p, Limits* l, apgx_price_epoch* out) {
+  apgx_price_policy pol;
+  apgx_price_limits lim;
+  apgx_price_defaults(&pol, &lim);
+  if (policy) pol = *policy;
+  if (limits) lim = *limits;
+  std::memset(out, 0, sizeof(*out));
+  out->epoch_index = epoch_index;
+  out->outcome_cell = kNone;
+  *p = Params{};
+  if (__builtin_mul_overflow(epoch_index, pol.present_factor_increment, &p->factor) ||
+      __builtin_add_overflow(p->factor, pol.initial_present_factor, &p->factor)) {
+    out->outcome = APGX_PRICE_PRESENT_FACTOR_OVERFLOW;
+    return false;
+  }
+  out->present_factor = p->factor;
+  p->inc = pol.historical_price_increment;
+  p->max_value = lim.maximum_price_value;
+  p->cells = cells;
+  *l = Limits{lim.maximum_price_entries, lim.maximum_hot_resources, lim.maximum_aggregate_price};
+  return true;
+}
+
+// One epoch on device planes with the given partial and outcome scratch:
+// scan, decide, write (only when the checks pass), then the outcome record.
+// `history` is updated in place; present and total may be null.
+int RunEpoch(const Params& p, const Limits& l, const uint32_t* usage, const uint8_t* cap, uint32_t* history,
+             uint32_t* present, uint32_t* total, BlockPartial* part, Decision* decision, apgx_price_epoch* out) {
+  const uint64_t blocks = (p.cells + kBlockCells - 1) / kBlockCells;
+  hipLaunchKernelGGL(PriceScan, dim3(static_cast<unsigned>(blocks)), dim3(kThreads), 0, nullptr, p, usage, cap,
+                     history, part);
+  APGX_HIP(hipGetLastError(), "price scan launch");
+  hipLaunchKernelGGL(PriceDecide, dim3(1), dim3(kDecideThreads), 0, nullptr, p, l, usage, cap, history, part, blocks,
+                     decision);
+  APGX_HIP(hipGetLastError(), "price decision launch");
+  hipLaunchKernelGGL(PriceWrite, dim3(static_cast<unsigned>((p.cells + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                     nullptr, p, usage, cap, history, present, total, decision);
+  APGX_HIP(hipGetLastError(), "price write launch");
+  Decision d{};
+  APGX_HIP(hipMemcpy(&d, decision, sizeof(d), hipMemcpyDeviceToHost), "price outcome readback");
+  out->outcome = static_cast<uint32_t>(d.outcome);
+  out->outcome_cell = d.cell;
+  out->hot_resources = d.hot;
+  out->price_entries = d.entries;
+  out->aggregate_price = d.pass ? d.aggregate : 0;
+  return APGX_OK;
+}
 
 }  // namespace
 }  // namespace apgx
@@ -222,28 +444,11 @@ extern "C" int apgx_price_update(uint64_t epoch_index, const apgx_price_policy* 
   using namespace apgx;
   if (out == nullptr || (cells && (usage == nullptr || history == nullptr)))
     return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_price_update: null argument");
-  apgx_price_policy pol;
-  apgx_price_limits lim;
-  apgx_price_defaults(&pol, &lim);
-  if (policy) pol = *policy;
-  if (limits) lim = *limits;
-  if (lim.maximum_price_value > 0xFFFFFFFFull)
+  if (limits && limits->maximum_price_value > 0xFFFFFFFFull)
     return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_price_update: maximum_price_value must fit the u32 price planes");
-  std::memset(out, 0, sizeof(*out));
-  out->epoch_index = epoch_index;
-  out->outcome_cell = kNone;
-  Params p{};
-  if (__builtin_mul_overflow(epoch_index, pol.present_factor_increment, &p.factor) ||
-      __builtin_add_overflow(p.factor, pol.initial_present_factor, &p.factor)) {
-    out->outcome = APGX_PRICE_PRESENT_FACTOR_OVERFLOW;
-    ClearError();
-    return APGX_OK;
-  }
-  out->present_factor = p.factor;
-  p.inc = pol.historical_price_increment;
-  p.max_value = lim.maximum_price_value;
-  p.cells = cells;
-  if (cells == 0) {
+  Params p;
+  Limits l;
+  if (!EpochParams(epoch_index, policy, limits, cells, &p, &l, out) || cells == 0) {
     ClearError();
     return APGX_OK;
   }
@@ -252,97 +457,92 @@ extern "C" int apgx_price_update(uint64_t epoch_index, const apgx_price_policy* 
   APGX_HIP(hipMalloc(&b.usage, 4 * cells), "hipMalloc");
   APGX_HIP(hipMalloc(&b.hist, 4 * cells), "hipMalloc");
   APGX_HIP(hipMalloc(&b.part, sizeof(BlockPartial) * blocks), "hipMalloc");
-  APGX_HIP(hipMalloc(&b.result, 4 * sizeof(unsigned long long)), "hipMalloc");
+  APGX_HIP(hipMalloc(&b.decision, sizeof(Decision)), "hipMalloc");
+  if (present) APGX_HIP(hipMalloc(&b.present, 4 * cells), "hipMalloc");
+  if (total) APGX_HIP(hipMalloc(&b.total, 4 * cells), "hipMalloc");
   APGX_HIP(hipMemcpy(b.usage, usage, 4 * cells, hipMemcpyHostToDevice), "usage upload");
   APGX_HIP(hipMemcpy(b.hist, history, 4 * cells, hipMemcpyHostToDevice), "history upload");
   if (capacity) {
     APGX_HIP(hipMalloc(&b.cap, cells), "hipMalloc");
     APGX_HIP(hipMemcpy(b.cap, capacity, cells, hipMemcpyHostToDevice), "capacity upload");
   }
-  hipLaunchKernelGGL(PriceScan, dim3(static_cast<unsigned>(blocks)), dim3(kThreads), 0, nullptr, p, b.usage, b.cap,
-                     b.hist, b.part);
-  APGX_HIP(hipGetLastError(), "price scan launch");
-  std::vector<BlockPartial> part(blocks);
-  APGX_HIP(hipMemcpy(part.data(), b.part, sizeof(BlockPartial) * blocks, hipMemcpyDeviceToHost), "partial readback");
-
-  uint64_t over = 0, prior_entries = 0, entries = 0;
-  uint64_t first_arith = kNone, first_value = kNone;
-  for (const BlockPartial& q : part) {
-    over += q.over_count;
-    entries += q.entry_count;
-    prior_entries += q.entry_count - q.new_count;
-    first_arith = std::min<uint64_t>(first_arith, q.first_arith);
-    first_value = std::min<uint64_t>(first_value, q.first_value);
+  if (int rc = RunEpoch(p, l, b.usage, b.cap, b.hist, b.present, b.total, b.part, b.decision, out)) return rc;
+  if (out->outcome == APGX_PRICE_OK) {
+    APGX_HIP(hipMemcpy(history, b.hist, 4 * cells, hipMemcpyDeviceToHost), "history readback");
+    if (present) APGX_HIP(hipMemcpy(present, b.present, 4 * cells, hipMemcpyDeviceToHost), "present readback");
+    if (total) APGX_HIP(hipMemcpy(total, b.total, 4 * cells, hipMemcpyDeviceToHost), "total readback");
   }
-  out->hot_resources = over;
-  out->price_entries = entries;
-  auto locate = [&](uint64_t block, uint64_t new_target, uint64_t agg_start, uint64_t arith_cell,
-                    unsigned long long r[4]) -> int {
-    hipLaunchKernelGGL(PriceLocate, dim3(1), dim3(64), 0, nullptr, p, b.usage, b.cap, b.hist, block, new_target,
-                       agg_start, lim.maximum_aggregate_price, arith_cell, b.result);
-    APGX_HIP(hipGetLastError(), "price locate launch");
-    APGX_HIP(hipMemcpy(r, b.result, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost), "locate readback");
-    return APGX_OK;
-  };
-  auto finish = [&](uint32_t outcome, uint64_t cell) {
-    out->outcome = outcome;
-    out->outcome_cell = cell;
+  ClearError();
+  return APGX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// No-allocation price epoch.  Same kernels as apgx_price_update, on device
+// planes, with the block partials and the outcome record in `scratch`, a
+// fixed array in this library's device image (one copy per GPU, held until
+// the process ends): 65,536 partials of 64 bytes (2^26 cells at most) and the
+// 64-byte record, 4 MiB + 64 B.  Its device address is looked up once per
+// device, with that device current, on the entry point's first call there.
+// Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+// race.  Nothing in it needs clearing: each call writes every partial it
+// reads and the whole outcome record.
+// ---------------------------------------------------------------------------
+namespace apgx {
+namespace {
+
+constexpr uint64_t kNoAllocBlocks = 1ull << 16;
+constexpr uint64_t kNoAllocCells = kNoAllocBlocks * kBlockCells;
+constexpr size_t kScratchBytes = kNoAllocBlocks * sizeof(BlockPartial) + sizeof(Decision);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_address[kMaxScratchDevices] = {};
+
+// Device address of this device's `scratch` (the array's name is a host
+// stand-in, never a device address).
+int ScratchAddress(void** out) {
+  int device = -1;
+  APGX_HIP(hipGetDevice(&device), "hipGetDevice");
+  if (device < 0 || device >= kMaxScratchDevices)
+    return Fail(APGX_E_UNSUPPORTED, "apgx_price_update_no_alloc: device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_address[device] == nullptr) {
+    void* ptr = nullptr;
+    APGX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    g_scratch_address[device] = ptr;
+  }
+  *out = g_scratch_address[device];
+  return APGX_OK;
+}
+
+}  // namespace
+}  // namespace apgx
+
+extern "C" int apgx_price_update_no_alloc(uint64_t epoch_index, const apgx_price_policy* policy,
+                                          const apgx_price_limits* limits, uint64_t cells, const uint32_t* usage,
+                                          const uint8_t* capacity, uint32_t* history, uint32_t* present,
+                                          uint32_t* total, apgx_price_epoch* out) {
+  using namespace apgx;
+  if (out == nullptr || (cells && (usage == nullptr || history == nullptr)))
+    return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_price_update_no_alloc: null argument");
+  if (limits && limits->maximum_price_value > 0xFFFFFFFFull)
+    return Fail(APGX_E_INTERNAL_INVARIANT,
+                "apgx_price_update_no_alloc: maximum_price_value must fit the u32 price planes");
+  if (cells > kNoAllocCells)
+    return Fail(APGX_E_RESOURCE_EXHAUSTED,
+                "apgx_price_update_no_alloc: at most 67108864 cells (its fixed scratch holds 65536 block partials)");
+  void* base = nullptr;
+  if (int rc = ScratchAddress(&base)) return rc;
+  Params p;
+  Limits l;
+  if (!EpochParams(epoch_index, policy, limits, cells, &p, &l, out) || cells == 0) {
     ClearError();
     return APGX_OK;
-  };
-  // 1. hot-resource bound (checked before any price, negotiated_regeneration_plan.cc:690-697)
-  if (over > lim.maximum_hot_resources) return finish(APGX_PRICE_HOT_RESOURCE_BOUND, kNone);
-  // 2. overused cells in order: entry bound (the k-th new entry with
-  //    prior + k >= bound), then that cell's overflow checks.
-  uint64_t entry_cell = kNone;
-  const uint64_t new_entries = entries - prior_entries;
-  if (new_entries > 0 && prior_entries + new_entries > lim.maximum_price_entries) {
-    const uint64_t k = lim.maximum_price_entries > prior_entries ? lim.maximum_price_entries - prior_entries : 0;
-    uint64_t seen = 0;
-    for (uint64_t i = 0; i < blocks; ++i) {
-      if (seen + part[i].new_count > k) {
-        unsigned long long r[4];
-        if (int rc = locate(i, k - seen, 0, kNone, r); rc != APGX_OK) return rc;
-        entry_cell = r[0];
-        break;
-      }
-      seen += part[i].new_count;
-    }
   }
-  if (entry_cell != kNone && entry_cell <= first_arith) return finish(APGX_PRICE_ENTRY_BOUND, entry_cell);
-  if (first_arith != kNone) {
-    unsigned long long r[4];
-    if (int rc = locate(first_arith / kBlockCells, kNone, 0, first_arith, r); rc != APGX_OK) return rc;
-    return finish(static_cast<uint32_t>(r[3]), first_arith);
-  }
-  // 3. entries in order: value bound, then the running aggregate.
-  uint64_t agg = 0, agg_cell = kNone;
-  uint32_t agg_kind = 0;
-  for (uint64_t i = 0; i < blocks; ++i) {
-    uint64_t next = 0;
-    const bool ovf = __builtin_add_overflow(agg, static_cast<uint64_t>(part[i].sum_total), &next);
-    if (ovf || next > lim.maximum_aggregate_price) {
-      unsigned long long r[4];
-      if (int rc = locate(i, kNone, agg, kNone, r); rc != APGX_OK) return rc;
-      agg_cell = r[1];
-      agg_kind = r[2] ? APGX_PRICE_AGGREGATE_OVERFLOW : APGX_PRICE_AGGREGATE_BOUND;
-      break;
-    }
-    agg = next;
-  }
-  if (first_value != kNone && first_value <= agg_cell) return finish(APGX_PRICE_VALUE_BOUND, first_value);
-  if (agg_cell != kNone) return finish(agg_kind, agg_cell);
-  out->aggregate_price = agg;
-  // 4. every check passed: write the planes.
-  if (present) APGX_HIP(hipMalloc(&b.present, 4 * cells), "hipMalloc");
-  if (total) APGX_HIP(hipMalloc(&b.total, 4 * cells), "hipMalloc");
-  hipLaunchKernelGGL(PriceWrite, dim3(static_cast<unsigned>((cells + kThreads - 1) / kThreads)), dim3(kThreads), 0,
-                     nullptr, p, b.usage, b.cap, b.hist, b.present, b.total);
-  APGX_HIP(hipGetLastError(), "price write launch");
-  APGX_HIP(hipMemcpy(history, b.hist, 4 * cells, hipMemcpyDeviceToHost), "history readback");
-  if (present) APGX_HIP(hipMemcpy(present, b.present, 4 * cells, hipMemcpyDeviceToHost), "present readback");
-  if (total) APGX_HIP(hipMemcpy(total, b.total, 4 * cells, hipMemcpyDeviceToHost), "total readback");
-  out->outcome = APGX_PRICE_OK;
+  auto* part = static_cast<BlockPartial*>(base);
+  auto* decision = reinterpret_cast<Decision*>(static_cast<unsigned char*>(base) + kNoAllocBlocks * sizeof(BlockPartial));
+  if (int rc = RunEpoch(p, l, usage, capacity, history, present, total, part, decision, out)) return rc;
   ClearError();
   return APGX_OK;
 }

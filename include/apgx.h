@@ -529,6 +529,22 @@ int apgx_price_update(uint64_t epoch_index, const apgx_price_policy* policy,
                       const uint8_t* capacity, uint32_t* history, uint32_t* present, uint32_t* total,
                       apgx_price_epoch* out);
 
+/* No counterpart in the reference: the price epoch of apgx_price_update for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same checks,
+ * outcomes and kernels, but usage, capacity (may be NULL), history (in/out),
+ * present and total (may be NULL) are DEVICE planes of `cells` entries on
+ * the current device, and the call allocates nothing: its block partials and
+ * outcome record live in a fixed 4 MiB scratch array of the library, one per
+ * GPU.  At most 67,108,864 cells (APGX_E_RESOURCE_EXHAUSTED beyond).  Calls
+ * that overlap on one GPU share that array and race; the call returns when
+ * the epoch has finished. */
+#define APGX_PRICE_NO_ALLOC_MAX_CELLS 67108864ull
+int apgx_price_update_no_alloc(uint64_t epoch_index, const apgx_price_policy* policy,
+                               const apgx_price_limits* limits, uint64_t cells, const uint32_t* usage,
+                               const uint8_t* capacity, uint32_t* history, uint32_t* present, uint32_t* total,
+                               apgx_price_epoch* out);
+
 /* ------------------------------------------------------------------------ */
 /* Learned history-cost prior                                               */
 /* ------------------------------------------------------------------------ */
