@@ -94,11 +94,15 @@ class TuneReport(C.Structure):
                 ("reserved", C.c_uint32), ("candidates", TuneCandidate * 24)]
 
 
+NEGOTIATION_PASS_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_uint32, C.c_uint64, C.c_uint32, C.c_double)
+
+
 class NegotiationPolicy(C.Structure):
     _fields_ = [("maximum_passes", C.c_uint32), ("batch_nets", C.c_uint32),
                 ("initial_present_cost", C.c_uint64), ("present_cost_increment", C.c_uint64),
                 ("historical_cost_increment", C.c_uint64),
-                ("maximum_congestion_cost_value", C.c_uint64)]
+                ("maximum_congestion_cost_value", C.c_uint64), ("on_pass", NEGOTIATION_PASS_FN),
+                ("on_pass_context", C.c_void_p)]
 
 
 class NegotiationSummary(C.Structure):
@@ -544,14 +548,19 @@ class Lattice:
                   search_options=None, **policy):
         """Negotiated-congestion allocator (apgx_negotiate).  policy keywords:
         maximum_passes, batch_nets, initial_present_cost, present_cost_increment,
-        historical_cost_increment, maximum_congestion_cost_value.  Returns a dict
+        historical_cost_increment, maximum_congestion_cost_value, and on_pass
+        (a callable (pass, overuse, routed nets, wall ms) run after every pass).  Returns a dict
         like the oracle's: termination, passes, routed, overuse, overuse_per_pass,
         history / usage planes (L, H, W) and per-net tree cells (None: unrouted)."""
         pins = np.ascontiguousarray(pins, dtype=np.uint32)
         pol = NegotiationPolicy()
         lib().apgx_negotiation_policy_defaults(C.byref(pol))
+        on_pass = policy.pop("on_pass", None)
         for k, v in policy.items():
             setattr(pol, k, v)
+        if on_pass is not None:
+            # Kept alive for the call: on_pass(pass, overuse, routed, wall_ms).
+            pol.on_pass = NEGOTIATION_PASS_FN(lambda _ctx, p, o, r, ms: on_pass(p, o, r, ms))
         L, H, W = self.shape
         n = L * H * W
         cap = None if capacity is None else np.ascontiguousarray(capacity, dtype=np.uint8).reshape(-1)

@@ -27,7 +27,7 @@ int HipFail(hipError_t error, const char* operation) {
 
 extern "C" const char* apgx_last_error(void) { return apgx::g_last_error.c_str(); }
 
-extern "C" uint32_t apgx_abi_version(void) { return (1u << 16) | 1u; }
+extern "C" uint32_t apgx_abi_version(void) { return (1u << 16) | 2u; }
 
 extern "C" int apgx_device_synchronize(void) {
   APGX_HIP(hipDeviceSynchronize(), "hipDeviceSynchronize");

@@ -197,6 +197,8 @@ extern "C" void apgx_negotiation_policy_defaults(apgx_negotiation_policy* p) {
   p->present_cost_increment = 100;
   p->historical_cost_increment = 10;
   p->maximum_congestion_cost_value = 10000000;
+  p->on_pass = nullptr;
+  p->on_pass_context = nullptr;
 }
 
 extern "C" int apgx_negotiate(apgx_lattice* lat, const uint8_t* capacity, const apgx_net_query* nets,
@@ -307,7 +309,8 @@ extern "C" int apgx_negotiate(apgx_lattice* lat, const uint8_t* capacity, const 
 
   // Batches in processing order: consecutive runs of batch_nets nets, or
   // (batch_nets == 0) the window-disjoint levels of the exact order.
-  std::vector<std::vector<uint32_t>> batches;
+  std::vector<std::vector<uint32_t>> batches, all_nets(1);
+  for (uint32_t n = 0; n < net_count; ++n) all_nets[0].push_back(n);
   if (pol.batch_nets == 0) {
     batches = WindowLevels(nets, net_count, d);
   } else {
@@ -327,7 +330,11 @@ extern "C" int apgx_negotiate(apgx_lattice* lat, const uint8_t* capacity, const 
     if (__builtin_mul_overflow(static_cast<uint64_t>(pass), pol.present_cost_increment, &pf) ||
         __builtin_add_overflow(pf, pol.initial_present_cost, &pf))
       return Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_negotiate: present factor overflowed uint64");
-    for (const std::vector<uint32_t>& members : batches) {
+    // A zero present factor prices no cell by its usage: the snapshot is base +
+    // history for every batch of the pass, no route depends on another, and
+    // one batch of every net gives the routes of any order.
+    const std::vector<std::vector<uint32_t>>& pass_batches = pf == 0 ? all_nets : batches;
+    for (const std::vector<uint32_t>& members : pass_batches) {
       const uint32_t count = static_cast<uint32_t>(members.size());
       rip_list.clear();
       for (uint32_t n : members) rip_list.insert(rip_list.end(), neg->routes[n].begin(), neg->routes[n].end());
@@ -409,6 +416,9 @@ extern "C" int apgx_negotiate(apgx_lattice* lat, const uint8_t* capacity, const 
     sum.completed_passes = pass + 1;
     sum.total_overuse_units = overuse;
     sum.routed_nets = routed;
+    if (pol.on_pass)
+      pol.on_pass(pol.on_pass_context, pass, overuse, routed,
+                  std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall_start).count());
     if (routed == net_count && overuse == 0) {
       sum.termination = APGX_NEGOTIATION_FEASIBLE;
       break;

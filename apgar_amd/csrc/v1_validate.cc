@@ -13,6 +13,8 @@
 #include <cstring>
 #include <queue>
 #include <string>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 #include "apgx.h"
@@ -22,6 +24,8 @@
 namespace apgx {
 namespace v1check {
 namespace {
+
+using namespace ::apgx::v1;
 
 constexpr uint64_t kInf = ~0ull;
 constexpr uint32_t kNone = APGX_V1_INVALID_INDEX;
@@ -203,6 +207,319 @@ bool AllPredecessorsValid(const Context& c, const apgx_v1_query& q, const uint64
   return true;
 }
 
+// The inputs of one batch check, shared by the first-failure and the
+// per-query entry points.
+struct Batch {
+  const apgx_v1_header* header;
+  Context c;
+  uint64_t policy_count;
+  const apgx_v1_batch_request* request;
+  const apgx_v1_query* queries;
+  uint32_t query_count;
+  const apgx_v1_result* results;
+  const apgx_v1_compact_path* paths;
+  const uint32_t* states;
+  uint64_t state_count;
+  const apgx_v1_workspace_view* ws;
+  const apgx_v1_batch_telemetry* t;
+  GeneratorShape shape;
+  bool full;
+  uint64_t S;
+  std::vector<uint32_t> result_of;  // readback position of each query's result; kNone: omitted or duplicated
+};
+
+bool Reject(apgx_v1_validation* v, uint32_t qi, uint32_t failure, const char* id, const std::string& detail) {
+  std::memset(v, 0, sizeof(*v));
+  v->query_index = qi;
+  v->failure = failure;
+  std::strncpy(v->invariant, id, sizeof(v->invariant) - 1);
+  std::strncpy(v->detail, detail.c_str(), sizeof(v->detail) - 1);
+  return false;
+}
+
+bool Invariant(apgx_v1_validation* v, uint32_t qi, const char* id, const std::string& detail) {
+  return Reject(v, qi, APGX_V1_CHECK_INTERNAL_INVARIANT, id, detail);
+}
+
+void Accept(apgx_v1_validation* v, uint32_t qi) {
+  std::memset(v, 0, sizeof(*v));
+  v->valid = 1;
+  v->query_index = qi;
+  v->failure = APGX_V1_CHECK_OK;
+}
+
+// Checks that hold for the batch as a whole (a failure here fails every
+// query): the result set names the admitted queries (:2185-2258), the readback
+// kind and flat bounds, the canonical compact-path storage, and the memory and
+// launch telemetry (ValidateCandidateBatchTelemetry, :1841-1955).  Fills
+// result_of: results in query order map one to one; otherwise by query id,
+// where an id that came back twice maps no query.
+bool CheckBatch(Batch& b, apgx_v1_validation* v) {
+  const uint32_t n = b.query_count;
+  b.result_of.assign(n, kNone);
+  bool canonical = true;
+  for (uint32_t i = 0; i < n && canonical; ++i) canonical = b.results[i].query_id == b.queries[i].query_id;
+  if (canonical) {
+    for (uint32_t i = 0; i < n; ++i) b.result_of[i] = i;
+  } else {
+    std::vector<std::pair<uint64_t, uint32_t>> admitted(n), returned(n);
+    for (uint32_t i = 0; i < n; ++i) {
+      admitted[i] = {b.queries[i].query_id, i};
+      returned[i] = {b.results[i].query_id, i};
+    }
+    std::sort(admitted.begin(), admitted.end());
+    std::sort(returned.begin(), returned.end());
+    const auto first_admitted = [&](uint64_t id) {
+      return std::lower_bound(admitted.begin(), admitted.end(), std::make_pair(id, 0u));
+    };
+    for (const auto& r : returned) {
+      const auto it = first_admitted(r.first);
+      if (it == admitted.end() || it->first != r.first)
+        return Invariant(v, kNone, "gpu.batch.query.identity.v1",
+                         "GPU candidate batch returned an extra, foreign, or missing query result");
+    }
+    for (size_t k = 0; k < returned.size();) {
+      size_t e = k + 1;
+      while (e < returned.size() && returned[e].first == returned[k].first) ++e;
+      if (e - k == 1)
+        for (auto it = first_admitted(returned[k].first); it != admitted.end() && it->first == returned[k].first; ++it)
+          b.result_of[it->second] = returned[k].second;
+      k = e;
+    }
+  }
+
+  const apgx_v1_batch_telemetry& t = *b.t;
+  const apgx_v1_workspace_view& ws = *b.ws;
+  const uint64_t total_states = static_cast<uint64_t>(n) * b.S;
+  const bool arrays_empty = ws.label_count == 0 && ws.predecessor_count == 0 && ws.state_owner_count == 0 &&
+                            ws.predecessor_owner_count == 0;
+  if ((!b.full && (ws.readback_kind != APGX_V1_READBACK_COMPACT_PATHS || !arrays_empty ||
+                   b.state_count > total_states || (n && b.paths == nullptr))) ||
+      (b.full && (ws.readback_kind != APGX_V1_READBACK_FULL_WORKSPACE || ws.label_count != total_states ||
+                  ws.predecessor_count != total_states || ws.state_owner_count != total_states ||
+                  ws.predecessor_owner_count != total_states || b.state_count != 0 ||
+                  (total_states && (!ws.labels || !ws.predecessors || !ws.state_owners || !ws.predecessor_owners)))))
+    return Invariant(v, kNone, "gpu.batch.workspace.bounds.v1",
+                     "GPU candidate batch readback kind or flat workspace bounds are invalid");
+  if (!b.full) {
+    uint64_t expected = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      if (b.paths[i].state_offset != expected)
+        return Invariant(v, kNone, "gpu.batch.compact_path.bounds.v1",
+                         "GPU compact sweep paths are not one canonical contiguous batch");
+      expected += b.paths[i].state_count;
+    }
+    if (expected != b.state_count)
+      return Invariant(v, kNone, "gpu.batch.compact_path.bounds.v1",
+                       "GPU compact sweep path storage has trailing or missing states");
+  }
+  const uint64_t device_bytes = LayoutBatch(b.full, n, b.policy_count, b.c.node_count).total;
+  if (t.persistent_device_bytes != b.header->estimated_persistent_device_bytes || t.batch_device_bytes != device_bytes ||
+      t.batch_host_bytes != HostBytes(b.full, n, b.c.node_count, b.state_count))
+    return Invariant(v, kNone, "gpu.batch.memory.accounting.v1",
+                     "GPU candidate batch device or host memory telemetry disagrees with deterministic accounting");
+  const apgx_v1_batch_request& rq = *b.request;
+  if (t.peak_device_bytes != t.persistent_device_bytes + t.workspace_capacity_device_bytes ||
+      t.peak_device_bytes < t.persistent_device_bytes || t.workspace_capacity_device_bytes < t.batch_device_bytes ||
+      t.persistent_device_bytes > rq.maximum_device_bytes ||
+      t.workspace_capacity_device_bytes > rq.maximum_device_bytes - t.persistent_device_bytes ||
+      t.device_to_host_readback_bytes != ReadbackBytes(b.full, n, b.c.node_count, b.state_count) ||
+      !std::isfinite(t.kernel_milliseconds) || t.kernel_milliseconds < 0.0 || t.chunk_rounds != b.shape.chunk_rounds ||
+      t.dispatched_rounds > rq.maximum_rounds || t.finalization_launch_count > 1 ||
+      (t.dispatched_rounds == 0 && t.finalization_launch_count != 1))
+    return Invariant(v, kNone, "gpu.batch.telemetry.v1", "GPU candidate batch peak, timing, or chunk telemetry is invalid");
+  const uint64_t readbacks =
+      (static_cast<uint64_t>(t.dispatched_rounds) + b.shape.chunk_rounds - 1) / b.shape.chunk_rounds;
+  if (t.kernel_launch_count != b.shape.fixed_launches + readbacks + t.finalization_launch_count ||
+      t.blocking_status_readback_count != readbacks)
+    return Invariant(v, kNone, "gpu.batch.telemetry.v1", "GPU candidate batch launch or blocking-readback telemetry is invalid");
+  return true;
+}
+
+// One query's checks (the body of the reference's validate_query,
+// :2259-2380): its envelope (ValidateCandidateBatchEnvelope, :1732-1839), its
+// completion telemetry (:456-484), then the full workspace or the compact path.
+// `visited` is scratch of (S + 63) / 64 words owned by the caller's worker.
+bool CheckQuery(const Batch& b, uint32_t qi, std::vector<uint64_t>& visited, uint64_t* path_cost,
+                apgx_v1_validation* v) {
+  *path_cost = kInf;
+  const uint32_t j = b.result_of[qi];
+  if (j == kNone)
+    return Invariant(v, qi, "gpu.batch.query.identity.v1", "GPU candidate batch omitted or duplicated a query result");
+  const Context& c = b.c;
+  const uint64_t S = b.S;
+  const apgx_v1_batch_telemetry& t = *b.t;
+  const apgx_v1_batch_request& rq = *b.request;
+  const apgx_v1_header* header = b.header;
+  const apgx_v1_query& q = b.queries[qi];
+  const apgx_v1_result& r = b.results[j];
+  const uint64_t total_states = static_cast<uint64_t>(b.query_count) * S;
+  if (r.schema_version != APGX_V1_SCHEMA_VERSION || r.batch_id != q.batch_id || r.query_id != q.query_id ||
+      r.workspace_owner != q.workspace_owner || r.input_ordinal != q.input_ordinal ||
+      r.start_node != q.start_node || r.goal_node != q.goal_node || r.generator != q.generator ||
+      r.policy_identity != q.policy_identity || r.routing_profile_fingerprint != q.routing_profile_fingerprint ||
+      r.compiler_version != header->compiler_version ||
+      r.source_board_content_hash != header->source_board_content_hash ||
+      r.compiler_profile_fingerprint != header->compiler_profile_fingerprint ||
+      r.rule_bucket_identity != header->rule_bucket_identity ||
+      r.device_view_fingerprint != header->device_view_fingerprint || r.reserved[0] != 0 || r.reserved[1] != 0)
+    return Reject(v, qi, APGX_V1_CHECK_VALIDATION_FAILED, "",
+                  "GPU candidate batch result associations do not match its immutable query workspace");
+  const apgx_v1_query_envelope& env = b.ws->queries[j];
+  if (env.workspace_offset != q.workspace_offset || env.workspace_state_count != S)
+    return Invariant(v, qi, "gpu.batch.workspace.bounds.v1",
+                     "GPU candidate batch query-major workspace has invalid array bounds");
+  const uint64_t off = env.workspace_offset;
+  if (b.full) {
+    if (off > total_states || env.workspace_state_count > total_states - off)
+      return Invariant(v, qi, "gpu.batch.workspace.bounds.v1",
+                       "GPU candidate batch query-major workspace has invalid array bounds");
+    for (uint64_t k = 0; k < S; ++k)
+      if (b.ws->state_owners[off + k] != q.workspace_owner || b.ws->predecessor_owners[off + k] != q.workspace_owner)
+        return Invariant(v, qi, "gpu.batch.workspace.owner.v1",
+                         "GPU candidate batch workspace contains foreign query ownership");
+  } else {
+    const apgx_v1_compact_path& p = b.paths[j];
+    const bool reached = r.completion == APGX_V1_REACHED;
+    const uint64_t end = p.state_offset + static_cast<uint64_t>(p.state_count);
+    if (p.schema_version != APGX_V1_SCHEMA_VERSION || p.query_id != q.query_id ||
+        p.workspace_owner != q.workspace_owner || p.state_count > S || end < p.state_offset || end > b.state_count ||
+        (reached ? p.state_count == 0 : p.state_count != 0))
+      return Invariant(v, qi, "gpu.batch.compact_path.bounds.v1",
+                       "GPU compact sweep path has invalid associations or array bounds");
+  }
+  if (env.rounds != r.rounds || r.rounds > t.dispatched_rounds)
+    return Invariant(v, qi, "gpu.batch.query.telemetry.v1",
+                     "GPU candidate query round telemetry disagrees with its result header or shared dispatch envelope");
+  if ((r.completion == APGX_V1_BUDGET_EXHAUSTED || r.completion == APGX_V1_CANCELLED) &&
+      t.finalization_launch_count != 1)
+    return Invariant(v, qi, "gpu.batch.query.telemetry.v1",
+                     "GPU unfinished query completion lacks its shared finalization launch");
+  const unsigned __int128 max_examined =
+      static_cast<unsigned __int128>(env.rounds) *
+      (b.shape.fixed_examined_work_per_round +
+       static_cast<unsigned __int128>(c.node_count) * b.shape.examined_work_per_node_per_round);
+  if (env.examined_work > max_examined ||
+      env.heading_turn_relaxations > max_examined * b.shape.turn_relaxations_per_examined_work)
+    return Invariant(v, qi, "gpu.batch.query.telemetry.v1",
+                     "GPU candidate query work telemetry exceeds deterministic bounds");
+  if (r.completion > APGX_V1_CANCELLED)
+    return Invariant(v, qi, "gpu.completion.unknown.v1", "GPU result contains an unknown completion code");
+  if (r.rounds > rq.maximum_rounds)
+    return Invariant(v, qi, "", "GPU round telemetry exceeds the configured execution budget");
+  if (r.completion == APGX_V1_BUDGET_EXHAUSTED && r.rounds != rq.maximum_rounds)
+    return Invariant(v, qi, "", "GPU reported budget exhaustion before consuming its round budget");
+  if ((r.completion == APGX_V1_REACHED || r.completion == APGX_V1_DISCONNECTED) && r.rounds == 0)
+    return Invariant(v, qi, "", "GPU reported a converged completion without an execution round");
+
+  const uint32_t start_state = q.start_node * 9u + 8u;
+  if (b.full) {
+    const uint64_t* lab = b.ws->labels + off;
+    const uint32_t* pre = b.ws->predecessors + off;
+    const bool exact = r.completion == APGX_V1_REACHED || r.completion == APGX_V1_DISCONNECTED;
+    std::string why;
+    const char* id = "";
+    if (!AllPredecessorsValid(c, q, lab, pre, exact, &why, &id)) return Invariant(v, qi, id, why);
+    if (r.completion == APGX_V1_REACHED) {
+      uint64_t best = kInf;
+      uint32_t best_state = kNone;
+      for (uint32_t h = 0; h < 8; ++h) {
+        const uint32_t st = q.goal_node * 9u + h;
+        if (lab[st] < best || (lab[st] == best && st < best_state)) {
+          best = lab[st];
+          best_state = st;
+        }
+      }
+      if (best == kInf || r.goal_state != best_state)
+        return Invariant(v, qi, "", "GPU goal state is not the stable minimum finite heading state");
+      uint32_t cursor = best_state;
+      for (uint64_t steps = 0;; ++steps) {
+        if (cursor >= S) return Invariant(v, qi, "", "GPU reconstruction escaped the allocated state array");
+        if (cursor == start_state) break;
+        if (steps >= S) return Invariant(v, qi, "", "GPU predecessor reconstruction exceeded its acyclic state bound");
+        cursor = pre[cursor];
+      }
+      *path_cost = best;
+    } else if (r.completion == APGX_V1_DISCONNECTED) {
+      for (uint32_t h = 0; h < 8; ++h)
+        if (lab[q.goal_node * 9u + h] != kInf)
+          return Invariant(v, qi, "", "GPU reported disconnected with a finite goal label");
+      if (GoalReachable(c, q))
+        return Invariant(v, qi, "gpu.disconnected.cpu_reachable.v1",
+                         "GPU reported disconnected for a query the CPU oracle reached under the identical policy");
+    }
+    Accept(v, qi);
+    return true;
+  }
+  // Compact path (ValidateAndReconstructCompactGpuPath, :969-1131).
+  const apgx_v1_compact_path& p = b.paths[j];
+  if (r.completion == APGX_V1_DISCONNECTED && GoalReachable(c, q))
+    return Invariant(v, qi, "gpu.disconnected.cpu_reachable.v1",
+                     "GPU reported disconnected for a query the CPU oracle reached under the identical policy");
+  if (r.completion != APGX_V1_REACHED) {
+    Accept(v, qi);
+    return true;
+  }
+  const uint32_t* rev = b.states + p.state_offset;
+  const uint32_t n = p.state_count;
+  if (rev[0] != r.goal_state || rev[n - 1] != start_state || r.goal_state >= S ||
+      r.goal_state / 9u != q.goal_node || r.goal_state % 9u >= 8u)
+    return Invariant(v, qi, "gpu.batch.compact_path.endpoint.v1", "path does not bind the goal to the canonical start");
+  for (uint32_t k = 0; k < n; ++k) {
+    if (rev[k] >= S) return Invariant(v, qi, "gpu.batch.compact_path.bounds.v1", "path state out of range");
+    visited[rev[k] / 64] = 0;
+  }
+  uint64_t total = 0;
+  for (uint32_t k = 0; k < n; ++k) {
+    const uint32_t st = rev[n - 1 - k];
+    const uint64_t mask = 1ull << (st % 64);
+    if (visited[st / 64] & mask) return Invariant(v, qi, "gpu.batch.compact_path.cycle.v1", "repeated path state");
+    visited[st / 64] |= mask;
+    if (k == 0) continue;
+    const uint32_t pred = rev[n - k];
+    const uint32_t pnode = pred / 9u, node = st / 9u, inc = st % 9u, pinc = pred % 9u;
+    if (inc >= 8u || (pinc == 8u && pred != start_state))
+      return Invariant(v, qi, "gpu.batch.compact_path.heading.v1", "noncanonical incoming heading");
+    if (!AdjacentLegal(c, pnode, node, inc))
+      return Invariant(v, qi, "gpu.batch.compact_path.edge.v1", "path step is not one adjacent legal edge");
+    uint64_t w;
+    bool banned;
+    if (!Transition(c, q, pnode, inc, pinc, &w, &banned)) {
+      if (banned) return Invariant(v, qi, "gpu.policy.banned_predecessor.v1", "path uses a banned resource");
+      return Invariant(v, qi, "gpu.batch.compact_path.edge.v1", "path step cost is invalid or overflows");
+    }
+    if (!CheckedAdd(total, w, &total))
+      return Invariant(v, qi, "gpu.batch.compact_path.edge.v1", "path cost overflows");
+  }
+  *path_cost = total;
+  Accept(v, qi);
+  return true;
+}
+
+// Argument checks and the Batch both entry points start from.
+int MakeBatch(const apgx_v1_header* header, const apgx_v1_node* nodes, uint64_t node_count,
+               const apgx_v1_batch_request* request, const apgx_v1_query* queries, uint32_t query_count,
+               const apgx_v1_policy_edge* policy, uint64_t policy_count, const apgx_v1_result* results,
+               const apgx_v1_compact_path* paths, const uint32_t* states, uint64_t state_count,
+               const apgx_v1_workspace_view* workspace, const apgx_v1_batch_telemetry* telemetry, const char* what,
+               Batch* b) {
+  if (header == nullptr || nodes == nullptr || request == nullptr || telemetry == nullptr || workspace == nullptr ||
+      (query_count && (queries == nullptr || results == nullptr)) || (policy_count && policy == nullptr) ||
+      (state_count && states == nullptr))
+    return Fail(APGX_E_INTERNAL_INVARIANT, std::string(what) + ": null argument");
+  if (query_count && workspace->queries == nullptr)
+    return Fail(APGX_E_INTERNAL_INVARIANT, std::string(what) + ": null workspace envelopes");
+  *b = Batch{header, Context{header, nodes, node_count, policy}, policy_count, request, queries, query_count,
+             results, paths, states, state_count, workspace, telemetry, GeneratorShape{}, false, node_count * 9, {}};
+  return APGX_OK;
+}
+
+// Validation workers for a batch of `queries`: one per 64 queries, at most 8
+// (CandidateCompactValidationWorkerCountV1, include/apgar/gpu/planar_router.h:439-450).
+uint32_t WorkerCount(uint32_t queries) {
+  return queries == 0 ? 0 : std::min<uint32_t>(8, 1 + (queries - 1) / 64);
+}
+
 }  // namespace
 }  // namespace v1check
 }  // namespace apgx
@@ -217,241 +534,89 @@ extern "C" int apgx_v1_validate_batch(const apgx_v1_header* header, const apgx_v
                                       const apgx_v1_batch_telemetry* telemetry, uint64_t* path_costs,
                                       apgx_v1_validation* out) {
   using namespace apgx;
-  using namespace apgx::v1;
   using namespace apgx::v1check;
-  if (out == nullptr || header == nullptr || nodes == nullptr || request == nullptr || telemetry == nullptr ||
-      workspace == nullptr || (query_count && (queries == nullptr || results == nullptr)) ||
-      (policy_count && policy == nullptr) || (state_count && states == nullptr))
-    return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_v1_validate_batch: null argument");
-  std::memset(out, 0, sizeof(*out));
-  out->query_index = kNone;
+  if (out == nullptr) return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_v1_validate_batch: null argument");
+  Batch b;
+  if (int e = MakeBatch(header, nodes, node_count, request, queries, query_count, policy, policy_count, results,
+                        paths, states, state_count, workspace, telemetry, "apgx_v1_validate_batch", &b);
+      e != APGX_OK)
+    return e;
   for (uint32_t i = 0; i < query_count && path_costs; ++i) path_costs[i] = kInf;
-  auto reject = [&](uint32_t qi, uint32_t failure, const char* invariant, const std::string& detail) {
-    out->valid = 0;
-    out->query_index = qi;
-    out->failure = failure;
-    std::strncpy(out->invariant, invariant, sizeof(out->invariant) - 1);
-    std::strncpy(out->detail, detail.c_str(), sizeof(out->detail) - 1);
-    ClearError();
-    return APGX_OK;
-  };
-  auto invariant = [&](uint32_t qi, const char* id, const std::string& detail) {
-    return reject(qi, APGX_V1_CHECK_INTERNAL_INVARIANT, id, detail);
-  };
-  const Context c{header, nodes, node_count, policy};
-  const uint64_t S = node_count * 9;
-  GeneratorShape shape{};
-  if (!DescribeGenerator(request->generator, &shape))
-    return reject(kNone, APGX_V1_CHECK_VALIDATION_FAILED, "gpu.generator.unknown.v1",
-                  "GPU candidate batch telemetry names an unknown generator");
-  const bool full = shape.full_workspace;
-
-  // Every result names exactly one admitted query (:2195-2218).
-  {
-    std::vector<uint64_t> want(query_count), got(query_count);
-    for (uint32_t i = 0; i < query_count; ++i) {
-      want[i] = queries[i].query_id;
-      got[i] = results[i].query_id;
-    }
-    std::sort(want.begin(), want.end());
-    std::sort(got.begin(), got.end());
-    if (want != got)
-      return invariant(kNone, "gpu.batch.query.identity.v1",
-                       "GPU candidate batch returned an extra, foreign, or missing query result");
-  }
-
-  // Batch telemetry (ValidateCandidateBatchTelemetry, :1841-1955).
-  const apgx_v1_batch_telemetry& t = *telemetry;
-  const uint64_t total_states = static_cast<uint64_t>(query_count) * S;
-  const apgx_v1_workspace_view& ws = *workspace;
-  if (query_count && ws.queries == nullptr)
-    return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_v1_validate_batch: null workspace envelopes");
-  const bool arrays_empty = ws.label_count == 0 && ws.predecessor_count == 0 && ws.state_owner_count == 0 &&
-                            ws.predecessor_owner_count == 0;
-  if ((!full && (ws.readback_kind != APGX_V1_READBACK_COMPACT_PATHS || !arrays_empty || state_count > total_states ||
-                 (query_count && paths == nullptr))) ||
-      (full && (ws.readback_kind != APGX_V1_READBACK_FULL_WORKSPACE || ws.label_count != total_states ||
-                ws.predecessor_count != total_states || ws.state_owner_count != total_states ||
-                ws.predecessor_owner_count != total_states || state_count != 0 ||
-                (total_states && (!ws.labels || !ws.predecessors || !ws.state_owners || !ws.predecessor_owners)))))
-    return invariant(kNone, "gpu.batch.workspace.bounds.v1",
-                     "GPU candidate batch readback kind or flat workspace bounds are invalid");
-  if (!full) {
-    uint64_t expected = 0;
-    for (uint32_t i = 0; i < query_count; ++i) {
-      if (paths[i].state_offset != expected)
-        return invariant(kNone, "gpu.batch.compact_path.bounds.v1",
-                         "GPU compact sweep paths are not one canonical contiguous batch");
-      expected += paths[i].state_count;
-    }
-    if (expected != state_count)
-      return invariant(kNone, "gpu.batch.compact_path.bounds.v1",
-                       "GPU compact sweep path storage has trailing or missing states");
-  }
-  const uint64_t device_bytes = LayoutBatch(full, query_count, policy_count, node_count).total;
-  if (t.persistent_device_bytes != header->estimated_persistent_device_bytes || t.batch_device_bytes != device_bytes ||
-      t.batch_host_bytes != HostBytes(full, query_count, node_count, state_count))
-    return invariant(kNone, "gpu.batch.memory.accounting.v1",
-                     "GPU candidate batch device or host memory telemetry disagrees with deterministic accounting");
-  if (t.peak_device_bytes != t.persistent_device_bytes + t.workspace_capacity_device_bytes ||
-      t.peak_device_bytes < t.persistent_device_bytes || t.workspace_capacity_device_bytes < t.batch_device_bytes ||
-      t.persistent_device_bytes > request->maximum_device_bytes ||
-      t.workspace_capacity_device_bytes > request->maximum_device_bytes - t.persistent_device_bytes ||
-      t.device_to_host_readback_bytes != ReadbackBytes(full, query_count, node_count, state_count) ||
-      !std::isfinite(t.kernel_milliseconds) || t.kernel_milliseconds < 0.0 || t.chunk_rounds != shape.chunk_rounds ||
-      t.dispatched_rounds > request->maximum_rounds || t.finalization_launch_count > 1 ||
-      (t.dispatched_rounds == 0 && t.finalization_launch_count != 1))
-    return invariant(kNone, "gpu.batch.telemetry.v1", "GPU candidate batch peak, timing, or chunk telemetry is invalid");
-  const uint64_t readbacks = (static_cast<uint64_t>(t.dispatched_rounds) + shape.chunk_rounds - 1) / shape.chunk_rounds;
-  if (t.kernel_launch_count != shape.fixed_launches + readbacks + t.finalization_launch_count ||
-      t.blocking_status_readback_count != readbacks)
-    return invariant(kNone, "gpu.batch.telemetry.v1", "GPU candidate batch launch or blocking-readback telemetry is invalid");
-
-  std::vector<uint64_t> visited((S + 63) / 64, 0);
-  for (uint32_t qi = 0; qi < query_count; ++qi) {
-    const apgx_v1_query& q = queries[qi];
-    const apgx_v1_result& r = results[qi];
-    // Envelope (ValidateCandidateBatchEnvelope, :1732-1839).
-    if (r.schema_version != APGX_V1_SCHEMA_VERSION || r.batch_id != q.batch_id || r.query_id != q.query_id ||
-        r.workspace_owner != q.workspace_owner || r.input_ordinal != q.input_ordinal ||
-        r.start_node != q.start_node || r.goal_node != q.goal_node || r.generator != q.generator ||
-        r.policy_identity != q.policy_identity || r.routing_profile_fingerprint != q.routing_profile_fingerprint ||
-        r.compiler_version != header->compiler_version ||
-        r.source_board_content_hash != header->source_board_content_hash ||
-        r.compiler_profile_fingerprint != header->compiler_profile_fingerprint ||
-        r.rule_bucket_identity != header->rule_bucket_identity ||
-        r.device_view_fingerprint != header->device_view_fingerprint || r.reserved[0] != 0 || r.reserved[1] != 0)
-      return reject(qi, APGX_V1_CHECK_VALIDATION_FAILED, "",
-                    "GPU candidate batch result associations do not match its immutable query workspace");
-    const apgx_v1_query_envelope& env = ws.queries[qi];
-    if (env.workspace_offset != q.workspace_offset || env.workspace_state_count != S)
-      return invariant(qi, "gpu.batch.workspace.bounds.v1",
-                       "GPU candidate batch query-major workspace has invalid array bounds");
-    const uint64_t off = env.workspace_offset;
-    if (full) {
-      if (off > total_states || env.workspace_state_count > total_states - off)
-        return invariant(qi, "gpu.batch.workspace.bounds.v1",
-                         "GPU candidate batch query-major workspace has invalid array bounds");
-      for (uint64_t k = 0; k < S; ++k)
-        if (ws.state_owners[off + k] != q.workspace_owner || ws.predecessor_owners[off + k] != q.workspace_owner)
-          return invariant(qi, "gpu.batch.workspace.owner.v1",
-                           "GPU candidate batch workspace contains foreign query ownership");
-    } else {
-      const apgx_v1_compact_path& p = paths[qi];
-      const bool reached = r.completion == APGX_V1_REACHED;
-      const uint64_t end = p.state_offset + static_cast<uint64_t>(p.state_count);
-      if (p.schema_version != APGX_V1_SCHEMA_VERSION || p.query_id != q.query_id ||
-          p.workspace_owner != q.workspace_owner || p.state_count > S || end < p.state_offset || end > state_count ||
-          (reached ? p.state_count == 0 : p.state_count != 0))
-        return invariant(qi, "gpu.batch.compact_path.bounds.v1",
-                         "GPU compact sweep path has invalid associations or array bounds");
-    }
-    if (env.rounds != r.rounds || r.rounds > t.dispatched_rounds)
-      return invariant(qi, "gpu.batch.query.telemetry.v1",
-                       "GPU candidate query round telemetry disagrees with its result header or shared dispatch envelope");
-    if ((r.completion == APGX_V1_BUDGET_EXHAUSTED || r.completion == APGX_V1_CANCELLED) &&
-        t.finalization_launch_count != 1)
-      return invariant(qi, "gpu.batch.query.telemetry.v1",
-                       "GPU unfinished query completion lacks its shared finalization launch");
-    const unsigned __int128 max_examined =
-        static_cast<unsigned __int128>(env.rounds) *
-        (shape.fixed_examined_work_per_round + static_cast<unsigned __int128>(node_count) *
-                                                   shape.examined_work_per_node_per_round);
-    if (env.examined_work > max_examined ||
-        env.heading_turn_relaxations > max_examined * shape.turn_relaxations_per_examined_work)
-      return invariant(qi, "gpu.batch.query.telemetry.v1", "GPU candidate query work telemetry exceeds deterministic bounds");
-    // ValidateCompletionTelemetry (:456-484).
-    if (r.completion > APGX_V1_CANCELLED)
-      return invariant(qi, "gpu.completion.unknown.v1", "GPU result contains an unknown completion code");
-    if (r.rounds > request->maximum_rounds)
-      return invariant(qi, "", "GPU round telemetry exceeds the configured execution budget");
-    if (r.completion == APGX_V1_BUDGET_EXHAUSTED && r.rounds != request->maximum_rounds)
-      return invariant(qi, "", "GPU reported budget exhaustion before consuming its round budget");
-    if ((r.completion == APGX_V1_REACHED || r.completion == APGX_V1_DISCONNECTED) && r.rounds == 0)
-      return invariant(qi, "", "GPU reported a converged completion without an execution round");
-
-    const uint32_t start_state = q.start_node * 9u + 8u;
-    if (full) {
-      const uint64_t* lab = ws.labels + off;
-      const uint32_t* pre = ws.predecessors + off;
-      const bool exact = r.completion == APGX_V1_REACHED || r.completion == APGX_V1_DISCONNECTED;
-      std::string why;
-      const char* id = "";
-      if (!AllPredecessorsValid(c, q, lab, pre, exact, &why, &id)) return invariant(qi, id, why);
-      if (r.completion == APGX_V1_REACHED) {
-        uint64_t best = kInf;
-        uint32_t best_state = kNone;
-        for (uint32_t h = 0; h < 8; ++h) {
-          const uint32_t st = q.goal_node * 9u + h;
-          if (lab[st] < best || (lab[st] == best && st < best_state)) {
-            best = lab[st];
-            best_state = st;
-          }
-        }
-        if (best == kInf || r.goal_state != best_state)
-          return invariant(qi, "", "GPU goal state is not the stable minimum finite heading state");
-        uint32_t cursor = best_state;
-        for (uint64_t steps = 0;; ++steps) {
-          if (cursor >= S) return invariant(qi, "", "GPU reconstruction escaped the allocated state array");
-          if (cursor == start_state) break;
-          if (steps >= S) return invariant(qi, "", "GPU predecessor reconstruction exceeded its acyclic state bound");
-          cursor = pre[cursor];
-        }
-        if (path_costs) path_costs[qi] = best;
-      } else if (r.completion == APGX_V1_DISCONNECTED) {
-        for (uint32_t h = 0; h < 8; ++h)
-          if (lab[q.goal_node * 9u + h] != kInf)
-            return invariant(qi, "", "GPU reported disconnected with a finite goal label");
-        if (GoalReachable(c, q))
-          return invariant(qi, "gpu.disconnected.cpu_reachable.v1",
-                           "GPU reported disconnected for a query the CPU oracle reached under the identical policy");
-      }
-      continue;
-    }
-    // Compact path (ValidateAndReconstructCompactGpuPath, :969-1131).
-    const apgx_v1_compact_path& p = paths[qi];
-    if (r.completion == APGX_V1_DISCONNECTED && GoalReachable(c, q))
-      return invariant(qi, "gpu.disconnected.cpu_reachable.v1",
-                       "GPU reported disconnected for a query the CPU oracle reached under the identical policy");
-    if (r.completion != APGX_V1_REACHED) continue;
-    const uint32_t* rev = states + p.state_offset;
-    const uint32_t n = p.state_count;
-    if (rev[0] != r.goal_state || rev[n - 1] != start_state || r.goal_state >= S ||
-        r.goal_state / 9u != q.goal_node || r.goal_state % 9u >= 8u)
-      return invariant(qi, "gpu.batch.compact_path.endpoint.v1", "path does not bind the goal to the canonical start");
-    for (uint32_t k = 0; k < n; ++k) {
-      if (rev[k] >= S) return invariant(qi, "gpu.batch.compact_path.bounds.v1", "path state out of range");
-      visited[rev[k] / 64] = 0;
-    }
-    uint64_t total = 0;
-    for (uint32_t k = 0; k < n; ++k) {
-      const uint32_t st = rev[n - 1 - k];
-      const uint64_t mask = 1ull << (st % 64);
-      if (visited[st / 64] & mask) return invariant(qi, "gpu.batch.compact_path.cycle.v1", "repeated path state");
-      visited[st / 64] |= mask;
-      if (k == 0) continue;
-      const uint32_t pred = rev[n - k];
-      const uint32_t pnode = pred / 9u, node = st / 9u, inc = st % 9u, pinc = pred % 9u;
-      if (inc >= 8u || (pinc == 8u && pred != start_state))
-        return invariant(qi, "gpu.batch.compact_path.heading.v1", "noncanonical incoming heading");
-      if (!AdjacentLegal(c, pnode, node, inc))
-        return invariant(qi, "gpu.batch.compact_path.edge.v1", "path step is not one adjacent legal edge");
-      uint64_t w;
-      bool banned;
-      if (!Transition(c, q, pnode, inc, pinc, &w, &banned)) {
-        if (banned) return invariant(qi, "gpu.policy.banned_predecessor.v1", "path uses a banned resource");
-        return invariant(qi, "gpu.batch.compact_path.edge.v1", "path step cost is invalid or overflows");
-      }
-      if (!CheckedAdd(total, w, &total))
-        return invariant(qi, "gpu.batch.compact_path.edge.v1", "path cost overflows");
-    }
-    if (path_costs) path_costs[qi] = total;
-  }
-  out->valid = 1;
-  out->failure = APGX_V1_CHECK_OK;
   ClearError();
+  if (!v1::DescribeGenerator(request->generator, &b.shape)) {
+    Reject(out, kNone, APGX_V1_CHECK_VALIDATION_FAILED, "gpu.generator.unknown.v1",
+           "GPU candidate batch telemetry names an unknown generator");
+    return APGX_OK;
+  }
+  b.full = b.shape.full_workspace;
+  if (!CheckBatch(b, out)) return APGX_OK;
+  std::vector<uint64_t> visited((b.S + 63) / 64, 0);
+  for (uint32_t qi = 0; qi < query_count; ++qi) {
+    uint64_t cost;
+    if (!CheckQuery(b, qi, visited, &cost, out)) return APGX_OK;
+    if (path_costs) path_costs[qi] = cost;
+  }
+  Accept(out, kNone);
   return APGX_OK;
 }
+
+extern "C" int apgx_v1_validate_batch_queries(const apgx_v1_header* header, const apgx_v1_node* nodes,
+                                              uint64_t node_count, const apgx_v1_batch_request* request,
+                                              const apgx_v1_query* queries, uint32_t query_count,
+                                              const apgx_v1_policy_edge* policy, uint64_t policy_count,
+                                              const apgx_v1_result* results, const apgx_v1_compact_path* paths,
+                                              const uint32_t* states, uint64_t state_count,
+                                              const apgx_v1_workspace_view* workspace,
+                                              const apgx_v1_batch_telemetry* telemetry, uint64_t* path_costs,
+                                              apgx_v1_validation* verdicts, apgx_v1_validation* batch) {
+  using namespace apgx;
+  using namespace apgx::v1check;
+  if (batch == nullptr || (query_count && verdicts == nullptr))
+    return Fail(APGX_E_INTERNAL_INVARIANT, "apgx_v1_validate_batch_queries: null argument");
+  Batch b;
+  if (int e = MakeBatch(header, nodes, node_count, request, queries, query_count, policy, policy_count, results,
+                        paths, states, state_count, workspace, telemetry, "apgx_v1_validate_batch_queries", &b);
+      e != APGX_OK)
+    return e;
+  for (uint32_t i = 0; i < query_count && path_costs; ++i) path_costs[i] = kInf;
+  ClearError();
+  bool ok = v1::DescribeGenerator(request->generator, &b.shape);
+  if (!ok) {
+    Reject(batch, kNone, APGX_V1_CHECK_VALIDATION_FAILED, "gpu.generator.unknown.v1",
+           "GPU candidate batch telemetry names an unknown generator");
+  } else {
+    b.full = b.shape.full_workspace;
+    ok = CheckBatch(b, batch);
+  }
+  if (!ok) {
+    // A batch-level failure is every query's verdict (SetAdmittedBatchFailure).
+    for (uint32_t i = 0; i < query_count; ++i) verdicts[i] = *batch;
+    return APGX_OK;
+  }
+  Accept(batch, kNone);
+  // Contiguous partitions, one per worker; every verdict has its own slot, so
+  // the output does not depend on scheduling.  A worker that cannot be started
+  // runs on the caller.
+  const uint32_t workers = WorkerCount(query_count);
+  std::vector<uint64_t> costs(query_count, kInf);
+  const auto run = [&](uint32_t w) {
+    std::vector<uint64_t> visited((b.S + 63) / 64, 0);
+    const uint32_t begin = static_cast<uint32_t>(uint64_t{query_count} * w / workers);
+    const uint32_t end = static_cast<uint32_t>(uint64_t{query_count} * (w + 1) / workers);
+    for (uint32_t qi = begin; qi < end; ++qi) CheckQuery(b, qi, visited, &costs[qi], &verdicts[qi]);
+  };
+  std::vector<std::thread> threads;
+  uint32_t next = 0;
+  try {
+    for (; next + 1 < workers; ++next) threads.emplace_back(run, next);
+  } catch (const std::system_error&) {
+  }
+  for (; next < workers; ++next) run(next);
+  for (std::thread& t : threads) t.join();
+  for (uint32_t i = 0; i < query_count && path_costs; ++i) path_costs[i] = costs[i];
+  return APGX_OK;
+}
+
 
 namespace apgx {
 namespace v1route {

@@ -220,6 +220,10 @@ _SIGS = {
     "apgx_v1_validate_batch": (C.c_int, [P(Header), P(Node), U64, P(BatchRequest), P(Query), U32, P(PolicyEdge),
                                          U64, P(Result), P(CompactPath), P(U32), U64, P(WorkspaceView),
                                          P(BatchTelemetry), P(U64), P(Validation)]),
+    "apgx_v1_validate_batch_queries": (C.c_int, [P(Header), P(Node), U64, P(BatchRequest), P(Query), U32,
+                                                 P(PolicyEdge), U64, P(Result), P(CompactPath), P(U32), U64,
+                                                 P(WorkspaceView), P(BatchTelemetry), P(U64), P(Validation),
+                                                 P(Validation)]),
     "apgx_v1_route_execute": (C.c_int, [C.c_void_p, P(RouteRequest), P(U32), P(C.c_void_p)]),
     "apgx_v1_route_readback": (C.c_int, [C.c_void_p, P(RouteResult), P(U64), P(U32), U64]),
     "apgx_v1_route_pending_inject_fault": (C.c_int, [C.c_void_p, U32]),
@@ -466,10 +470,9 @@ def request_for(board: Board, *, batch_id=1, maximum_rounds=100_000, generator=G
     return BatchRequest(1, generator, batch_id, maximum_rounds, 0, S, S, maximum_device_bytes, 1 << 63)
 
 
-def validate(board: Board, request, queries, policy_edges, results, paths, states, telemetry, workspace=None):
-    """Host trust-boundary check of a readback (apgx_v1_validate_batch).
-    `workspace` None means a compact readback with canonical envelopes.
-    Returns (valid, query_index, invariant id, detail, path costs, failure class)."""
+def _validation_args(board: Board, request, queries, policy_edges, results, paths, states, telemetry, workspace):
+    """The leading arguments of apgx_v1_validate_batch[_queries] and the
+    ctypes objects they point into (kept alive by the caller)."""
     n = len(queries)
     S = len(board.nodes) * 9
     if workspace is None:
@@ -489,13 +492,39 @@ def validate(board: Board, request, queries, policy_edges, results, paths, state
     rarr = (Result * max(n, 1))(*results)
     carr = (CompactPath * max(n, 1))(*paths)
     sarr = (U32 * max(len(states), 1))(*states)
+    args = (C.byref(board.header), board.nodes, len(board.nodes), C.byref(request), qarr, n, parr,
+            len(policy_edges), rarr, carr, sarr, len(states), C.byref(wv), C.byref(telemetry))
+    return args, (env, arrays, wv, qarr, parr, rarr, carr, sarr)
+
+
+def validate(board: Board, request, queries, policy_edges, results, paths, states, telemetry, workspace=None):
+    """Host trust-boundary check of a readback (apgx_v1_validate_batch).
+    `workspace` None means a compact readback with canonical envelopes.
+    Returns (valid, query_index, invariant id, detail, path costs, failure class)."""
+    n = len(queries)
+    args, _keep = _validation_args(board, request, queries, policy_edges, results, paths, states, telemetry,
+                                   workspace)
     costs = (U64 * max(n, 1))()
     out = Validation()
-    check(_lib().apgx_v1_validate_batch(C.byref(board.header), board.nodes, len(board.nodes), C.byref(request),
-                                        qarr, n, parr, len(policy_edges), rarr, carr, sarr, len(states),
-                                        C.byref(wv), C.byref(telemetry), costs, C.byref(out)))
+    check(_lib().apgx_v1_validate_batch(*args, costs, C.byref(out)))
     return (bool(out.valid), out.query_index, out.invariant.decode(), out.detail.decode(), list(costs)[:n],
             out.failure)
+
+
+def validate_queries(board: Board, request, queries, policy_edges, results, paths, states, telemetry,
+                     workspace=None):
+    """Per-query verdicts of a readback (apgx_v1_validate_batch_queries).
+    Returns (batch verdict, [query verdicts], path costs); a verdict is a tuple
+    (valid, query_index, invariant id, detail, failure class)."""
+    n = len(queries)
+    args, _keep = _validation_args(board, request, queries, policy_edges, results, paths, states, telemetry,
+                                   workspace)
+    costs = (U64 * max(n, 1))()
+    verdicts = (Validation * max(n, 1))()
+    batch = Validation()
+    check(_lib().apgx_v1_validate_batch_queries(*args, costs, verdicts, C.byref(batch)))
+    as_tuple = lambda v: (bool(v.valid), v.query_index, v.invariant.decode(), v.detail.decode(), v.failure)
+    return as_tuple(batch), [as_tuple(verdicts[i]) for i in range(n)], list(costs)[:n]
 
 
 def generator_shape(generator) -> GeneratorShape:

@@ -351,10 +351,16 @@ int apgx_autotune(apgx_lattice* lattice, const apgx_net_query* nets, uint32_t ne
  * disjoint windows (a net's level is one above the highest level of an
  * earlier net whose window overlaps it), so a level routed at once equals its
  * nets routed one after another, with less latency.  All arithmetic is integer
- * and runs on the GPU. */
+ * and runs on the GPU.  A pass whose present factor is zero prices no cell by
+ * its usage, so its snapshot costs do not depend on any route: its nets are
+ * routed as one batch in every mode, with the same routes as in any order. */
 #define APGX_NEGOTIATION_FEASIBLE 0u
 #define APGX_NEGOTIATION_STALLED 1u
 #define APGX_NEGOTIATION_PASS_LIMIT 2u
+
+/* Progress of a negotiation, reported after every completed pass. */
+typedef void (*apgx_negotiation_pass_fn)(void* context, uint32_t pass, uint64_t overuse_units,
+                                         uint32_t routed_nets, double wall_ms);
 
 typedef struct apgx_negotiation_policy {
   uint32_t maximum_passes;                 /* default 8 */
@@ -363,6 +369,8 @@ typedef struct apgx_negotiation_policy {
   uint64_t present_cost_increment;         /* default 100 */
   uint64_t historical_cost_increment;      /* default 10 */
   uint64_t maximum_congestion_cost_value;  /* default 10,000,000 */
+  apgx_negotiation_pass_fn on_pass;        /* optional, called on the calling thread (default NULL) */
+  void* on_pass_context;
 } apgx_negotiation_policy;
 
 typedef struct apgx_negotiation_summary {
@@ -914,13 +922,15 @@ void apgx_v1_pending_release(apgx_v1_pending* pending);
 
 /* Host trust boundary: re-checks an untrusted readback before any result is
  * used, as RouteCandidateBatchWithPreparedPreflight does (src/gpu/
- * planar_router.cc:2195-2380).  Order: generator; batch query identity (every
- * result names one admitted query, :2195-2218); batch telemetry (readback kind
+ * planar_router.cc:2195-2380).  Order: generator; batch query identity (no
+ * result names a foreign query, :2195-2218); batch telemetry (readback kind
  * and flat bounds, compact contiguity, byte accounting, peak / chunk / launch /
  * readback counts: ValidateCandidateBatchTelemetry, :1841-1955); then per query
  * the envelope (header echo, workspace bounds, ownership stamps or compact
- * header, round / work telemetry: ValidateCandidateBatchEnvelope, :1732-1839)
- * and the result itself.  Compact paths (sweep): endpoints, cycles, headings,
+ * header, round / work telemetry: ValidateCandidateBatchEnvelope, :1732-1839;
+ * results out of query order are matched by query id, and a query whose
+ * result is missing or duplicated fails gpu.batch.query.identity.v1) and the
+ * result itself.  Compact paths (sweep): endpoints, cycles, headings,
  * edges, banned resources and cost (:969-1131).  Full workspace (frontier):
  * every predecessor exact for reached / disconnected and a lower bound
  * otherwise (ValidateAllPredecessors, :491-612), the goal state as the least
@@ -970,6 +980,31 @@ int apgx_v1_validate_batch(const apgx_v1_header* header, const apgx_v1_node* nod
                            uint64_t state_count, const apgx_v1_workspace_view* workspace,
                            const apgx_v1_batch_telemetry* telemetry, uint64_t* path_costs,
                            apgx_v1_validation* out);
+
+/* The same checks with a verdict per query, as the reference's host router
+ * returns one result per admitted query (src/gpu/planar_router.cc:2185-2436):
+ * a corrupted query fails while its peers keep theirs.  `batch` gets the
+ * batch-level verdict: the result set must name the admitted queries (a
+ * foreign query id fails the batch), then the readback kind, flat bounds,
+ * compact-path storage and memory / launch telemetry; on a batch failure
+ * every verdicts[i] is that failure.  Otherwise batch->valid = 1 and
+ * verdicts[i] (query_index i) is query i's own: valid, or its first failing
+ * check as for apgx_v1_validate_batch.  Results are matched to queries by
+ * query id when they are not in query order; a query whose result is missing
+ * or came back twice fails with gpu.batch.query.identity.v1.  Queries are
+ * checked by min(8, ceil(query_count / 64)) host threads over contiguous
+ * partitions (the reference's CandidateCompactValidationWorkerCountV1,
+ * include/apgar/gpu/planar_router.h:439-450); the output does not depend on
+ * their scheduling.  path_costs as for apgx_v1_validate_batch, for every
+ * valid reached query.  Returns APGX_OK unless an argument is null. */
+int apgx_v1_validate_batch_queries(const apgx_v1_header* header, const apgx_v1_node* nodes, uint64_t node_count,
+                                   const apgx_v1_batch_request* request, const apgx_v1_query* queries,
+                                   uint32_t query_count, const apgx_v1_policy_edge* policy_edges,
+                                   uint64_t policy_edge_count, const apgx_v1_result* results,
+                                   const apgx_v1_compact_path* paths, const uint32_t* states,
+                                   uint64_t state_count, const apgx_v1_workspace_view* workspace,
+                                   const apgx_v1_batch_telemetry* telemetry, uint64_t* path_costs,
+                                   apgx_v1_validation* verdicts, apgx_v1_validation* batch);
 
 /* This backend's execution shape and deterministic byte accounting, for a
  * host that validates telemetry (the reference's DescribePlanarGenerator,

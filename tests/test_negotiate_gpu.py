@@ -84,6 +84,20 @@ def test_bottleneck_board(apgx, oracle, gpu):
     lat.close()
 
 
+def test_pass_callback_reports_every_pass(apgx, gpu):
+    _say("[case] allocator per-pass callback")
+    cost, nets, pins = make_board(apgx, SIX_LAYER_1536, 1003, **SMALL)
+    lat = apgx.Lattice(cost, via_cost=SIX_LAYER_1536["via"], cost_bytes=2)
+    seen = []
+    g = lat.negotiate(nets, pins, maximum_passes=6, batch_nets=0,
+                      on_pass=lambda p, o, r, ms: seen.append((p, o, r, ms)))
+    assert [p for p, _, _, _ in seen] == list(range(g["passes"]))
+    assert [o for _, o, _, _ in seen] == g["overuse_per_pass"]
+    assert seen[-1][2] == g["routed"]
+    assert all(ms > 0 for *_, ms in seen) and seen[-1][3] <= g["wall_ms"]
+    lat.close()
+
+
 def test_value_bound_is_reported(apgx, gpu):
     cost = np.array([[[1, 5, 1]]], dtype=np.uint32)
     nets, p = apgx.nets_from_lists([[0, 2]])

@@ -120,3 +120,29 @@ def test_occupancy_and_overuse_follow_from_routes(oracle, seed):
     assert int(hist.sum()) == 10 * sum(r["overuse_per_pass"][:-1])
     if r["termination"] == 0:
         assert r["overuse"] == 0 and r["routed"] == len(nets)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_zero_present_factor_pass_is_order_free(oracle, seed):
+    """A pass with present factor 0 prices cells by base + history only, so its
+    routes do not depend on the order or batching of its nets: one pass in
+    batches of 1 (the reference's order) equals one batch of every net.  (The
+    library routes such passes as one batch.)"""
+    rng = np.random.default_rng(500 + seed)
+    cost = random_lattice(rng, 2, 24, 30, blocked=0.15, cmax=6)
+    free = np.flatnonzero(cost.reshape(-1))
+    pins = rng.choice(free, size=24, replace=False).astype(np.uint32)
+    nets = [(2 * i, 2, None) for i in range(12)]
+    history = rng.integers(0, 40, cost.size).astype(np.uint32)
+    a = oracle.negotiate(cost, 5, nets, pins, maximum_passes=1, batch_nets=1, initial_history=history)
+    b = oracle.negotiate(cost, 5, nets, pins, maximum_passes=1, batch_nets=len(nets), initial_history=history)
+    assert a["status"] == b["status"] == 0
+    assert a["overuse_per_pass"] == b["overuse_per_pass"]
+    assert [list(x) if x is not None else None for x in a["routes"]] == \
+           [list(x) if x is not None else None for x in b["routes"]]
+    # With a nonzero present factor the order matters: nets routed one at a
+    # time see each other's occupancy and avoid it; one batch does not.
+    kw = dict(maximum_passes=1, initial_history=history, initial_present_cost=50)
+    c = oracle.negotiate(cost, 5, nets, pins, batch_nets=1, **kw)
+    d = oracle.negotiate(cost, 5, nets, pins, batch_nets=len(nets), **kw)
+    assert c["overuse_per_pass"][0] < d["overuse_per_pass"][0]

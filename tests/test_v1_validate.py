@@ -143,7 +143,7 @@ def test_each_fault_is_reported(apgx, oracle, fault):
     b, req, queries, rb, _ = oracle_readback(oracle, "high_turn_maze", generator)
     ok, qi, inv, detail, _, failure = _validate(b, req, queries, inject(fault, rb))
     assert not ok and inv == want and failure == v1.CHECK_INTERNAL_INVARIANT, (inv, detail)
-    assert qi == (0xFFFFFFFF if want in BATCH_LEVEL else 0)
+    assert qi == (0xFFFFFFFF if want in BATCH_LEVEL else 0)  # query_identity: query 0's result is missing
 
 
 def test_fault_leaves_original_intact(apgx, oracle):
@@ -239,8 +239,16 @@ def test_gpu_readbacks_validate_and_faults_are_caught(apgx, gpu, generator):
     for fault in FAULTS:
         if CONTRACT[fault][0] != generator:
             continue
-        ok, qi, inv, detail, _, _ = _validate(b, req, queries, inject(fault, rb))
+        bad = inject(fault, rb)
+        ok, qi, inv, detail, _, _ = _validate(b, req, queries, bad)
         assert not ok and inv == CONTRACT[fault][1], (fault, inv, detail)
+        batch, verdicts, got = _verdicts(b, req, queries, bad)
+        if CONTRACT[fault][1] in BATCH_LEVEL:
+            assert not batch[0] and verdicts == [batch] * len(queries)
+            continue
+        failed = {0, 1} if fault == "query_identity" else {0}
+        assert batch[0] and all(v[0] != (i in failed) for i, v in enumerate(verdicts)), fault
+        assert all(got[i] == costs[i] for i in range(len(queries)) if i not in failed), fault
 
 
 def _same_readback(a, b):
@@ -305,3 +313,89 @@ def test_expected_batch_bytes(apgx, generator, qn, pn, nn, used):
     S = qn * nn * 9
     readback = qn * 120 + (S * 28 if frontier else qn * 32 + 4 * used)
     assert (d, h, r) == (layout_bytes(frontier, qn, pn, nn), readback + qn * 120, readback)
+
+
+def _verdicts(b, req, queries, rb, policy=()):
+    return v1.validate_queries(b, req, queries, list(policy), rb.results, rb.paths, rb.states, rb.telemetry,
+                               rb.workspace)
+
+
+@pytest.mark.parametrize("fault", FAULTS)
+def test_per_query_verdicts_isolate_a_corrupted_query(apgx, oracle, fault):
+    """apgx_v1_validate_batch_queries: a query-level fault fails its own
+    query(ies) while the peers keep their validated costs; a batch-level fault
+    is every query's verdict (src/gpu/planar_router.cc:2185-2436)."""
+    generator, want = CONTRACT[fault]
+    b, req, queries, rb, costs = oracle_readback(oracle, "high_turn_maze", generator)
+    clean = [c if c is not None else INF for c in costs]
+    batch, verdicts, got = _verdicts(b, req, queries, rb)
+    assert batch[0] and all(v[0] for v in verdicts) and got == clean
+    assert [v[1] for v in verdicts] == list(range(len(queries)))
+    batch, verdicts, got = _verdicts(b, req, queries, inject(fault, rb))
+    if want in BATCH_LEVEL:
+        assert not batch[0] and batch[2] == want and batch[1] == 0xFFFFFFFF
+        assert verdicts == [batch] * len(queries)
+        assert got == [INF] * len(queries)
+        return
+    failed = {0, 1} if fault == "query_identity" else {0}
+    assert batch[0]
+    for i, v in enumerate(verdicts):
+        if i in failed:
+            assert not v[0] and v[1] == i and v[2] == want and v[4] == v1.CHECK_INTERNAL_INVARIANT, v
+            assert got[i] == INF
+        else:
+            assert v[0] and v[4] == v1.CHECK_OK and got[i] == clean[i], (i, v)
+
+
+def test_foreign_query_id_fails_the_batch(apgx, oracle):
+    b, req, queries, rb, _ = oracle_readback(oracle, "high_turn_maze", v1.GENERATOR_FRONTIER)
+    r2 = inject("query_identity", rb)
+    r2.results[0].query_id = 999_999
+    batch, verdicts, _ = _verdicts(b, req, queries, r2)
+    assert not batch[0] and batch[1] == 0xFFFFFFFF and batch[2] == "gpu.batch.query.identity.v1"
+    assert all(v == batch for v in verdicts)
+    assert _validate(b, req, queries, r2)[1:3] == (0xFFFFFFFF, "gpu.batch.query.identity.v1")
+    # A batch of one: the next query's id is foreign there.
+    b1, req1, q1, rb1, _ = oracle_readback(oracle, "high_turn_maze", v1.GENERATOR_FRONTIER, k=0)
+    assert len(q1) == 1
+    assert _validate(b1, req1, q1, inject("query_identity", rb1))[1:3] == (0xFFFFFFFF, "gpu.batch.query.identity.v1")
+
+
+def test_results_out_of_query_order_are_matched_by_id(apgx, oracle):
+    """A full-workspace readback whose results (and their envelopes) come back
+    permuted validates query by query, as the reference maps them by id."""
+    b, req, queries, rb, costs = oracle_readback(oracle, "high_turn_maze", v1.GENERATOR_FRONTIER)
+    order = [3, 0, 4, 1, 2]
+    ws = rb.workspace.copy()
+    ws.envelopes = [rb.workspace.envelopes[i] for i in order]
+    perm = v1.Readback([rb.results[i] for i in order], [rb.paths[i] for i in order], rb.states, rb.telemetry, ws)
+    batch, verdicts, got = _verdicts(b, req, queries, perm)
+    assert batch[0] and all(v[0] for v in verdicts)
+    assert got == [c if c is not None else INF for c in costs]
+    ok, qi, inv, detail, got1, _ = _validate(b, req, queries, perm)
+    assert ok and got1 == got, (qi, inv, detail)
+
+
+def test_many_query_batch_uses_workers_and_keeps_peers(apgx, oracle):
+    """200 compact queries (4 validation workers): two corrupted queries in
+    different partitions fail, every other query keeps its published cost."""
+    b, req, queries, rb, costs = oracle_readback(oracle, "high_turn_maze", v1.GENERATOR_SWEEP, k=199)
+    assert len(queries) == 200
+    clean = [c if c is not None else INF for c in costs]
+    reached = [i for i, r in enumerate(rb.results) if r.completion == v1.REACHED and rb.paths[i].state_count >= 4]
+    a, z = reached[1], reached[-1]
+    assert a < 100 <= z
+    states = list(rb.states)
+    for i in (a, z):
+        off = rb.paths[i].state_offset
+        states[off + 1] = states[off + 2]  # a repeated state
+    bad = v1.Readback(rb.results, rb.paths, states, rb.telemetry, rb.workspace)
+    batch, verdicts, got = _verdicts(b, req, queries, bad)
+    assert batch[0]
+    for i, v in enumerate(verdicts):
+        if i in (a, z):
+            assert not v[0] and v[2] == "gpu.batch.compact_path.cycle.v1" and got[i] == INF
+        else:
+            assert v[0] and got[i] == clean[i]
+    first = _validate(b, req, queries, bad)
+    assert not first[0] and first[1] == a
