@@ -408,6 +408,7 @@ def search_workload(args, workload, steps, warmup, cpu):
     st = all_stats[-1]
     L = cost.shape[0]
     cfg = run_config(workload, opts, args.landmarks)
+    log(f"[bench] {workload} profile config: {json.dumps(cfg)}")
     traffic, tsrc = read_profile("traffic", cfg)
     traffic_bytes = None
     if traffic:
