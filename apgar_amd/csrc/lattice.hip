@@ -659,6 +659,7 @@ constexpr int kRP = kRT + 1;     // LDS pitch of the transpose buffers
 constexpr uint32_t kBlocked8 = 0xFFFFFFFFu;
 constexpr uint32_t kBlockedHalf = 0xFFFFu;  // packed cost half of a blocked cell
 constexpr uint32_t kRegMaxCost = 4095;      // (cost << 3) must stay a positive int16
+constexpr uint32_t kRegWideMaxCost = 32767; // WIDE tiles: the unshifted cost must stay a positive int16
 
 __device__ __forceinline__ uint32_t Relax8(uint32_t carry, uint32_t c8, uint32_t code) {
   return (__builtin_elementwise_add_sat(carry, c8) & ~7u) | code;
@@ -679,15 +680,31 @@ __device__ __forceinline__ uint32_t CostLo(uint32_t pair) {
 __device__ __forceinline__ uint32_t CostHi(uint32_t pair) {
   return static_cast<uint32_t>(static_cast<int32_t>(pair) >> 16);
 }
-__device__ __forceinline__ uint32_t PackCost(uint32_t c) { return c ? c << 3 : kBlockedHalf; }
-template <int I>
-__device__ __forceinline__ uint32_t CostAt(const uint32_t* pairs) {
-  return (I & 1) ? CostHi(pairs[I >> 1]) : CostLo(pairs[I >> 1]);
+// WIDE tiles hold the unshifted cost (up to kRegWideMaxCost) and shift it at
+// each relaxation: one more VALU op, off the carry chain (it does not depend on
+// the carry).  A blocked half (-1) shifts to 0xFFFFFFF8, which saturates the
+// add for every label (all labels are at least 7), as all ones does.
+template <bool WIDE>
+__device__ __forceinline__ uint32_t PackCost(uint32_t c) {
+  return c ? (WIDE ? c : c << 3) : kBlockedHalf;
+}
+// Makes the packed cost registers opaque to the compiler, so that values
+// derived from them are recomputed after this point instead of kept live.
+__device__ __forceinline__ void RegBarrier(uint32_t (&pairs)[32]) {
+#pragma unroll
+  for (int i = 0; i < 32; ++i) asm volatile("" : "+v"(pairs[i]));
+}
+template <bool WIDE>
+__device__ __forceinline__ uint32_t Cost8(uint32_t pair, bool hi) {
+  const uint32_t s = hi ? CostHi(pair) : CostLo(pair);
+  return WIDE ? s << 3 : s;
 }
 
 // PAYS: vias also pay the entered cell's cost (allocator searches); a
 // template parameter so the plain kernel's register allocation is untouched.
-template <int L, bool PAYS>
+// WIDE: costs up to kRegWideMaxCost (allocator snapshots under a high present
+// factor), held unshifted.
+template <int L, bool PAYS, bool WIDE = false>
 __global__ __launch_bounds__(64 * L) __attribute__((amdgpu_waves_per_eu(2)))
 void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
   extern __shared__ uint32_t smem[];
@@ -746,7 +763,7 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
 #pragma unroll
       for (int y = 0; y < kRT; ++y) {
         const uint32_t c = cost_bytes == 2 ? cbase16[y * a.W + lane] : cbase32[y * a.W + lane];
-        const uint32_t e = PackCost(c);
+        const uint32_t e = PackCost<WIDE>(c);
         if (y & 1) cst[y >> 1] |= e << 16; else cst[y >> 1] = e;
       }
       if (initd) {
@@ -780,7 +797,7 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
         const uint32_t coff = row * a.W + xc;
         uint32_t c = cost_bytes == 2 ? cbase16[coff] : cbase32[coff];
         c = (xin && y < ny) ? c : 0u;
-        T[y * kRP + lane] = PackCost(c);
+        T[y * kRP + lane] = PackCost<WIDE>(c);
       }
       WaveSync();
 #pragma unroll
@@ -842,14 +859,15 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
       uint32_t carry = halo_s;
 #pragma unroll
       for (int y = 0; y < kRT; ++y) {
-        const uint32_t c8 = (y & 1) ? CostHi(cst[y >> 1]) : CostLo(cst[y >> 1]);
+        const uint32_t c8 = Cost8<WIDE>(cst[y >> 1], y & 1);
         lab[y] = min(lab[y], Relax8(carry, c8, kFromSouth));
         carry = lab[y];
       }
+      if constexpr (WIDE) RegBarrier(cst);  // recompute the shifted costs: no 64 live copies
       carry = halo_n;
 #pragma unroll
       for (int y = kRT - 1; y >= 0; --y) {
-        const uint32_t c8 = (y & 1) ? CostHi(cst[y >> 1]) : CostLo(cst[y >> 1]);
+        const uint32_t c8 = Cost8<WIDE>(cst[y >> 1], y & 1);
         lab[y] = min(lab[y], Relax8(carry, c8, kFromNorth));
         carry = lab[y];
       }
@@ -861,14 +879,15 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
       carry = halo_w;
 #pragma unroll
       for (int x = 0; x < kRT; ++x) {
-        const uint32_t c8 = (x & 1) ? CostHi(cstr[x >> 1]) : CostLo(cstr[x >> 1]);
+        const uint32_t c8 = Cost8<WIDE>(cstr[x >> 1], x & 1);
         lab[x] = min(lab[x], Relax8(carry, c8, kFromWest));
         carry = lab[x];
       }
+      if constexpr (WIDE) RegBarrier(cstr);
       carry = halo_e;
 #pragma unroll
       for (int x = kRT - 1; x >= 0; --x) {
-        const uint32_t c8 = (x & 1) ? CostHi(cstr[x >> 1]) : CostLo(cstr[x >> 1]);
+        const uint32_t c8 = Cost8<WIDE>(cstr[x >> 1], x & 1);
         lab[x] = min(lab[x], Relax8(carry, c8, kFromEast));
         carry = lab[x];
       }
@@ -891,11 +910,12 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
       if (a.via != 0) {  // the host passes via = 0 for single-layer lattices
         __syncthreads();  // every layer published in its buffer
         if constexpr (PAYS) {
+          if constexpr (WIDE) RegBarrier(cst);
           // Via weight via + cost of the entered cell: the packed cost half
           // (all ones when blocked) saturates the sum for blocked cells.
 #pragma unroll
           for (int y = 0; y < kRT; ++y) {
-            const uint32_t c8 = (y & 1) ? CostHi(cst[y >> 1]) : CostLo(cst[y >> 1]);
+            const uint32_t c8 = Cost8<WIDE>(cst[y >> 1], y & 1);
             const uint32_t wb = __builtin_elementwise_add_sat(via_b, c8);
             const uint32_t wa = __builtin_elementwise_add_sat(via_a, c8);
             const uint32_t fb = __builtin_elementwise_add_sat(below[y * kRP + lane], wb) & ~7u;  // kFromBelow = 0
@@ -1061,23 +1081,27 @@ size_t RegTileLdsBytes(uint32_t layers) {
   return static_cast<size_t>(layers) * (kRT * kRP + 2 * kRT) * 4;
 }
 
-template <bool PAYS>
+template <bool PAYS, bool WIDE>
 const void* RegKernelForT(uint32_t layers) {
   switch (layers) {
-    case 1: return reinterpret_cast<const void*>(&RegRoundKernel<1, PAYS>);
-    case 2: return reinterpret_cast<const void*>(&RegRoundKernel<2, PAYS>);
-    case 3: return reinterpret_cast<const void*>(&RegRoundKernel<3, PAYS>);
-    case 4: return reinterpret_cast<const void*>(&RegRoundKernel<4, PAYS>);
-    case 5: return reinterpret_cast<const void*>(&RegRoundKernel<5, PAYS>);
-    case 6: return reinterpret_cast<const void*>(&RegRoundKernel<6, PAYS>);
-    case 7: return reinterpret_cast<const void*>(&RegRoundKernel<7, PAYS>);
-    case 8: return reinterpret_cast<const void*>(&RegRoundKernel<8, PAYS>);
+    case 1: return reinterpret_cast<const void*>(&RegRoundKernel<1, PAYS, WIDE>);
+    case 2: return reinterpret_cast<const void*>(&RegRoundKernel<2, PAYS, WIDE>);
+    case 3: return reinterpret_cast<const void*>(&RegRoundKernel<3, PAYS, WIDE>);
+    case 4: return reinterpret_cast<const void*>(&RegRoundKernel<4, PAYS, WIDE>);
+    case 5: return reinterpret_cast<const void*>(&RegRoundKernel<5, PAYS, WIDE>);
+    case 6: return reinterpret_cast<const void*>(&RegRoundKernel<6, PAYS, WIDE>);
+    case 7: return reinterpret_cast<const void*>(&RegRoundKernel<7, PAYS, WIDE>);
+    case 8: return reinterpret_cast<const void*>(&RegRoundKernel<8, PAYS, WIDE>);
     default: return nullptr;
   }
 }
 
-const void* RegKernelFor(uint32_t layers, bool via_pays_cell) {
-  return via_pays_cell ? RegKernelForT<true>(layers) : RegKernelForT<false>(layers);
+// Wide register tiles are built for allocator searches only (vias paying the
+// entered cell), where snapshot costs grow with the present factor; plain
+// searches with costs above kRegMaxCost run LDS tiles.
+const void* RegKernelFor(uint32_t layers, bool via_pays_cell, uint32_t max_cost) {
+  if (!via_pays_cell) return RegKernelForT<false, false>(layers);
+  return max_cost > kRegMaxCost ? RegKernelForT<true, true>(layers) : RegKernelForT<true, false>(layers);
 }
 
 __device__ __forceinline__ uint64_t LocalIndex(const NetStatic& ns, uint32_t W, uint32_t H,
@@ -1663,13 +1687,14 @@ namespace apgx {
 namespace {
 
 // Register-resident 64 x 64 tiles: one wave per layer, up to eight layers,
-// cell costs up to kRegMaxCost.
-bool RegTileUsable(uint32_t layers, uint32_t max_cost, size_t lds_limit) {
-  return layers <= 8 && max_cost <= kRegMaxCost && RegTileLdsBytes(layers) <= lds_limit;
+// cell costs up to kRegMaxCost, or kRegWideMaxCost in allocator searches.
+bool RegTileUsable(uint32_t layers, uint32_t max_cost, size_t lds_limit, bool via_pays_cell) {
+  const uint32_t limit = via_pays_cell ? kRegWideMaxCost : kRegMaxCost;
+  return layers <= 8 && max_cost <= limit && RegTileLdsBytes(layers) <= lds_limit;
 }
 
-TileShape DefaultShape(const apgx_lattice_desc& d, uint32_t max_cost, size_t lds_limit) {
-  if (RegTileUsable(d.layers, max_cost, lds_limit)) return TileShape{64, 64};
+TileShape DefaultShape(const apgx_lattice_desc& d, uint32_t max_cost, size_t lds_limit, bool via_pays_cell) {
+  if (RegTileUsable(d.layers, max_cost, lds_limit, via_pays_cell)) return TileShape{64, 64};
   const TileShape candidates[] = {{64, 64}, {64, 32}, {32, 32}, {32, 16}};
   // Prefer the largest tile that fits the register-resident cell budget and
   // leaves room for two workgroups per CU.
@@ -1795,16 +1820,17 @@ int apgx::RouteNets(apgx_lattice* lat, const CostOverride* ov, const apgx_net_qu
   const size_t lds_limit = lat->lds_limit;
   const uint32_t max_cost = ov ? ov->max_cost : lat->max_cell_cost;
   const uint32_t cost_bytes = ov ? 4u : lat->desc.cost_bytes;
-  TileShape shape = DefaultShape(lat->desc, max_cost, lds_limit);
+  const bool pays = ov != nullptr && ov->via_pays_cell;
+  TileShape shape = DefaultShape(lat->desc, max_cost, lds_limit, pays);
   if (opt.tile_w || opt.tile_h) shape = TileShape{static_cast<int>(opt.tile_w), static_cast<int>(opt.tile_h)};
   Kernels k{};
   if (!KernelsForShape(shape, &k))
     return Fail(APGX_E_UNSUPPORTED, "apgx_route_nets: unsupported tile shape");
-  const bool reg_tile = shape.tx == 64 && shape.ty == 64 && RegTileUsable(d.layers, max_cost, lds_limit);
+  const bool reg_tile = shape.tx == 64 && shape.ty == 64 && RegTileUsable(d.layers, max_cost, lds_limit, pays);
   uint32_t block = kThreads;
   size_t lds = TileLdsBytes(shape, d.layers);
   if (reg_tile) {
-    k.round = RegKernelFor(d.layers, ov != nullptr && ov->via_pays_cell);
+    k.round = RegKernelFor(d.layers, pays, max_cost);
     block = 64 * d.layers;
     lds = RegTileLdsBytes(d.layers);
   } else if (static_cast<uint64_t>(shape.tx) * shape.ty * d.layers > kMaxTileCells) {
@@ -2321,7 +2347,7 @@ extern "C" int apgx_autotune(apgx_lattice* lat, const apgx_net_query* nets, uint
   for (uint32_t i = 0; i < sample; ++i)
     picked[i] = nets[static_cast<uint64_t>(i) * net_count / std::max(sample, 1u)];
   const size_t lds_limit = lat->lds_limit;
-  const bool reg = RegTileUsable(lat->desc.layers, lat->max_cell_cost, lds_limit);
+  const bool reg = RegTileUsable(lat->desc.layers, lat->max_cell_cost, lds_limit, false);
   const uint32_t mean = static_cast<uint32_t>(std::max(1.0, lat->mean_cost));
   std::vector<apgx_tune_candidate> cands;
   const uint32_t deltas[] = {16 * mean, 48 * mean, 256 * mean, 0xFFFFFFFFu};

@@ -20,5 +20,5 @@ def test_register_tile_kernels_do_not_spill():
         text = out.read_text()
     meta = re.findall(r"\.name:\s+(\S+)\n(?:.*\n)*?\s+\.vgpr_spill_count:\s+(\d+)", text)
     reg = [(n, int(c)) for n, c in meta if "RegRoundKernel" in n]
-    assert len(reg) == 16  # 1..8 layers, with and without via-pays-cell
+    assert len(reg) == 24  # 1..8 layers: plain, via-pays-cell, via-pays-cell with wide costs
     assert all(c == 0 for _, c in reg), reg

@@ -55,6 +55,24 @@ def test_small_boards_match_oracle(apgx, oracle, gpu, seed, batch):
     lat.close()
 
 
+@pytest.mark.parametrize("seed", [7, 8])
+def test_high_present_costs_match_oracle(apgx, oracle, gpu, seed):
+    """Present factors of 6,000 + 3,000 per pass: snapshot costs cross the
+    register tiles' narrow limit (4,095, costs kept pre-shifted) into the wide
+    range (up to 32,767, shifted at each relaxation) and, where a cell is
+    overused twice, beyond it (LDS tiles); every pass bit-exact."""
+    _say(f"[case] allocator parity under high present costs seed={seed}")
+    cost, nets, pins = make_board(apgx, SIX_LAYER_1536, 1000 + seed, **SMALL)
+    via = SIX_LAYER_1536["via"]
+    lat = apgx.Lattice(cost, via_cost=via, cost_bytes=2)
+    kw = dict(maximum_passes=6, initial_present_cost=6000, present_cost_increment=3000)
+    g = lat.negotiate(nets, pins, batch_nets=0, **kw)
+    o = oracle.negotiate(cost, via, oracle_nets(nets), pins, batch_nets=1, **kw)
+    assert o["status"] == 0
+    assert_same(g, o)
+    lat.close()
+
+
 def test_seeded_history_and_capacity_match_oracle(apgx, oracle, gpu):
     _say("[case] allocator parity with seeded history and capacity plane")
     cost, nets, pins = make_board(apgx, SIX_LAYER_1536, 1010, **SMALL)
