@@ -102,14 +102,16 @@ class NegotiationPolicy(C.Structure):
                 ("initial_present_cost", C.c_uint64), ("present_cost_increment", C.c_uint64),
                 ("historical_cost_increment", C.c_uint64),
                 ("maximum_congestion_cost_value", C.c_uint64), ("on_pass", NEGOTIATION_PASS_FN),
-                ("on_pass_context", C.c_void_p)]
+                ("on_pass_context", C.c_void_p), ("group_nets", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class NegotiationSummary(C.Structure):
     _fields_ = [("termination", C.c_uint32), ("completed_passes", C.c_uint32),
                 ("routed_nets", C.c_uint32), ("batches", C.c_uint32),
                 ("total_overuse_units", C.c_uint64), ("tile_visits", C.c_uint64),
-                ("search_device_ms", C.c_double), ("wall_ms", C.c_double)]
+                ("search_device_ms", C.c_double), ("wall_ms", C.c_double),
+                ("search_rounds", C.c_uint64), ("group_routed_nets", C.c_uint64),
+                ("redone_nets", C.c_uint64)]
 
 
 class BoardGenParams(C.Structure):
@@ -548,7 +550,7 @@ class Lattice:
                   search_options=None, **policy):
         """Negotiated-congestion allocator (apgx_negotiate).  policy keywords:
         maximum_passes, batch_nets, initial_present_cost, present_cost_increment,
-        historical_cost_increment, maximum_congestion_cost_value, and on_pass
+        historical_cost_increment, maximum_congestion_cost_value, group_nets, and on_pass
         (a callable (pass, overuse, routed nets, wall ms) run after every pass).  Returns a dict
         like the oracle's: termination, passes, routed, overuse, overuse_per_pass,
         history / usage planes (L, H, W) and per-net tree cells (None: unrouted)."""
@@ -605,6 +607,8 @@ class Lattice:
                 "overuse_per_pass": [int(v) for v in per_pass[:summ.completed_passes]],
                 "batches": summ.batches, "tile_visits": summ.tile_visits,
                 "search_device_ms": summ.search_device_ms, "wall_ms": summ.wall_ms,
+                "search_rounds": summ.search_rounds, "group_routed_nets": summ.group_routed_nets,
+                "redone_nets": summ.redone_nets,
                 "history": hist.reshape(L, H, W), "occupancy": usage.reshape(L, H, W),
                 "routes": routes}
 

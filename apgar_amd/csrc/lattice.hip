@@ -79,6 +79,10 @@ struct NetStatic {
   uint32_t pin_count;
   uint32_t conn_offset;
   uint64_t label_offset;
+  // Element offset of this net's cost plane from SearchArgs::cost: nets of
+  // one search may each read their own plane (an allocator's speculative
+  // group routes every net on its own snapshot); 0 for a shared plane.
+  uint64_t cost_offset;
 };
 
 struct NetState {
@@ -419,7 +423,7 @@ __device__ void ProcessTile(const SearchArgs& a, uint32_t r, uint32_t g, uint32_
       uint32_t c = 0, v = kInf;
       if (x < nx && y < ny) {
         const uint64_t gidx =
-            (static_cast<uint64_t>(l) * a.H + ns.y0 + by + y) * a.W + ns.x0 + bx + x;
+            ns.cost_offset + (static_cast<uint64_t>(l) * a.H + ns.y0 + by + y) * a.W + ns.x0 + bx + x;
         c = cost_bytes == 2 ? LoadCost<uint16_t>(a.cost, gidx) : LoadCost<uint32_t>(a.cost, gidx);
         if (initd) v = lab[(l * h + by + y) * w + bx + x];
         if (c != 0) {
@@ -557,7 +561,8 @@ __device__ void ProcessTile(const SearchArgs& a, uint32_t r, uint32_t g, uint32_
           // border is queued only if this edge beats its halo label (the
           // staged ring, untouched by the sweeps; kInf when not yet written).
           const uint32_t d = v >> 3;
-          const uint64_t gidx = (static_cast<uint64_t>(l) * a.H + ns.y0 + by + y) * a.W + ns.x0 + bx + x;
+          const uint64_t gidx =
+              ns.cost_offset + (static_cast<uint64_t>(l) * a.H + ns.y0 + by + y) * a.W + ns.x0 + bx + x;
           auto wakes = [&](int64_t off, uint32_t halo, uint32_t code) {
             const uint64_t cell = static_cast<uint64_t>(static_cast<int64_t>(gidx) + off);
             const uint32_t c = cost_bytes == 2 ? LoadCost<uint16_t>(a.cost, cell) : LoadCost<uint32_t>(a.cost, cell);
@@ -743,7 +748,7 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
     const uint32_t w = ns.w;
     // Uniform bases; per-cell offsets fit in 32 bits (cells < 2^32).
     uint32_t* lbase = a.labels + ns.label_offset + (static_cast<uint64_t>(l) * ns.h + by) * w + bx;
-    const uint64_t cbase_index = (static_cast<uint64_t>(l) * a.H + ns.y0 + by) * a.W + ns.x0 + bx;
+    const uint64_t cbase_index = ns.cost_offset + (static_cast<uint64_t>(l) * a.H + ns.y0 + by) * a.W + ns.x0 + bx;
     const uint16_t* cbase16 = static_cast<const uint16_t*>(a.cost) + cbase_index;
     const uint32_t* cbase32 = static_cast<const uint32_t*>(a.cost) + cbase_index;
     const bool initd = a.tile_init[g] != kNever;
@@ -1415,7 +1420,7 @@ __global__ __launch_bounds__(256) void SaturationCheckKernel(SearchArgs a, uint3
   const NetStatic ns = a.nets[list[blockIdx.y]];
   const uint64_t cells = static_cast<uint64_t>(ns.w) * ns.h * a.L;
   auto cost_at = [&](uint32_t x, uint32_t y, uint32_t l) -> uint32_t {
-    const uint64_t gi = (static_cast<uint64_t>(l) * a.H + ns.y0 + y) * a.W + ns.x0 + x;
+    const uint64_t gi = ns.cost_offset + (static_cast<uint64_t>(l) * a.H + ns.y0 + y) * a.W + ns.x0 + x;
     return cost_bytes == 2 ? LoadCost<uint16_t>(a.cost, gi) : LoadCost<uint32_t>(a.cost, gi);
   };
   uint32_t found = 0;
@@ -1884,6 +1889,7 @@ int apgx::RouteNets(apgx_lattice* lat, const CostOverride* ov, const apgx_net_qu
     n.x0 = win.x0; n.y0 = win.y0; n.w = win.width; n.h = win.height;
     n.pin_offset = q.pin_offset; n.pin_count = q.pin_count;
     n.conn_offset = static_cast<uint32_t>(conns);
+    n.cost_offset = (ov && ov->net_offset) ? ov->net_offset[i] : 0u;
     if (!ok) {
       st[i].status = APGX_NET_INVALID;
       continue;

@@ -56,6 +56,10 @@ struct CostOverride {
   uint32_t max_cost;
   // 1: via moves also pay the entered cell's cost (via + plane[v]).
   uint32_t via_pays_cell;
+  // Optional, one per net of the search: net i reads the plane at
+  // plane + net_offset[i] (whole-lattice planes laid end to end), so each net
+  // can search its own snapshot; nullptr: every net reads `plane`.
+  const uint64_t* net_offset = nullptr;
 };
 
 // explore: every net searches until its queue is empty (no pin is ever

@@ -40,6 +40,11 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr uint32_t kGrid = 2048;  // grid-stride kernels: 8 waves' worth per CU
+// Searches of at most this many nets use a bucket of kLatencyDeltaCells mean
+// cell costs (16 times the lattice search's default of 48) unless the caller
+// sets frontier_delta.
+constexpr uint32_t kLatencyBatchNets = 8;
+constexpr double kLatencyDeltaCells = 768.0;
 
 // Parameters of the captured batch-prep graph, in device memory.
 struct BatchParams {
@@ -140,6 +145,42 @@ __global__ __launch_bounds__(kBlock) void HistoryKernel(const uint8_t* cap, cons
   if (__any(bound) && (threadIdx.x & 63) == 0) atomicOr(&flags->bound, 1u);
 }
 
+// Snapshot of one net's window (every layer) into that net's own plane, at the
+// lattice cell index: free cells get base + history + pf * max(0, usage + 1 -
+// capacity), blocked cells 0.  The bound checks of the snapshot instant run on
+// the host copy of the usage plane; here the cost is only clamped below the
+// search's saturated distance (values that large fail those checks, and the
+// run with them).  *max_cost receives the window's largest cost.
+template <typename CostT>
+__global__ __launch_bounds__(kBlock) void WindowSnapshotKernel(const CostT* base, const uint8_t* cap,
+                                                               const uint32_t* history, const uint32_t* usage,
+                                                               uint32_t* dst, uint32_t W, uint32_t H,
+                                                               uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                                                               uint32_t L, uint64_t pf, uint32_t* max_cost) {
+  const uint64_t plane = static_cast<uint64_t>(w) * h, n = plane * L;
+  uint32_t local_max = 0;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<uint64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t l = i / plane, r = i % plane;
+    const uint64_t c = (l * H + y0 + r / w) * W + x0 + r % w;
+    const uint32_t b = base[c];
+    uint32_t out = 0;
+    if (b != 0) {
+      const uint64_t capacity = cap ? cap[c] : 1u;
+      const uint64_t predicted = static_cast<uint64_t>(usage[c]) + 1;
+      const uint64_t over = predicted > capacity ? predicted - capacity : 0;
+      uint64_t present = 0, cost = 0;
+      const bool ovf = __builtin_mul_overflow(pf, over, &present) ||
+                       __builtin_add_overflow(present, static_cast<uint64_t>(history[c]) + b, &cost);
+      out = (ovf || cost > 0x1FFFFFFFull) ? 0x1FFFFFFFu : static_cast<uint32_t>(cost);
+    }
+    dst[c] = out;
+    local_max = max(local_max, out);
+  }
+  local_max = WaveMax(local_max);
+  if ((threadIdx.x & 63) == 0) atomicMax(max_cost, local_max);
+}
+
 struct DeviceBuffers {
   uint8_t* cap = nullptr;
   uint32_t* history = nullptr;
@@ -149,10 +190,14 @@ struct DeviceBuffers {
   uint64_t list_capacity = 0;
   BatchParams* params = nullptr;
   Flags* flags = nullptr;
+  uint32_t* work = nullptr;      // speculative groups: usage with the group's rip-ups applied
+  uint32_t* slots = nullptr;     // speculative groups: one whole-lattice snapshot plane per member
+  uint32_t* slot_max = nullptr;  // largest cost of each member's window snapshot
   ~DeviceBuffers() {
     for (void* p : {static_cast<void*>(cap), static_cast<void*>(history), static_cast<void*>(usage),
                     static_cast<void*>(snap), static_cast<void*>(list), static_cast<void*>(params),
-                    static_cast<void*>(flags)})
+                    static_cast<void*>(flags), static_cast<void*>(work), static_cast<void*>(slots),
+                    static_cast<void*>(slot_max)})
       if (p) (void)hipFree(p);
   }
 };
@@ -199,6 +244,8 @@ extern "C" void apgx_negotiation_policy_defaults(apgx_negotiation_policy* p) {
   p->maximum_congestion_cost_value = 10000000;
   p->on_pass = nullptr;
   p->on_pass_context = nullptr;
+  p->group_nets = 0;
+  p->reserved = 0;
 }
 
 extern "C" int apgx_negotiate(apgx_lattice* lat, const uint8_t* capacity, const apgx_net_query* nets,
@@ -312,7 +359,7 @@ extern "C" int apgx_negotiate(apgx_lattice* lat, const uint8_t* capacity, const 
   std::vector<std::vector<uint32_t>> batches, all_nets(1);
   for (uint32_t n = 0; n < net_count; ++n) all_nets[0].push_back(n);
   if (pol.batch_nets == 0) {
-    batches = WindowLevels(nets, net_count, d);
+    if (pol.group_nets == 0) batches = WindowLevels(nets, net_count, d);
   } else {
     for (uint32_t b0 = 0; b0 < net_count; b0 += pol.batch_nets) {
       std::vector<uint32_t> v;
@@ -325,6 +372,258 @@ extern "C" int apgx_negotiate(apgx_lattice* lat, const uint8_t* capacity, const 
   std::vector<apgx_net_result> results;
   std::vector<apgx_connection> conns;
   std::vector<uint32_t> path_cells;
+  // Searches `members` on `ov` and reads the results back into results /
+  // conns / path_cells (result i belongs to members[i]).
+  auto search = [&](const std::vector<uint32_t>& members, const CostOverride& ov) -> int {
+    const uint32_t count = static_cast<uint32_t>(members.size());
+    batch_nets.clear();
+    for (uint32_t n : members) batch_nets.push_back(nets[n]);
+    // Bucket width: a batch of a few nets leaves the GPU nearly idle, so its
+    // search time is rounds times the latency of one round; wide buckets cut
+    // the rounds (tile visits they waste are nearly free there).  A large
+    // batch keeps the lattice search's default (visits cost throughput).
+    // six_layer_1536, an exact-order pass of 1-2 nets per search: 17.8 s at
+    // the default width, 9.6 s at 16 times it; the 2000-net first pass:
+    // 0.33 s at the default, 0.75 s at 16 times it.
+    apgx_route_options opt = sopt;
+    if (opt.frontier_delta == 0 && count <= kLatencyBatchNets)
+      opt.frontier_delta = static_cast<uint32_t>(std::max(1.0, kLatencyDeltaCells * lat->mean_cost));
+    apgx_search* s = nullptr;
+    if (int rc = RouteNets(lat, &ov, batch_nets.data(), count, pins, pin_count, &opt, &s); rc != APGX_OK)
+      return rc;
+    uint64_t nconn = 0, ncell = 0;
+    int rc = apgx_search_sizes(s, &nconn, &ncell);
+    results.resize(count);
+    conns.resize(std::max<uint64_t>(nconn, 1));
+    path_cells.resize(std::max<uint64_t>(ncell, 1));
+    if (rc == APGX_OK) rc = apgx_search_readback(s, results.data(), conns.data(), nconn, path_cells.data(), ncell);
+    apgx_route_stats st{};
+    if (rc == APGX_OK) rc = apgx_search_stats(s, &st);
+    apgx_search_release(s);
+    if (rc != APGX_OK) return rc;
+    sum.search_device_ms += st.device_ms;
+    sum.tile_visits += st.tile_visits;
+    sum.search_rounds += st.rounds;
+    ++sum.batches;
+    for (const apgx_net_result& r : results)
+      if (r.status == APGX_NET_BUDGET_EXHAUSTED)
+        return Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_negotiate: a search exhausted its round budget");
+    return APGX_OK;
+  };
+  // The route a net installs after its search (result i): its new tree when
+  // reached (pin 0, then each connection path from its pin up to, not
+  // including, the tree cell it joins), else its incumbent.
+  auto installed_route = [&](uint32_t i, uint32_t n, std::vector<uint32_t>* out) {
+    const apgx_net_result& r = results[i];
+    if (r.status != APGX_NET_REACHED) {
+      *out = neg->routes[n];
+      return false;
+    }
+    out->clear();
+    out->push_back(pins[nets[n].pin_offset]);
+    for (uint32_t k = 0; k < r.connection_count; ++k) {
+      const apgx_connection& c = conns[r.connection_offset + k];
+      for (uint32_t j = 0; j + 1 < c.cell_count; ++j) out->push_back(path_cells[c.cell_offset + j]);
+    }
+    return true;
+  };
+  auto launch_scatter = [&](uint32_t* plane, const uint32_t* list, uint64_t count, int delta) -> int {
+    if (count == 0) return APGX_OK;
+    hipLaunchKernelGGL(ScatterKernel, dim3(grid), dim3(kBlock), 0, stream, plane,
+                       static_cast<const BatchParams*>(nullptr), list, count, delta);
+    APGX_HIP(hipGetLastError(), "scatter launch");
+    return APGX_OK;
+  };
+  auto launch_window_snapshot = [&](uint32_t n, const uint32_t* usage, uint32_t* dst, uint32_t* max_cost,
+                                    uint64_t pf) -> int {
+    const apgx_window& q = nets[n].window;
+    const apgx_window win = (q.width == 0 && q.height == 0) ? apgx_window{0, 0, d.width, d.height} : q;
+    if (win.width == 0 || win.height == 0 || win.x0 + static_cast<uint64_t>(win.width) > d.width ||
+        win.y0 + static_cast<uint64_t>(win.height) > d.height)
+      return APGX_OK;  // an invalid window: the search reports the net invalid without reading costs
+    const uint64_t n_cells = static_cast<uint64_t>(win.width) * win.height * d.layers;
+    const uint32_t blocks = static_cast<uint32_t>(std::min<uint64_t>(grid, (n_cells + kBlock - 1) / kBlock));
+    if (d.cost_bytes == 2)
+      hipLaunchKernelGGL(WindowSnapshotKernel<uint16_t>, dim3(blocks), dim3(kBlock), 0, stream,
+                         static_cast<const uint16_t*>(lat->d_cost), static_cast<const uint8_t*>(b.cap),
+                         static_cast<const uint32_t*>(b.history), usage, dst, d.width, d.height, win.x0, win.y0,
+                         win.width, win.height, d.layers, pf, max_cost);
+    else
+      hipLaunchKernelGGL(WindowSnapshotKernel<uint32_t>, dim3(blocks), dim3(kBlock), 0, stream,
+                         static_cast<const uint32_t*>(lat->d_cost), static_cast<const uint8_t*>(b.cap),
+                         static_cast<const uint32_t*>(b.history), usage, dst, d.width, d.height, win.x0, win.y0,
+                         win.width, win.height, d.layers, pf, max_cost);
+    APGX_HIP(hipGetLastError(), "window snapshot launch");
+    return APGX_OK;
+  };
+
+  // Speculative groups (batch_nets = 0, group_nets > 0): device planes for the
+  // group's working usage and one snapshot plane per member; host copies of the
+  // usage and history planes for the per-net bound checks; per-cell stamps of
+  // the commit that last installed the cell.
+  const uint32_t group = pol.batch_nets == 0 ? std::min(pol.group_nets, std::max(net_count, 1u)) : 0u;
+  std::vector<uint32_t> host_usage, host_history, stamp;
+  uint32_t seq = 0;
+  if (group > 0) {
+    APGX_HIP(hipMalloc(&b.work, 4 * cells), "hipMalloc");
+    APGX_HIP(hipMalloc(&b.slots, 4 * cells * group), "hipMalloc");
+    APGX_HIP(hipMalloc(&b.slot_max, 4 * group), "hipMalloc");
+    host_usage.resize(cells);
+    host_history.resize(cells);
+    stamp.assign(cells, 0);
+  }
+  // The bound checks of the snapshot instant of a net, on the host planes:
+  // congestion and cell cost of `cell` at usage host_usage[cell].
+  auto cell_within_bounds = [&](uint64_t cell, uint64_t pf) -> bool {
+    const uint64_t bcost = lat->HostCost(cell);
+    if (bcost == 0) return true;
+    const uint64_t capv = capacity ? capacity[cell] : 1u;
+    const uint64_t predicted = static_cast<uint64_t>(host_usage[cell]) + 1;
+    const uint64_t over = predicted > capv ? predicted - capv : 0;
+    uint64_t present = 0, cong = 0;
+    if (__builtin_mul_overflow(pf, over, &present) ||
+        __builtin_add_overflow(present, static_cast<uint64_t>(host_history[cell]), &cong))
+      return false;
+    return cong <= max_cong && bcost + cong <= max_cell;
+  };
+  // One pass of the exact order in speculative groups.
+  auto group_pass = [&](uint64_t pf) -> int {
+    APGX_HIP(hipMemcpy(host_usage.data(), b.usage, 4 * cells, hipMemcpyDeviceToHost), "usage readback");
+    APGX_HIP(hipMemcpy(host_history.data(), b.history, 4 * cells, hipMemcpyDeviceToHost), "history readback");
+    // The first net's snapshot instant is checked over the whole plane on the
+    // device (its rip-up, the plane-wide snapshot and its bound flag, then the
+    // rip-up undone); every later instant differs from the one before only at
+    // the cells the previous net installed.
+    {
+      std::vector<uint32_t> first = neg->routes[0];
+      if (int rc = ensure_list(std::max<uint64_t>(first.size(), 1)); rc != APGX_OK) return rc;
+      if (!first.empty())
+        APGX_HIP(hipMemcpyAsync(b.list, first.data(), 4 * first.size(), hipMemcpyHostToDevice, stream),
+                 "rip-up upload");
+      const BatchParams params{b.list, first.size(), pf};
+      APGX_HIP(hipMemcpyAsync(b.params, &params, sizeof(params), hipMemcpyHostToDevice, stream), "params upload");
+      host_flags = Flags{0, 0, 0};
+      APGX_HIP(hipMemcpyAsync(b.flags, &host_flags, sizeof(Flags), hipMemcpyHostToDevice, stream), "flags clear");
+      APGX_HIP(hipGraphLaunch(guard.g, stream), "hipGraphLaunch");
+      if (int rc = launch_scatter(b.usage, b.list, first.size(), 1); rc != APGX_OK) return rc;
+      APGX_HIP(hipMemcpyAsync(&host_flags, b.flags, sizeof(Flags), hipMemcpyDeviceToHost, stream), "flags readback");
+      APGX_HIP(hipStreamSynchronize(stream), "hipStreamSynchronize");
+      if (host_flags.bound)
+        return Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_negotiate: congestion cost exceeds the configured value bound");
+    }
+    std::vector<uint32_t> prev_installed, tree, upload;
+    std::vector<uint32_t> pending_rip, pending_install;
+    std::vector<uint64_t> offsets(group);
+    for (uint32_t k = 0; k < group; ++k) offsets[k] = static_cast<uint64_t>(k) * cells;
+    // Applies the pending commits to the committed usage plane.
+    auto flush = [&]() -> int {
+      if (pending_rip.empty() && pending_install.empty()) return APGX_OK;
+      APGX_HIP(hipStreamSynchronize(stream), "hipStreamSynchronize");  // the list buffer is free again
+      upload = pending_rip;
+      upload.insert(upload.end(), pending_install.begin(), pending_install.end());
+      if (int rc = ensure_list(upload.size()); rc != APGX_OK) return rc;
+      APGX_HIP(hipMemcpyAsync(b.list, upload.data(), 4 * upload.size(), hipMemcpyHostToDevice, stream),
+               "commit upload");
+      if (int rc = launch_scatter(b.usage, b.list, pending_rip.size(), -1); rc != APGX_OK) return rc;
+      if (int rc = launch_scatter(b.usage, b.list + pending_rip.size(), pending_install.size(), 1); rc != APGX_OK)
+        return rc;
+      APGX_HIP(hipStreamSynchronize(stream), "hipStreamSynchronize");
+      pending_rip.clear();
+      pending_install.clear();
+      return APGX_OK;
+    };
+    std::vector<uint32_t> members;
+    std::vector<uint32_t> slot_max(group);
+    std::vector<std::vector<uint32_t>> spec_tree(group);
+    std::vector<uint8_t> spec_reached(group);
+    std::vector<uint64_t> rip_offset(group + 1);
+    for (uint32_t g0 = 0; g0 < net_count; g0 += group) {
+      const uint32_t g1 = static_cast<uint32_t>(std::min<uint64_t>(net_count, uint64_t{g0} + group));
+      members.clear();
+      for (uint32_t n = g0; n < g1; ++n) members.push_back(n);
+      // Snapshots: the working plane starts as the committed usage; member k's
+      // snapshot follows the rip-ups of members 0..k.
+      upload.clear();
+      for (uint32_t k = 0; k < members.size(); ++k) {
+        rip_offset[k] = upload.size();
+        upload.insert(upload.end(), neg->routes[members[k]].begin(), neg->routes[members[k]].end());
+      }
+      rip_offset[members.size()] = upload.size();
+      if (int rc = ensure_list(std::max<uint64_t>(upload.size(), 1)); rc != APGX_OK) return rc;
+      APGX_HIP(hipMemcpyAsync(b.work, b.usage, 4 * cells, hipMemcpyDeviceToDevice, stream), "usage copy");
+      if (!upload.empty())
+        APGX_HIP(hipMemcpyAsync(b.list, upload.data(), 4 * upload.size(), hipMemcpyHostToDevice, stream),
+                 "rip-up upload");
+      APGX_HIP(hipMemsetAsync(b.slot_max, 0, 4 * group, stream), "slot max clear");
+      for (uint32_t k = 0; k < members.size(); ++k) {
+        if (int rc = launch_scatter(b.work, b.list + rip_offset[k], rip_offset[k + 1] - rip_offset[k], -1);
+            rc != APGX_OK)
+          return rc;
+        if (int rc = launch_window_snapshot(members[k], b.work, b.slots + offsets[k], b.slot_max + k, pf);
+            rc != APGX_OK)
+          return rc;
+      }
+      APGX_HIP(hipMemcpyAsync(slot_max.data(), b.slot_max, 4 * group, hipMemcpyDeviceToHost, stream),
+               "slot max readback");
+      APGX_HIP(hipStreamSynchronize(stream), "hipStreamSynchronize");
+      uint32_t max_cost = lat->min_cell_cost;
+      for (uint32_t k = 0; k < members.size(); ++k) max_cost = std::max(max_cost, slot_max[k]);
+      CostOverride ov{b.slots, max_cost, 1u};
+      ov.net_offset = offsets.data();
+      if (int rc = search(members, ov); rc != APGX_OK) return rc;
+      sum.group_routed_nets += members.size();
+      for (uint32_t k = 0; k < members.size(); ++k)
+        spec_reached[k] = installed_route(k, members[k], &spec_tree[k]) ? 1u : 0u;
+      const uint32_t start_seq = seq;
+      // Commit in order.
+      for (uint32_t k = 0; k < members.size(); ++k) {
+        const uint32_t n = members[k];
+        // Snapshot instant of n: its rip-up, then the cells the previous net
+        // installed are the only ones whose cost can have risen since the
+        // previous instant.
+        for (uint32_t c : neg->routes[n]) --host_usage[c];
+        if (n > 0)
+          for (uint32_t c : prev_installed)
+            if (!cell_within_bounds(c, pf))
+              return Fail(APGX_E_RESOURCE_EXHAUSTED,
+                          "apgx_negotiate: congestion cost exceeds the configured value bound");
+        // A net that is not reached keeps its incumbent: reachability does not
+        // depend on the costs, so that holds on the exact snapshot too.
+        bool reached = spec_reached[k] != 0;
+        bool conflict = false;
+        if (reached)
+          for (uint32_t c : spec_tree[k]) conflict |= stamp[c] > start_seq;
+        tree.swap(spec_tree[k]);
+        pending_rip.insert(pending_rip.end(), neg->routes[n].begin(), neg->routes[n].end());
+        if (conflict) {
+          // Searched again on the exact snapshot: every earlier net committed.
+          if (int rc = flush(); rc != APGX_OK) return rc;
+          uint32_t redo_max = 0;
+          APGX_HIP(hipMemsetAsync(b.slot_max, 0, 4, stream), "slot max clear");
+          if (int rc = launch_window_snapshot(n, b.usage, b.slots, b.slot_max, pf); rc != APGX_OK) return rc;
+          APGX_HIP(hipMemcpyAsync(&redo_max, b.slot_max, 4, hipMemcpyDeviceToHost, stream), "slot max readback");
+          APGX_HIP(hipStreamSynchronize(stream), "hipStreamSynchronize");
+          const std::vector<uint32_t> one{n};
+          const CostOverride exact{b.slots, std::max(redo_max, lat->min_cell_cost), 1u};
+          if (int rc = search(one, exact); rc != APGX_OK) return rc;
+          ++sum.redone_nets;
+          reached = installed_route(0, n, &tree);
+        }
+        if (reached) neg->routed[n] = 1;
+        neg->routes[n] = tree;
+        ++seq;
+        for (uint32_t c : tree) {
+          stamp[c] = seq;
+          ++host_usage[c];
+        }
+        pending_install.insert(pending_install.end(), tree.begin(), tree.end());
+        prev_installed = tree;
+      }
+      if (int rc = flush(); rc != APGX_OK) return rc;
+    }
+    return APGX_OK;
+  };
+
   for (uint32_t pass = 0; pass < pol.maximum_passes; ++pass) {
     uint64_t pf = 0;
     if (__builtin_mul_overflow(static_cast<uint64_t>(pass), pol.present_cost_increment, &pf) ||
@@ -333,7 +632,11 @@ extern "C" int apgx_negotiate(apgx_lattice* lat, const uint8_t* capacity, const 
     // A zero present factor prices no cell by its usage: the snapshot is base +
     // history for every batch of the pass, no route depends on another, and
     // one batch of every net gives the routes of any order.
-    const std::vector<std::vector<uint32_t>>& pass_batches = pf == 0 ? all_nets : batches;
+    if (pf != 0 && group > 0 && net_count > 0) {
+      if (int rc = group_pass(pf); rc != APGX_OK) return rc;
+    }
+    static const std::vector<std::vector<uint32_t>> kNoBatches;
+    const std::vector<std::vector<uint32_t>>& pass_batches = pf == 0 ? all_nets : (group > 0 ? kNoBatches : batches);
     for (const std::vector<uint32_t>& members : pass_batches) {
       const uint32_t count = static_cast<uint32_t>(members.size());
       rip_list.clear();
@@ -353,41 +656,16 @@ extern "C" int apgx_negotiate(apgx_lattice* lat, const uint8_t* capacity, const 
         return Fail(APGX_E_RESOURCE_EXHAUSTED,
                     "apgx_negotiate: congestion cost exceeds the configured value bound");
       // Route the batch on the snapshot plane.
-      batch_nets.clear();
-      for (uint32_t n : members) batch_nets.push_back(nets[n]);
       const CostOverride ov{b.snap, std::max(host_flags.max_cost, lat->min_cell_cost), 1u};
-      apgx_search* s = nullptr;
-      if (int rc = RouteNets(lat, &ov, batch_nets.data(), count, pins, pin_count, &sopt, &s); rc != APGX_OK)
-        return rc;
-      uint64_t nconn = 0, ncell = 0;
-      int rc = apgx_search_sizes(s, &nconn, &ncell);
-      results.resize(count);
-      conns.resize(std::max<uint64_t>(nconn, 1));
-      path_cells.resize(std::max<uint64_t>(ncell, 1));
-      if (rc == APGX_OK) rc = apgx_search_readback(s, results.data(), conns.data(), nconn, path_cells.data(), ncell);
-      apgx_route_stats st{};
-      if (rc == APGX_OK) rc = apgx_search_stats(s, &st);
-      apgx_search_release(s);
-      if (rc != APGX_OK) return rc;
-      sum.search_device_ms += st.device_ms;
-      sum.tile_visits += st.tile_visits;
-      ++sum.batches;
+      if (int rc = search(members, ov); rc != APGX_OK) return rc;
       // Install new routes (a disconnected or invalid net keeps its incumbent).
       install_list.clear();
+      std::vector<uint32_t> tree;
       for (uint32_t i = 0; i < count; ++i) {
         const uint32_t n = members[i];
-        const apgx_net_result& r = results[i];
-        if (r.status == APGX_NET_REACHED) {
-          std::vector<uint32_t>& t = neg->routes[n];
-          t.clear();
-          t.push_back(pins[nets[n].pin_offset]);
-          for (uint32_t k = 0; k < r.connection_count; ++k) {
-            const apgx_connection& c = conns[r.connection_offset + k];
-            for (uint32_t j = 0; j + 1 < c.cell_count; ++j) t.push_back(path_cells[c.cell_offset + j]);
-          }
+        if (installed_route(i, n, &tree)) {
+          neg->routes[n] = tree;
           neg->routed[n] = 1;
-        } else if (r.status == APGX_NET_BUDGET_EXHAUSTED) {
-          return Fail(APGX_E_RESOURCE_EXHAUSTED, "apgx_negotiate: a search exhausted its round budget");
         }
         install_list.insert(install_list.end(), neg->routes[n].begin(), neg->routes[n].end());
       }
@@ -395,10 +673,7 @@ extern "C" int apgx_negotiate(apgx_lattice* lat, const uint8_t* capacity, const 
         if (int e = ensure_list(install_list.size()); e != APGX_OK) return e;
         APGX_HIP(hipMemcpyAsync(b.list, install_list.data(), 4 * install_list.size(), hipMemcpyHostToDevice,
                                 stream), "install upload");
-        hipLaunchKernelGGL(ScatterKernel, dim3(grid), dim3(kBlock), 0, stream, b.usage,
-                           static_cast<const BatchParams*>(nullptr), static_cast<const uint32_t*>(b.list),
-                           static_cast<uint64_t>(install_list.size()), 1);
-        APGX_HIP(hipGetLastError(), "install launch");
+        if (int e = launch_scatter(b.usage, b.list, install_list.size(), 1); e != APGX_OK) return e;
       }
     }
     // Accounting at the pass boundary.

@@ -353,7 +353,21 @@ int apgx_autotune(apgx_lattice* lattice, const apgx_net_query* nets, uint32_t ne
  * nets routed one after another, with less latency.  All arithmetic is integer
  * and runs on the GPU.  A pass whose present factor is zero prices no cell by
  * its usage, so its snapshot costs do not depend on any route: its nets are
- * routed as one batch in every mode, with the same routes as in any order. */
+ * routed as one batch in every mode, with the same routes as in any order.
+ *
+ * Speculative groups (batch_nets = 0 and group_nets > 0): the
+ * exact order is kept, but group_nets consecutive nets are searched in one
+ * launch, each on its own snapshot of its window taken after the rip-ups of
+ * the group's earlier nets and before their installs.  Nets then commit in
+ * order.  Between that snapshot and the exact one (every earlier net
+ * committed) costs can only rise, and only at cells an earlier group member
+ * installed; a search result does not change when costs rise at cells off
+ * its tree, so a net whose tree avoids every such cell commits its result,
+ * and a net whose tree meets one is searched again on the exact snapshot.
+ * Routes, planes, passes and termination equal the one-net-at-a-time order,
+ * and so do the bound checks: they are evaluated per net, at each net's own
+ * rip-up, over the whole plane (first net of a pass) and then over the cells
+ * the previous net installed, the only cells whose cost can have risen. */
 #define APGX_NEGOTIATION_FEASIBLE 0u
 #define APGX_NEGOTIATION_STALLED 1u
 #define APGX_NEGOTIATION_PASS_LIMIT 2u
@@ -371,6 +385,9 @@ typedef struct apgx_negotiation_policy {
   uint64_t maximum_congestion_cost_value;  /* default 10,000,000 */
   apgx_negotiation_pass_fn on_pass;        /* optional, called on the calling thread (default NULL) */
   void* on_pass_context;
+  uint32_t group_nets;                     /* batch_nets = 0: nets per speculative group; 0 (default):
+                                              window-disjoint levels */
+  uint32_t reserved;
 } apgx_negotiation_policy;
 
 typedef struct apgx_negotiation_summary {
@@ -382,6 +399,9 @@ typedef struct apgx_negotiation_summary {
   uint64_t tile_visits;                    /* summed over every search */
   double search_device_ms;                 /* summed search envelopes */
   double wall_ms;                          /* host wall time of apgx_negotiate */
+  uint64_t search_rounds;                  /* search rounds summed over every batch */
+  uint64_t group_routed_nets;              /* net searches run in speculative groups */
+  uint64_t redone_nets;                    /* of those, searched again on the exact snapshot */
 } apgx_negotiation_summary;
 
 typedef struct apgx_negotiation apgx_negotiation;

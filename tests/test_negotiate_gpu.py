@@ -38,14 +38,16 @@ def assert_same(gpu, orc):
             assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("seed,batch", [(0, 1), (1, 4), (2, 16), (3, 30), (4, 0), (5, 0)])
-def test_small_boards_match_oracle(apgx, oracle, gpu, seed, batch):
-    """batch 0 (window-disjoint levels) must equal the oracle's one-net-at-a-time order."""
-    _say(f"[case] allocator parity seed={seed} batch={batch}")
+@pytest.mark.parametrize("seed,batch,group", [(0, 1, 0), (1, 4, 0), (2, 16, 0), (3, 30, 0), (4, 0, 0), (5, 0, 0),
+                                              (4, 0, 32), (5, 0, 32)])
+def test_small_boards_match_oracle(apgx, oracle, gpu, seed, batch, group):
+    """batch 0 must equal the oracle's one-net-at-a-time order, both as
+    window-disjoint levels (group 0) and as speculative groups."""
+    _say(f"[case] allocator parity seed={seed} batch={batch} group={group}")
     cost, nets, pins = make_board(apgx, SIX_LAYER_1536, 1000 + seed, **SMALL)
     via = SIX_LAYER_1536["via"]
     lat = apgx.Lattice(cost, via_cost=via, cost_bytes=2)
-    g = lat.negotiate(nets, pins, maximum_passes=12, batch_nets=batch)
+    g = lat.negotiate(nets, pins, maximum_passes=12, batch_nets=batch, group_nets=group)
     o = oracle.negotiate(cost, via, oracle_nets(nets), pins, maximum_passes=12, batch_nets=max(batch, 1))
     if batch == 0:
         assert g["batches"] < 12 * len(nets)  # levels group nets
@@ -73,6 +75,39 @@ def test_high_present_costs_match_oracle(apgx, oracle, gpu, seed):
     lat.close()
 
 
+REDONE = []
+
+
+@pytest.mark.parametrize("seed,group", [(10, 1), (11, 4), (12, 8), (13, 30), (14, 16)])
+def test_speculative_groups_match_oracle(apgx, oracle, gpu, seed, group):
+    """Speculative groups of 1 to 30 nets (a group searches its nets at once,
+    each on its own snapshot, commits them in order and searches a net again
+    on the exact snapshot when its tree meets a cell an earlier member
+    installed): every pass bit-exact with the oracle's one-net order."""
+    _say(f"[case] allocator speculative groups seed={seed} group={group}")
+    cost, nets, pins = make_board(apgx, SIX_LAYER_1536, 1000 + seed, **SMALL)
+    via = SIX_LAYER_1536["via"]
+    lat = apgx.Lattice(cost, via_cost=via, cost_bytes=2)
+    g = lat.negotiate(nets, pins, maximum_passes=12, batch_nets=0, group_nets=group)
+    o = oracle.negotiate(cost, via, oracle_nets(nets), pins, maximum_passes=12, batch_nets=1)
+    assert o["status"] == 0
+    assert_same(g, o)
+    # Every pass after the first (zero present factor: one batch) runs in groups.
+    assert g["group_routed_nets"] == (g["passes"] - 1) * len(nets)
+    if group == 1:
+        assert g["redone_nets"] == 0  # a group of one is never speculative
+    REDONE.append(g["redone_nets"])
+    _say(f"[info] passes={g['passes']} redone={g['redone_nets']} of {g['group_routed_nets']}")
+    lat.close()
+
+
+def test_speculation_was_exercised(apgx, gpu):
+    """The parametrised cases above must have searched some nets again."""
+    if len(REDONE) < 5:
+        pytest.skip("runs after the speculative group cases")
+    assert sum(REDONE) > 0
+
+
 def test_seeded_history_and_capacity_match_oracle(apgx, oracle, gpu):
     _say("[case] allocator parity with seeded history and capacity plane")
     cost, nets, pins = make_board(apgx, SIX_LAYER_1536, 1010, **SMALL)
@@ -86,6 +121,23 @@ def test_seeded_history_and_capacity_match_oracle(apgx, oracle, gpu):
     o = oracle.negotiate(cost, via, oracle_nets(nets), pins, capacity=cap, initial_history=hist0,
                          maximum_passes=10, batch_nets=8, present_cost_increment=50,
                          historical_cost_increment=20)
+    assert_same(g, o)
+    lat.close()
+
+
+def test_seeded_history_and_capacity_groups_match_oracle(apgx, oracle, gpu):
+    _say("[case] allocator speculative groups with seeded history and capacity plane")
+    cost, nets, pins = make_board(apgx, SIX_LAYER_1536, 1011, **SMALL)
+    rng = np.random.default_rng(5)
+    hist0 = (rng.integers(0, 6, size=cost.shape) * 10).astype(np.uint32)
+    cap = np.where(rng.random(cost.shape) < 0.05, 2, 1).astype(np.uint8)
+    via = SIX_LAYER_1536["via"]
+    lat = apgx.Lattice(cost, via_cost=via, cost_bytes=2)
+    kw = dict(maximum_passes=10, present_cost_increment=50, historical_cost_increment=20)
+    g = lat.negotiate(nets, pins, capacity=cap, initial_history=hist0, batch_nets=0, group_nets=12, **kw)
+    o = oracle.negotiate(cost, via, oracle_nets(nets), pins, capacity=cap, initial_history=hist0,
+                         batch_nets=1, **kw)
+    assert o["status"] == 0
     assert_same(g, o)
     lat.close()
 
@@ -124,6 +176,49 @@ def test_value_bound_is_reported(apgx, gpu):
         lat.negotiate(nets, p, capacity=np.zeros(3, dtype=np.uint8), initial_present_cost=7,
                       maximum_congestion_cost_value=6, maximum_passes=1)
     assert e.value.code == 2
+    lat.close()
+
+
+@pytest.mark.parametrize("group", [0, 2])
+def test_value_bound_at_a_later_net(apgx, oracle, gpu, group):
+    """The first net's snapshot is within the bound; the second net's is not,
+    because the first net installed its route (congestion 0 + 7 * 1 > 6).
+    The oracle's one-net order reports the bound; so must both exact modes."""
+    cost = np.ones((1, 1, 5), dtype=np.uint32)
+    nets, p = apgx.nets_from_lists([[0, 2], [1, 3]])
+    o = oracle.negotiate(cost, 0, [(0, 2, None), (2, 2, None)], p, initial_present_cost=7,
+                         maximum_congestion_cost_value=6, maximum_passes=1, batch_nets=1)
+    assert o["status"] != 0
+    lat = apgx.Lattice(cost, via_cost=0)
+    with pytest.raises(apgx.ApgxError) as e:
+        lat.negotiate(nets, p, initial_present_cost=7, maximum_congestion_cost_value=6, maximum_passes=1,
+                      batch_nets=0, group_nets=group)
+    assert e.value.code == 2
+    # Within the bound (7 <= 7) both exact modes route.
+    g = lat.negotiate(nets, p, initial_present_cost=7, maximum_congestion_cost_value=7, maximum_passes=1,
+                      batch_nets=0, group_nets=group)
+    o = oracle.negotiate(cost, 0, [(0, 2, None), (2, 2, None)], p, initial_present_cost=7,
+                         maximum_congestion_cost_value=7, maximum_passes=1, batch_nets=1)
+    assert o["status"] == 0
+    assert_same(g, o)
+    lat.close()
+
+
+def test_groups_equal_levels_on_the_full_board(apgx, gpu):
+    """six_layer_1536 seed 200 (the full 6 x 1536^2 lattice, its first 400
+    nets), three passes: speculative groups of 32 give the same routes, planes
+    and overuse as window-disjoint levels, both exact orders."""
+    _say("[case] six_layer_1536 seed 200, 400 nets, groups vs levels")
+    spec = SIX_LAYER_1536
+    cost, nets, pins = make_board(apgx, spec, 200)
+    sub = (apgx.NetQuery * 400)(*nets[:400])
+    lat = apgx.Lattice(cost, via_cost=spec["via"], cost_bytes=2)
+    a = lat.negotiate(sub, pins, maximum_passes=3, batch_nets=0, group_nets=0)
+    b = lat.negotiate(sub, pins, maximum_passes=3, batch_nets=0, group_nets=32)
+    _say(f"[info] levels {a['wall_ms']:.0f} ms, groups {b['wall_ms']:.0f} ms, redone {b['redone_nets']} "
+         f"of {b['group_routed_nets']}, overuse {b['overuse_per_pass']}")
+    assert_same(b, a)
+    assert b["redone_nets"] < b["group_routed_nets"]
     lat.close()
 
 
