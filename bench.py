@@ -54,7 +54,7 @@ def parse_args():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=1)
-    p.add_argument("--workload", choices=["four_layer_1024", "eight_layer_2048", "learned_prior"],
+    p.add_argument("--workload", choices=["four_layer_1024", "eight_layer_2048", "learned_prior", "allocator"],
                    default="four_layer_1024")
     p.add_argument("--queries", type=int, default=0, help="queries / nets (0: the workload's count)")
     p.add_argument("--cpu-sample", type=int, default=0, help="queries timed on the CPU oracle (0: auto)")
@@ -92,6 +92,13 @@ def parse_args():
                         "(per cell and per tile), the best any history prior could do")
     p.add_argument("--full-size-boards", type=int, default=0,
                    help="learned_prior: also compare on this many six_layer_1536 boards")
+    p.add_argument("--board-seed", type=int, default=200, help="allocator: six_layer_1536 board seed")
+    p.add_argument("--passes", type=int, default=40, help="allocator: pass limit")
+    p.add_argument("--group-nets", type=int, default=0,
+                   help="allocator: speculative group size of the exact order (0: window-disjoint levels)")
+    p.add_argument("--prior-gain", type=float, default=0.0,
+                   help="allocator: also run the board seeded by the learned prior, trained on --dataset "
+                        "(six_layer_512 plain runs), at this decode gain")
     return p.parse_args()
 
 
@@ -489,18 +496,106 @@ def search_workload(args, workload, steps, warmup, cpu):
     return line
 
 
+def allocator(args):
+    """Exact-order CS-RR-v1 on one six_layer_1536 board (2000 nets, capacity 1)
+    until zero overflow or the pass limit, with a line per pass on stderr; with
+    --prior-gain the same board again with its history seeded by the learned
+    prior (trained on the --dataset runs of six_layer_512 boards).  The JSON
+    line reports overuse and wall time per pass of each arm."""
+    from apgar_amd import prior_study as ps
+    from tests.lattice_cases import SIX_LAYER_1536, SIX_LAYER_512, make_board
+
+    cost, nets, pins = make_board(apgx, SIX_LAYER_1536, args.board_seed)
+    lat = apgx.Lattice(cost, via_cost=SIX_LAYER_1536["via"], cost_bytes=2)
+
+    def run(arm, history=None, extra_ms=0.0):
+        walls = []
+
+        def on_pass(p, overuse, routed, ms):
+            walls.append(ms)
+            log(f"[bench] allocator {arm} pass {p}: overuse {overuse} routed {routed}/{len(nets)} wall {ms:.0f} ms")
+
+        r = lat.negotiate(nets, pins, initial_history=history, maximum_passes=args.passes, batch_nets=0,
+                          group_nets=args.group_nets, on_pass=on_pass)
+        per_pass = [b - a for a, b in zip([0.0] + walls[:-1], walls)]
+        return {"termination": ["feasible", "stalled", "pass_limit"][r["termination"]], "passes": r["passes"],
+                "overuse_per_pass": r["overuse_per_pass"], "pass_wall_ms": per_pass,
+                "mean_exact_order_pass_ms": float(np.mean(per_pass[1:])) if len(per_pass) > 1 else None,
+                "wall_ms": r["wall_ms"] + extra_ms, "seed_ms": extra_ms,
+                "time_to_zero_overflow_ms": (r["wall_ms"] + extra_ms)
+                if r["termination"] == apgx.NEGOTIATION_FEASIBLE else None,
+                "search_rounds": r["search_rounds"], "tile_visits": r["tile_visits"],
+                "redone_nets": r["redone_nets"], "group_routed_nets": r["group_routed_nets"]}
+
+    plain = run("plain")
+    line = {"metric": "exact-order CS-RR-v1 pass time and overuse on six_layer_1536",
+            "value": plain["mean_exact_order_pass_ms"], "unit": "ms per exact-order pass (passes 1..)",
+            "n_gpus": 1, "higher_is_better": False, "vs_baseline": None, "dtype": "u32", "data": "synthetic",
+            "config": {"workload": "six_layer_1536 allocator board (6 x 1536^2, five 32x32-pad BGA fields, "
+                                   "2000 nets of 2-6 pins, capacity 1), exact net order",
+                       "board_seed": args.board_seed, "pass_limit": args.passes,
+                       "order": f"speculative groups of {args.group_nets}" if args.group_nets
+                       else "window-disjoint levels"},
+            "plain": plain}
+    if args.prior_gain > 0:
+        path = Path(args.dataset) if args.dataset else None
+        if path is not None and path.exists():
+            d = np.load(path, allow_pickle=False)
+            x, y = d["x"], d["y"]
+        else:
+            x, y, _ = ps.collect(SIX_LAYER_512, list(range(args.train_seeds)), log=log)
+            if path is not None:
+                path.parent.mkdir(parents=True, exist_ok=True)
+                np.savez(path, x=x, y=y)
+        n_fit = x.shape[0] - max(1, x.shape[0] // 10)
+        net, rep = ps.train(x[:n_fit], y[:n_fit], SIX_LAYER_512["layers"], epochs=args.epochs, batch=32)
+        log(f"[bench] prior trained on {n_fit} six_layer_512 boards: {rep}")
+        big = apgx.Prior(net.layers, cost.shape[1] // ps.TILE, cost.shape[2] // ps.TILE)
+        big.set_params(net.get_params())
+        t0 = time.perf_counter()
+        hist = big.seed_history(lat, pins, gain=args.prior_gain)
+        seed_ms = (time.perf_counter() - t0) * 1e3
+        line["learned"] = run("learned", history=hist, extra_ms=seed_ms)
+        line["learned"]["decode_gain"] = args.prior_gain
+        line["learned"]["training_boards"] = int(n_fit)
+        # Neither arm need reach zero overflow within the pass limit; the
+        # comparison then uses the first wall time at which each arm's overuse
+        # is at most the plain arm's final overuse.
+        target = plain["overuse_per_pass"][-1]
+
+        def time_to(arm):
+            for overuse, t in zip(arm["overuse_per_pass"], np.cumsum(arm["pass_wall_ms"])):
+                if overuse <= target:
+                    return float(t) + arm["seed_ms"]
+            return None
+
+        tp, tl = time_to(plain), time_to(line["learned"])
+        line["learned_vs_plain"] = {
+            "target_overuse": target, "plain_ms": tp, "learned_ms": tl,
+            "time_ratio": (tl / tp) if (tp and tl) else None,
+            "zero_overflow_time_ratio": (line["learned"]["time_to_zero_overflow_ms"] / plain["time_to_zero_overflow_ms"])
+            if plain["time_to_zero_overflow_ms"] and line["learned"]["time_to_zero_overflow_ms"] else None}
+        big.close()
+        net.close()
+    lat.close()
+    print(json.dumps(line))
+
+
 def main():
     args = parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
         return
-    if args.gpus != 1 and args.workload != "learned_prior":
+    if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); queries are "
               "independent and would shard by query list, which this build does not launch.",
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "learned_prior":
         learned_prior(args)
+        return
+    if args.workload == "allocator":
+        allocator(args)
         return
     line = search_workload(args, args.workload, args.steps, args.warmup, not args.no_cpu)
     if args.workload == "four_layer_1024" and not args.no_sub:

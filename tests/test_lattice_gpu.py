@@ -331,6 +331,41 @@ def test_autotune_picks_a_measured_candidate(apgx, gpu):
     lat.close()
 
 
+def test_autotune_candidates_agree_with_the_oracle(apgx, oracle, gpu):
+    """Every candidate the autotuner times (tile shapes 64x64 register / 64x32
+    and 32x32 LDS, bucket widths, sweep caps; with landmark bounds, as the
+    eight_layer_2048 benchmark runs it) routes the whole batch to the same
+    distances and canonical paths, and those equal the oracle's: the choice
+    changes speed only.  The reported choice is the fastest timing."""
+    _say("[case] autotune with landmarks on 64 nets of a 3x256x256 lattice: every candidate vs the oracle")
+    rng = np.random.default_rng(21)
+    cost = random_lattice(rng, 3, 256, 256, blocked=0.12, cmax=8)
+    via = 20
+    nets, pins = apgx.generate_nets(cost, 64, 2, 6, 32, 256, 13)
+    lat = apgx.Lattice(cost, via_cost=via)
+    lat.build_landmarks(8)
+    opts, report = lat.autotune(nets, pins, sample_nets=32, landmarks=True)
+    cands = report["candidates"]
+    assert len(cands) >= 12 and any(c[3] == 32 for c in cands)  # the landmark schedule is tried
+    best = cands[report["best"]]
+    assert all(best[4] <= c[4] for c in cands)
+    assert opts["tile"] == best[:2] and opts["max_sweeps"] == best[3]
+    expected = []
+    for n in nets:
+        win = (n.window.x0, n.window.y0, n.window.width, n.window.height)
+        expected.append(oracle.Dense(cost, via, win).route_net(pins[n.pin_offset:n.pin_offset + n.pin_count]))
+    for tw, th, delta, sweeps, _ in cands:
+        routes, _ = lat.route(nets, pins, tile=(tw, th), frontier_delta=delta, max_sweeps=sweeps, landmarks=True)
+        for i, (r, (rc, conns)) in enumerate(zip(routes, expected)):
+            if rc == 1:
+                assert r.status == apgx.NET_DISCONNECTED, (i, tw, th, delta, sweeps)
+                continue
+            assert r.status == apgx.NET_REACHED and len(r.connections) == len(conns), (i, tw, th, delta, sweeps)
+            for (gp, gd, gc), (op, od, oc) in zip(r.connections, conns):
+                assert (gp, gd) == (op, od) and np.array_equal(gc, oc), (i, tw, th, delta, sweeps)
+    lat.close()
+
+
 @pytest.mark.parametrize("schedule,max_sweeps,seed", [(1, 0, 0), (2, 0, 0), (2, 0, 1), (2, 3, 2), (2, 1, 3),
                                                       (1, 2, 4), (2, 0, 5), (3, 0, 1), (4, 5, 3), (3, 2, 0)])
 def test_register_tiles_sweep_schedules(apgx, oracle, gpu, schedule, max_sweeps, seed):
