@@ -129,6 +129,12 @@ typedef struct {
   uint64_t n, cap;
 } pq_t;
 
+/* Signed coordinates in an unsigned key: flipping the sign bit keeps int64
+ * order, as the reference's std::tie over int64 y and x does
+ * (src/routing/cpu_astar.cc:67-74). */
+static uint64_t coord_key(int64_t v) { return (uint64_t)v ^ 0x8000000000000000ull; }
+static int64_t key_coord(uint64_t k) { return (int64_t)(k ^ 0x8000000000000000ull); }
+
 static int item_less(const item_t* p, const item_t* q) {
   for (int i = 0; i < 7; ++i)
     if (p->k[i] != q->k[i]) return p->k[i] < q->k[i];
@@ -220,13 +226,13 @@ int orc_v1_astar(const orc_v1_node* nodes, uint64_t n, const orc_v1_profile* pr,
   uint64_t seq = 0, expanded = 0;
   uint64_t h0 = heuristic(pr, po, sx, sy, gx, gy);
   best[s * 9 + 8] = 0;
-  item_t it0 = {{h0, h0, 0, (uint64_t)sy, (uint64_t)sx, 8, seq++}};
+  item_t it0 = {{h0, h0, 0, coord_key(sy), coord_key(sx), 8, seq++}};
   pq_push(&q, it0);
   int64_t goal_state = -1;
   int err = 0;
   while (q.n) {
     item_t cur = pq_pop(&q);
-    int64_t cx = (int64_t)cur.k[4], cy = (int64_t)cur.k[3];
+    int64_t cx = key_coord(cur.k[4]), cy = key_coord(cur.k[3]);
     int inc = (int)cur.k[5];
     int64_t u = lookup(&lk, layer, cx, cy);
     uint64_t st = (uint64_t)u * 9 + inc;
@@ -250,7 +256,7 @@ int orc_v1_astar(const orc_v1_node* nodes, uint64_t n, const orc_v1_profile* pr,
       uint64_t hv = heuristic(pr, po, vx, vy, gx, gy);
       uint64_t f;
       if (checked_add(nc, hv, &f)) f = INF64;
-      item_t it = {{f, hv, nc, (uint64_t)vy, (uint64_t)vx, (uint64_t)d, seq++}};
+      item_t it = {{f, hv, nc, coord_key(vy), coord_key(vx), (uint64_t)d, seq++}};
       if (pq_push(&q, it)) { err = 1; break; }
     }
     if (err) break;
