@@ -92,6 +92,8 @@ def parse_args():
                         "(per cell and per tile), the best any history prior could do")
     p.add_argument("--full-size-boards", type=int, default=0,
                    help="learned_prior: also compare on this many six_layer_1536 boards")
+    p.add_argument("--prior-family", choices=["six_layer_512", "six_layer_512_dense"], default="six_layer_512",
+                   help="learned_prior: board family of the study (training, validation, held-out)")
     p.add_argument("--board-seed", type=int, default=200, help="allocator: six_layer_1536 board seed")
     p.add_argument("--passes", type=int, default=40, help="allocator: pass limit")
     p.add_argument("--group-nets", type=int, default=0,
@@ -236,14 +238,15 @@ def log(msg):
 
 
 def learned_prior(args):
-    """Learned history-cost prior on six_layer_512 boards: plain allocator runs on
+    """Learned history-cost prior on six_layer_512 (or six_layer_512_dense)
+    boards: plain allocator runs on
     training seeds 0.. give the targets, the network trains on the GPU, the
     decode gain is chosen on the last tenth of the training seeds, and held-out
     seeds 200.. compare time to zero overflow, learned seed vs zero history."""
     from apgar_amd import prior_study as ps
-    from tests.lattice_cases import SIX_LAYER_1536, SIX_LAYER_512
+    from tests.lattice_cases import SIX_LAYER_1536, SIX_LAYER_512, SIX_LAYER_512_DENSE
 
-    spec = SIX_LAYER_512
+    spec = SIX_LAYER_512_DENSE if args.prior_family == "six_layer_512_dense" else SIX_LAYER_512
     L = spec["layers"]
     if args.upper_bound:
         res = []
@@ -325,8 +328,8 @@ def learned_prior(args):
         "vs_baseline": None,
         "dtype": "bf16 (network), u32 (allocator)",
         "data": "synthetic",
-        "config": {"workload": "six_layer_512 allocator boards (one 32x32-pad BGA field, 200 nets of 2-6 "
-                               "pins, capacity 1), exact-order negotiation, 40-pass limit",
+        "config": {"workload": f"{args.prior_family} allocator boards (one 32x32-pad BGA field, {spec['nets']} "
+                               "nets of 2-6 pins, capacity 1), exact-order negotiation, 40-pass limit",
                    "train_seeds": f"0-{n_fit - 1}", "validation_seeds": f"{n_fit}-{len(seeds) - 1}",
                    "heldout_seeds": f"200-{199 + args.heldout_seeds}", "decode_gain": gain},
         "heldout": cmp,

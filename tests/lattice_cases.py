@@ -17,6 +17,8 @@ The five lattice workloads of this project (sizes, cost models, seeds):
   six_layer_512      the same generator on 6 x 512 x 512 with one field and 200
                      nets (learned-prior study boards; seeds 0-199 training,
                      200-249 held out)
+  six_layer_512_dense the same with 300 nets: about 570 used pads in the field,
+                     near six_layer_1536's 760, so runs need tens of passes
 """
 from __future__ import annotations
 
@@ -41,6 +43,10 @@ SIX_LAYER_1536 = dict(width=1536, height=1536, layers=6, cost_min=1, cost_max=8,
 # net density (200 nets per 512^2 against 2000 per 1536^2): the scale at which
 # the learned prior's 200 training and 50 held-out allocator runs fit in minutes.
 SIX_LAYER_512 = dict(SIX_LAYER_1536, width=512, height=512, nets=200, bga_fields=1, scatter_radius=192)
+
+# The same field under more nets: pad use per field closer to six_layer_1536's,
+# where the plain allocator needs tens of passes rather than three or four.
+SIX_LAYER_512_DENSE = dict(SIX_LAYER_512, nets=300)
 
 
 def make_board(apgx, spec, seed, **overrides):
