@@ -80,6 +80,8 @@ def parse_args():
     p.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                    help="only run a device-to-device copy of MIB MiB (HBM counter scale reference)")
     p.add_argument("--train-seeds", type=int, default=200, help="learned_prior: training boards")
+    p.add_argument("--train-seed-start", type=int, default=0,
+                   help="learned_prior --collect-only: first training seed collected (to collect in parts)")
     p.add_argument("--heldout-seeds", type=int, default=50, help="learned_prior: held-out boards")
     p.add_argument("--dataset", type=str, default="", help="learned_prior: .npz of collected training "
                    "data (loaded if present, else collected and written)")
@@ -270,6 +272,8 @@ def learned_prior(args):
         plain_ms = d["plain_ms"]
         log(f"[bench] loaded {x.shape[0]} training samples from {path}")
     else:
+        if args.collect_only and args.train_seed_start:
+            seeds = list(range(args.train_seed_start, args.train_seeds))
         x, y, runs = ps.collect(spec, seeds, log=log)
         plain_ms = np.array([r["wall_ms"] for r in runs])
         if path is not None:
