@@ -1151,6 +1151,12 @@ __global__ __launch_bounds__(kThreads) void SeedKernel(SearchArgs a) {
   }
 }
 
+__device__ __forceinline__ uint64_t ShflXorU64(uint64_t v, int mask) {
+  const uint32_t lo = __shfl_xor(static_cast<uint32_t>(v), mask, 64);
+  const uint32_t hi = __shfl_xor(static_cast<uint32_t>(v >> 32), mask, 64);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
 __device__ __forceinline__ uint32_t Neighbour(uint32_t cell, uint32_t code, uint32_t W,
                                               uint32_t plane) {
   switch (code) {
@@ -1210,18 +1216,20 @@ __device__ void PhaseNet(const SearchArgs& a, uint32_t r, uint32_t n) {
   }
   const NetStatic ns = a.nets[n];
   uint32_t* lab = a.labels + ns.label_offset;
-  const uint32_t plane = a.W * a.H;
-  uint32_t best = kInfD, best_pin = kNever;
-  for (uint32_t p = 0; p < ns.pin_count; ++p) {
-    if (st.connected & (1u << p)) continue;
-    const uint32_t cell = a.pins[ns.pin_offset + p];
+  // The nearest unconnected pin (ties: the lowest index): lane p reads pin p's
+  // label (pins <= 32), then a wave minimum over (distance, pin) picks it, so
+  // the pins' dependent loads overlap instead of running one pin after another.
+  const uint32_t lane = threadIdx.x & 63u;
+  uint64_t key = ~0ull;
+  if (lane < ns.pin_count && !(st.connected & (1u << lane))) {
+    const uint32_t cell = a.pins[ns.pin_offset + lane];
     const uint32_t g = TileOf<TX, TY>(ns, a.W, a.H, cell);
     const uint32_t d = a.tile_init[g] != kNever ? lab[LocalIndex(ns, a.W, a.H, cell)] >> 3 : kInfD;
-    if (d < best) {
-      best = d;
-      best_pin = p;
-    }
+    if (d < kInfD) key = (static_cast<uint64_t>(d) << 32) | lane;
   }
+  for (int o = 32; o > 0; o >>= 1) key = min(key, ShflXorU64(key, o));
+  const uint32_t best = key == ~0ull ? kInfD : static_cast<uint32_t>(key >> 32);
+  const uint32_t best_pin = key == ~0ull ? kNever : static_cast<uint32_t>(key);
   // Pending work can only create labels above its settling keys, so the
   // nearest pin is settled (its distance and every label on its predecessor
   // chain exact) once its distance is at most the smallest pending one.
