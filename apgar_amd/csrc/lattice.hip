@@ -1151,12 +1151,6 @@ __global__ __launch_bounds__(kThreads) void SeedKernel(SearchArgs a) {
   }
 }
 
-__device__ __forceinline__ uint64_t ShflXorU64(uint64_t v, int mask) {
-  const uint32_t lo = __shfl_xor(static_cast<uint32_t>(v), mask, 64);
-  const uint32_t hi = __shfl_xor(static_cast<uint32_t>(v >> 32), mask, 64);
-  return (static_cast<uint64_t>(hi) << 32) | lo;
-}
-
 __device__ __forceinline__ uint32_t Neighbour(uint32_t cell, uint32_t code, uint32_t W,
                                               uint32_t plane) {
   switch (code) {
@@ -1220,16 +1214,17 @@ __device__ void PhaseNet(const SearchArgs& a, uint32_t r, uint32_t n) {
   // label (pins <= 32), then a wave minimum over (distance, pin) picks it, so
   // the pins' dependent loads overlap instead of running one pin after another.
   const uint32_t lane = threadIdx.x & 63u;
-  uint64_t key = ~0ull;
+  uint32_t d = kInfD;
   if (lane < ns.pin_count && !(st.connected & (1u << lane))) {
     const uint32_t cell = a.pins[ns.pin_offset + lane];
     const uint32_t g = TileOf<TX, TY>(ns, a.W, a.H, cell);
-    const uint32_t d = a.tile_init[g] != kNever ? lab[LocalIndex(ns, a.W, a.H, cell)] >> 3 : kInfD;
-    if (d < kInfD) key = (static_cast<uint64_t>(d) << 32) | lane;
+    d = a.tile_init[g] != kNever ? lab[LocalIndex(ns, a.W, a.H, cell)] >> 3 : kInfD;
   }
-  for (int o = 32; o > 0; o >>= 1) key = min(key, ShflXorU64(key, o));
-  const uint32_t best = key == ~0ull ? kInfD : static_cast<uint32_t>(key >> 32);
-  const uint32_t best_pin = key == ~0ull ? kNever : static_cast<uint32_t>(key);
+  uint32_t m = d;  // minimum over lanes 0..31 (the pins)
+  for (int o = 16; o > 0; o >>= 1) m = min(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), o, 32)));
+  const uint32_t best = __builtin_amdgcn_readfirstlane(m);
+  const uint64_t tied = __ballot(lane < 32 && d == best);
+  const uint32_t best_pin = best < kInfD ? static_cast<uint32_t>(__builtin_ctzll(tied)) : kNever;
   // Pending work can only create labels above its settling keys, so the
   // nearest pin is settled (its distance and every label on its predecessor
   // chain exact) once its distance is at most the smallest pending one.
