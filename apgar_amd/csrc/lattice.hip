@@ -1210,21 +1210,18 @@ __device__ void PhaseNet(const SearchArgs& a, uint32_t r, uint32_t n) {
   }
   const NetStatic ns = a.nets[n];
   uint32_t* lab = a.labels + ns.label_offset;
-  // The nearest unconnected pin (ties: the lowest index): lane p reads pin p's
-  // label (pins <= 32), then a wave minimum over (distance, pin) picks it, so
-  // the pins' dependent loads overlap instead of running one pin after another.
-  const uint32_t lane = threadIdx.x & 63u;
-  uint32_t d = kInfD;
-  if (lane < ns.pin_count && !(st.connected & (1u << lane))) {
-    const uint32_t cell = a.pins[ns.pin_offset + lane];
+  const uint32_t plane = a.W * a.H;
+  uint32_t best = kInfD, best_pin = kNever;
+  for (uint32_t p = 0; p < ns.pin_count; ++p) {
+    if (st.connected & (1u << p)) continue;
+    const uint32_t cell = a.pins[ns.pin_offset + p];
     const uint32_t g = TileOf<TX, TY>(ns, a.W, a.H, cell);
-    d = a.tile_init[g] != kNever ? lab[LocalIndex(ns, a.W, a.H, cell)] >> 3 : kInfD;
+    const uint32_t d = a.tile_init[g] != kNever ? lab[LocalIndex(ns, a.W, a.H, cell)] >> 3 : kInfD;
+    if (d < best) {
+      best = d;
+      best_pin = p;
+    }
   }
-  uint32_t m = d;  // minimum over lanes 0..31 (the pins)
-  for (int o = 16; o > 0; o >>= 1) m = min(m, static_cast<uint32_t>(__shfl_xor(static_cast<int>(m), o, 32)));
-  const uint32_t best = __builtin_amdgcn_readfirstlane(m);
-  const uint64_t tied = __ballot(lane < 32 && d == best);
-  const uint32_t best_pin = best < kInfD ? static_cast<uint32_t>(__builtin_ctzll(tied)) : kNever;
   // Pending work can only create labels above its settling keys, so the
   // nearest pin is settled (its distance and every label on its predecessor
   // chain exact) once its distance is at most the smallest pending one.
