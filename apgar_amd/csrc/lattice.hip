@@ -1261,12 +1261,13 @@ __device__ void PhaseNet(const SearchArgs& a, uint32_t r, uint32_t n) {
 //
 // One wave walks a path, segment by segment: the wave loads the TX x TY plane
 // of the tile and layer holding the current cell into LDS with coalesced row
-// loads, then lane 0 follows the chain inside it at LDS latency (cell output
-// and reseed stores as it goes) until the chain leaves the segment.  The tiles
-// to queue (each change of tile along the path) collect in LDS and are pushed
-// by all lanes together.  A path thus costs one global round trip per segment
-// instead of one per cell, with the same cells, checks and pushes as a
-// cell-by-cell walk.
+// loads, lane 0 follows the chain inside it at LDS latency, recording the
+// cells' segment indices, until the chain leaves the segment or ends, and all
+// lanes then write the segment's path cells and reseed labels together.  The
+// tiles to queue (each change of tile along the path) collect in LDS and are
+// pushed by all lanes together.  A path thus costs one global round trip per
+// segment instead of one per cell, with the same cells, checks and pushes as
+// a cell-by-cell walk.
 __device__ __forceinline__ uint64_t ShflU64(uint64_t v, int src) {
   const uint32_t lo = __shfl(static_cast<uint32_t>(v), src, 64);
   const uint32_t hi = __shfl(static_cast<uint32_t>(v >> 32), src, 64);
@@ -1286,8 +1287,10 @@ template <int TX, int TY>
 __device__ void WalkConnection(const SearchArgs& a, uint32_t r, const NetStatic& ns, NetState& st,
                                DevConnection& c, uint32_t distance, uint32_t start, bool reseed, uint32_t n) {
   __shared__ uint32_t s_seg[TX * TY];
+  __shared__ uint16_t s_list[TX * TY];  // segment-local indices of the chain, in walk order
   __shared__ uint32_t s_push[kWalkPushes];
   __shared__ WalkState s_walk;
+  __shared__ uint32_t s_count, s_reseeded;  // chain cells in this segment, and how many are reseeded
   const int lane = static_cast<int>(threadIdx.x & 63u);
   uint32_t* lab = a.labels + ns.label_offset;
   const uint32_t plane = a.W * a.H;
@@ -1325,50 +1328,99 @@ __device__ void WalkConnection(const SearchArgs& a, uint32_t r, const NetStatic&
     for (uint32_t y = 0; y < ny; ++y)
       for (uint32_t x = static_cast<uint32_t>(lane); x < nx; x += 64) s_seg[y * TX + x] = seg[y * ns.w + x];
     WaveSync();
+    const uint32_t x_lo = ns.x0 + bx, y_lo = ns.y0 + by;
+    // Lane 0 follows the chain inside the segment, recording each cell's
+    // segment index: a step that stays in the segment (same layer, inside the
+    // tile) needs no other check.  The chase ends at a cell at distance 0 or at
+    // the first step that leaves the segment (a via, a tile edge or a bad
+    // code), which then takes the window and layer checks.  A chain is simple
+    // (distances fall along it), so a step inside the segment after all TX * TY
+    // cells were visited marks a corrupted chain.
     if (lane == 0) {
       WalkState u = w;
-      const uint32_t x_lo = ns.x0 + bx, y_lo = ns.y0 + by;
+      uint32_t lx = u.x - x_lo, ly = u.y - y_lo, cnt = 0, word = 0;
+      bool cycle = false;
       for (;;) {
-        const uint32_t lx = u.x - x_lo, ly = u.y - y_lo;  // unsigned: below the segment wraps high
-        if (u.l != w.l || lx >= nx || ly >= ny) break;    // the chain left the segment
-        const uint32_t word = s_seg[ly * TX + lx];
-        if (u.len >= bound) {
-          u.bad = 1;
+        const uint32_t i = ly * TX + lx;
+        word = s_seg[i];
+        s_list[cnt++] = static_cast<uint16_t>(i);
+        if ((word >> 3) == 0) break;
+        const uint32_t code = word & 7u;
+        uint32_t nlx = lx, nly = ly;
+        if (code == kFromSouth && ly > 0) --nly;
+        else if (code == kFromNorth && ly + 1 < ny) ++nly;
+        else if (code == kFromWest && lx > 0) --nlx;
+        else if (code == kFromEast && lx + 1 < nx) ++nlx;
+        else break;
+        if (cnt == TX * TY) {
+          cycle = true;
           break;
         }
-        const uint32_t cell = (u.l * a.H + u.y) * a.W + u.x;
-        a.cells[offset + u.len++] = cell;
-        if ((word >> 3) == 0) {
-          u.done = 1;
-          break;
-        }
+        lx = nlx;
+        ly = nly;
+      }
+      u.x = x_lo + lx;
+      u.y = y_lo + ly;
+      // Path cells written and cells reseeded (each cell that takes a valid
+      // step onward): the walk writes at most `bound` cells (its reservation).
+      const uint64_t room = w.len < bound ? bound - w.len : 0;
+      uint32_t reseeded;
+      if (cnt > room) {
+        cnt = static_cast<uint32_t>(room);  // each of these stepped inside the segment
+        reseeded = cnt;
+        u.bad = 1;
+      } else if ((word >> 3) == 0) {
+        reseeded = cnt - 1;
+        u.done = 1;
+      } else if (cycle) {
+        reseeded = cnt;
+        u.bad = 1;
+      } else {
         const uint32_t code = word & 7u;
         const bool ok = code <= kFromAbove && (code != kFromBelow || u.l > 0) &&
                         (code != kFromAbove || u.l + 1 < a.L) && (code != kFromWest || u.x > ns.x0) &&
                         (code != kFromEast || u.x + 1 < ns.x0 + ns.w) && (code != kFromSouth || u.y > ns.y0) &&
                         (code != kFromNorth || u.y + 1 < ns.y0 + ns.h);
         if (!ok) {
+          reseeded = cnt - 1;
           u.bad = 1;
-          break;
-        }
-        if (reseed) {
-          lab[(static_cast<uint64_t>(u.l) * ns.h + (u.y - ns.y0)) * ns.w + (u.x - ns.x0)] = kSourceWord;
-          const uint32_t g = ns.tile_offset + ((u.y - ns.y0) / TY) * ns.tiles_x + (u.x - ns.x0) / TX;
-          if (g != u.pushed) {
-            s_push[u.npush++] = g;
-            u.pushed = g;
+        } else {
+          reseeded = cnt;
+          switch (code) {
+            case kFromBelow: --u.l; break;
+            case kFromSouth: --u.y; break;
+            case kFromWest: --u.x; break;
+            case kFromEast: ++u.x; break;
+            case kFromNorth: ++u.y; break;
+            default: ++u.l; break;
           }
         }
-        switch (code) {
-          case kFromBelow: --u.l; break;
-          case kFromSouth: --u.y; break;
-          case kFromWest: --u.x; break;
-          case kFromEast: ++u.x; break;
-          case kFromNorth: ++u.y; break;
-          default: ++u.l; break;
+      }
+      // Every cell of a segment lies in one tile: it is queued once, unless it
+      // is the tile queued last.
+      if (reseed && reseeded > 0) {
+        const uint32_t g = ns.tile_offset + (by / TY) * ns.tiles_x + bx / TX;
+        if (g != u.pushed) {
+          s_push[u.npush++] = g;
+          u.pushed = g;
         }
       }
+      u.len = w.len + cnt;
+      s_count = cnt;
+      s_reseeded = reseed ? reseeded : 0;
       s_walk = u;
+    }
+    WaveSync();
+    {
+      // Every lane writes its share of the chain: the path cells, and the
+      // reseeded cells as sources of the next connection step.
+      const uint32_t cnt = s_count, reseeded = s_reseeded;
+      for (uint32_t i = static_cast<uint32_t>(lane); i < cnt; i += 64) {
+        const uint32_t li = s_list[i];
+        const uint32_t lx = li % TX, ly = li / TX;
+        a.cells[offset + w.len + i] = (w.l * a.H + y_lo + ly) * a.W + x_lo + lx;
+        if (i < reseeded) lab[(static_cast<uint64_t>(w.l) * ns.h + by + ly) * ns.w + bx + lx] = kSourceWord;
+      }
     }
     WaveSync();
   }
