@@ -197,6 +197,28 @@ def cpu_pairs_baseline(cost, via, pairs, sample, threads):
             "heuristic": HEURISTIC, "host": host_cpu()}
 
 
+def cpu_exact_pass_baseline(lat, cost, via, nets, pins, budget_s):
+    """The allocator oracle's exact-order pass on one thread (the order is
+    sequential, so more threads do not help it): pass 1 replayed from the
+    GPU's state after pass 0 for as many nets as fit in budget_s seconds, the
+    trees compared with the GPU's pass 1, the time extrapolated to the pass."""
+    from oracle import py as oracle_py
+
+    a = lat.negotiate(nets, pins, maximum_passes=1, batch_nets=0, group_nets=0)
+    b = lat.negotiate(nets, pins, maximum_passes=2, batch_nets=0, group_nets=0)
+    hist = 10 * np.maximum(a["occupancy"].astype(np.int64) - 1, 0)
+    onets = [(n.pin_offset, n.pin_count, (n.window.x0, n.window.y0, n.window.width, n.window.height)) for n in nets]
+    trees, secs = oracle_py.replay_exact_pass(cost, via, onets, pins, a["routes"], a["occupancy"], hist, 1,
+                                              budget_s=budget_s)
+    k = len(trees)
+    same = all((t is None) == (g is None) and (t is None or np.array_equal(t, g)) for t, g in zip(trees, b["routes"]))
+    return {"value": len(nets) * secs / k * 1e3, "unit": "ms per exact-order pass (extrapolated)", "cores": 1,
+            "kind": "port",
+            "sample": f"the first {k} of the {len(nets)} nets of pass 1, replayed on one thread from the GPU's "
+                      f"state after pass 0, {secs:.1f} s ({k / secs:.2f} nets/s)",
+            "sample_trees_equal_gpu": bool(same), "host": host_cpu()}
+
+
 def cpu_nets_baseline(cost, via, nets, pins, sample, threads):
     """CPU oracle multi-pin routing (repeated Dijkstra from the tree) on an evenly
     strided sample of nets, one net per thread, and on one thread alone."""
@@ -508,7 +530,9 @@ def allocator(args):
     until zero overflow or the pass limit, with a line per pass on stderr; with
     --prior-gain the same board again with its history seeded by the learned
     prior (trained on the --dataset runs of six_layer_512 boards).  The JSON
-    line reports overuse and wall time per pass of each arm."""
+    line reports overuse and wall time per pass of each arm and, unless
+    --no-cpu, the oracle's exact-order pass time on one thread, extrapolated
+    from a 20 s sample whose trees are checked against the GPU's."""
     from apgar_amd import prior_study as ps
     from tests.lattice_cases import SIX_LAYER_1536, SIX_LAYER_512, make_board
 
@@ -544,6 +568,10 @@ def allocator(args):
                        "order": f"speculative groups of {args.group_nets}" if args.group_nets
                        else "window-disjoint levels"},
             "plain": plain}
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_exact_pass_baseline(lat, cost, SIX_LAYER_1536["via"], nets, pins, 20.0)
+        log(f"[bench] allocator cpu baseline: {line['cpu_baseline']['sample']}, trees equal: "
+            f"{line['cpu_baseline']['sample_trees_equal_gpu']}")
     if args.prior_gain > 0:
         path = Path(args.dataset) if args.dataset else None
         if path is not None and path.exists():

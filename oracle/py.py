@@ -190,6 +190,59 @@ def negotiate(cost, via, nets, pins, *, capacity=None, initial_history=None, max
             "history": hist.reshape(L, H, W), "occupancy": occ.reshape(L, H, W), "routes": routes}
 
 
+def replay_exact_pass(cost, via, nets, pins, incumbents, occupancy, history, pass_index, *, first=None,
+                      budget_s=None, capacity=1, initial_present_cost=0, present_cost_increment=100,
+                      maximum_congestion_cost_value=10_000_000, maximum_cell_cost=(1 << 20) - 1):
+    """The first nets of one exact-order pass of the allocator oracle, started
+    from a given state: the incumbent trees (None: unrouted), usage and history
+    planes at the start of pass `pass_index`.  Per net, in net order
+    (src/allocator/cpu_sequential_negotiated_routing.cc:1024-1254, as
+    orc_negotiate with batches of one): rip up its incumbent, snapshot the cell
+    costs of its window, base + history + pf * max(0, usage + 1 - capacity)
+    with pf = initial + pass_index * increment (:306-326, :446-480), route it
+    (canonical multi-pin growth from pin 0, vias paying the entered cell), and
+    install the new tree, or restore the incumbent when the net is
+    disconnected.  Stops after `first` nets, or once `budget_s` seconds have
+    passed.  nets: (pin_offset, pin_count, window) per net.  Returns the trees
+    of the nets routed, in order, and the seconds spent."""
+    import time
+
+    base = np.ascontiguousarray(cost, dtype=np.uint32)
+    L, H, W = base.shape
+    occ = np.array(occupancy, dtype=np.int64).reshape(L, H, W)
+    hist = np.asarray(history, dtype=np.int64).reshape(L, H, W)
+    pf = initial_present_cost + pass_index * present_cost_increment
+    snap = base.copy()
+    trees = []
+    t0 = time.perf_counter()
+    for i, (off, cnt, win) in enumerate(nets):
+        if first is not None and i >= first:
+            break
+        if budget_s is not None and i > 0 and time.perf_counter() - t0 >= budget_s:
+            break
+        inc = incumbents[i]
+        if inc is not None:
+            np.subtract.at(occ.reshape(-1), np.asarray(inc, dtype=np.int64), 1)
+        x0, y0, w, h = win if win is not None else (0, 0, W, H)
+        sl = (slice(None), slice(y0, y0 + h), slice(x0, x0 + w))
+        b = base[sl].astype(np.int64)
+        cong = hist[sl] + pf * np.maximum(occ[sl] + 1 - capacity, 0)
+        if (np.where(b > 0, cong, 0) > maximum_congestion_cost_value).any() or \
+                (np.where(b > 0, b + cong, 0) > maximum_cell_cost).any():
+            raise ValueError(f"net {i}: congestion cost above the value bound")
+        snap[sl] = np.where(b > 0, b + cong, 0).astype(np.uint32)
+        rc, conns = Dense(snap, via, window=win, via_pays_cell=True).route_net(pins[off:off + cnt])
+        if rc == 0:
+            tree = np.concatenate([np.asarray(pins[off:off + 1], dtype=np.uint32)] +
+                                  [c[2][:-1] for c in conns]).astype(np.uint32)
+        else:
+            tree = inc
+        if tree is not None:
+            np.add.at(occ.reshape(-1), np.asarray(tree, dtype=np.int64), 1)
+        trees.append(tree)
+    return trees, time.perf_counter() - t0
+
+
 # ------------------------------------------------------------------ V1 model
 class V1Node(C.Structure):
     _fields_ = [("x", C.c_int64), ("y", C.c_int64), ("layer", C.c_uint32), ("mask", C.c_uint8),

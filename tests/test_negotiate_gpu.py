@@ -222,6 +222,36 @@ def test_groups_equal_levels_on_the_full_board(apgx, gpu):
     lat.close()
 
 
+def test_full_board_exact_passes_match_the_oracle_replay(apgx, oracle, gpu):
+    """six_layer_1536 seed 200 at full size (the whole oracle run is far too
+    slow here): the first nets of pass 0 (base costs) and of pass 1 (exact
+    order, present factor 100, history from pass 0's overuse) are the oracle's
+    trees, bit for bit, when the oracle replays each pass from the GPU's state
+    at its start."""
+    spec = SIX_LAYER_1536
+    _say("[case] six_layer_1536 seed 200: first nets of passes 0 and 1 vs the oracle replay")
+    cost, nets, pins = make_board(apgx, spec, 200)
+    lat = apgx.Lattice(cost, via_cost=spec["via"], cost_bytes=2)
+    a = lat.negotiate(nets, pins, maximum_passes=1, batch_nets=0, group_nets=0)
+    b = lat.negotiate(nets, pins, maximum_passes=2, batch_nets=0, group_nets=0)
+    lat.close()
+    hist = 10 * np.maximum(a["occupancy"].astype(np.int64) - 1, 0)
+    assert np.array_equal(hist, b["history"])  # the update after pass 0
+    onets = oracle_nets(nets)
+    zeros = np.zeros(cost.shape, dtype=np.int64)
+    first0, t0 = oracle.replay_exact_pass(cost, spec["via"], onets, pins, [None] * len(nets), zeros, zeros, 0,
+                                          first=3)
+    first1, t1 = oracle.replay_exact_pass(cost, spec["via"], onets, pins, a["routes"], a["occupancy"], hist, 1,
+                                          first=6)
+    _say(f"[info] oracle replay: 3 nets of pass 0 in {t0:.1f} s, 6 nets of pass 1 in {t1:.1f} s")
+    for orc, gpu_trees in ((first0, a["routes"]), (first1, b["routes"])):
+        for i, t in enumerate(orc):
+            assert t is not None and gpu_trees[i] is not None, i
+            assert np.array_equal(t, gpu_trees[i]), i
+    # Pass 1 moved some of these nets (else the check would only repeat pass 0).
+    assert any(not np.array_equal(a["routes"][i], b["routes"][i]) for i in range(6))
+
+
 def tree_is_legal(route, pins, flat_cost, W, H, L, window):
     cells = set(int(c) for c in route)
     assert len(cells) == len(route)

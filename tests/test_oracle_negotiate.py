@@ -146,3 +146,44 @@ def test_zero_present_factor_pass_is_order_free(oracle, seed):
     c = oracle.negotiate(cost, 5, nets, pins, batch_nets=1, **kw)
     d = oracle.negotiate(cost, 5, nets, pins, batch_nets=len(nets), **kw)
     assert c["overuse_per_pass"][0] < d["overuse_per_pass"][0]
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_replayed_pass_equals_the_oracle_pass(oracle, seed):
+    """replay_exact_pass (one exact-order pass from a given state, used at
+    full size where the whole oracle run is too slow) started from the state
+    after pass 0 gives the trees of pass 1 of a two-pass oracle run, net for
+    net, with windows; stopped early, the trees it routed are the first ones."""
+    rng = np.random.default_rng(700 + seed)
+    L, H, W = 2, 20, 24
+    cost = random_lattice(rng, L, H, W, blocked=0.1, cmax=6)
+    free = np.flatnonzero(cost.reshape(-1))
+    pins = rng.choice(free, size=48, replace=False).astype(np.uint32)
+    nets = []
+    for i in range(16):
+        p = pins[3 * i:3 * i + 3].astype(np.int64)
+        xs, ys = p % W, (p // W) % H
+        x0, y0 = max(int(xs.min()) - 2, 0), max(int(ys.min()) - 2, 0)
+        nets.append((3 * i, 3, (x0, y0, min(int(xs.max()) + 3, W) - x0, min(int(ys.max()) + 3, H) - y0)))
+    a = oracle.negotiate(cost, 5, nets, pins, maximum_passes=1)  # state after pass 0
+    b = oracle.negotiate(cost, 5, nets, pins, maximum_passes=2)
+    assert a["status"] == b["status"] == 0
+    assert a["overuse_per_pass"][0] > 0 and b["passes"] == 2
+    hist = 10 * np.maximum(a["occupancy"].astype(np.int64) - 1, 0)
+    assert np.array_equal(hist, b["history"])  # the update after the non-terminal pass 0
+    trees, _ = oracle.replay_exact_pass(cost, 5, nets, pins, a["routes"], a["occupancy"], hist, 1)
+    assert len(trees) == len(nets)
+    for x, y in zip(trees, b["routes"]):
+        assert (x is None) == (y is None)
+        if x is not None:
+            assert np.array_equal(x, y)
+    head, _ = oracle.replay_exact_pass(cost, 5, nets, pins, a["routes"], a["occupancy"], hist, 1, first=5)
+    assert len(head) == 5 and all(np.array_equal(x, y) for x, y in zip(head, trees))
+    # Planted mistakes change the trees: history left out, incumbents not
+    # ripped up.
+    def differs(other):
+        return any((x is None) != (y is None) or (x is not None and not np.array_equal(x, y))
+                   for x, y in zip(other, trees))
+
+    assert differs(oracle.replay_exact_pass(cost, 5, nets, pins, a["routes"], a["occupancy"], 0 * hist, 1)[0])
+    assert differs(oracle.replay_exact_pass(cost, 5, nets, pins, [None] * len(nets), a["occupancy"], hist, 1)[0])
