@@ -661,6 +661,7 @@ __global__ __launch_bounds__(kThreads) void RoundKernel(SearchArgs a, uint32_t c
 // ---------------------------------------------------------------------------
 constexpr int kRT = 64;          // tile side
 constexpr int kRP = kRT + 1;     // LDS pitch of the transpose buffers
+constexpr int kEdgeRows = 16;    // rows loaded together when staging a window-edge tile
 constexpr uint32_t kBlocked8 = 0xFFFFFFFFu;
 constexpr uint32_t kBlockedHalf = 0xFFFFu;  // packed cost half of a blocked cell
 constexpr uint32_t kRegMaxCost = 4095;      // (cost << 3) must stay a positive int16
@@ -795,14 +796,23 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
       // wave's LDS buffer (out-of-window cells masked to blocked / unreached),
       // then unrolled conflict-free reads fill registers.  No per-row
       // predicate lives in the unrolled register code.
+      // Rows go in groups of kEdgeRows: the group's loads are all issued
+      // before the first is used (a uniform test per row, of the cost width or
+      // of the tile's initialisation, would branch and serialise them).
       const uint32_t xc = min(static_cast<uint32_t>(lane), static_cast<uint32_t>(nx - 1));
-#pragma unroll 8
-      for (int y = 0; y < kRT; ++y) {
-        const uint32_t row = static_cast<uint32_t>(min(y, ny - 1));
-        const uint32_t coff = row * a.W + xc;
-        uint32_t c = cost_bytes == 2 ? cbase16[coff] : cbase32[coff];
-        c = (xin && y < ny) ? c : 0u;
-        T[y * kRP + lane] = PackCost<WIDE>(c);
+#pragma unroll 1
+      for (int y0 = 0; y0 < kRT; y0 += kEdgeRows) {
+        uint32_t c[kEdgeRows];
+        if (cost_bytes == 2) {
+#pragma unroll
+          for (int k = 0; k < kEdgeRows; ++k) c[k] = cbase16[static_cast<uint32_t>(min(y0 + k, ny - 1)) * a.W + xc];
+        } else {
+#pragma unroll
+          for (int k = 0; k < kEdgeRows; ++k) c[k] = cbase32[static_cast<uint32_t>(min(y0 + k, ny - 1)) * a.W + xc];
+        }
+#pragma unroll
+        for (int k = 0; k < kEdgeRows; ++k)
+          T[(y0 + k) * kRP + lane] = PackCost<WIDE>((xin && y0 + k < ny) ? c[k] : 0u);
       }
       WaveSync();
 #pragma unroll
@@ -811,11 +821,18 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
         cstr[i] = T[lane * kRP + 2 * i] | (T[lane * kRP + 2 * i + 1] << 16);
       }
       WaveSync();
+      if (initd) {
+#pragma unroll 1
+        for (int y0 = 0; y0 < kRT; y0 += kEdgeRows) {
+          uint32_t v[kEdgeRows];
+#pragma unroll
+          for (int k = 0; k < kEdgeRows; ++k) v[k] = lbase[static_cast<uint32_t>(min(y0 + k, ny - 1)) * w + xc];
+#pragma unroll
+          for (int k = 0; k < kEdgeRows; ++k) T[(y0 + k) * kRP + lane] = (xin && y0 + k < ny) ? v[k] : kInf;
+        }
+      } else {
 #pragma unroll 8
-      for (int y = 0; y < kRT; ++y) {
-        const uint32_t row = static_cast<uint32_t>(min(y, ny - 1));
-        uint32_t v = initd ? lbase[row * w + xc] : kInf;
-        T[y * kRP + lane] = (xin && y < ny) ? v : kInf;
+        for (int y = 0; y < kRT; ++y) T[y * kRP + lane] = kInf;
       }
       WaveSync();
 #pragma unroll
