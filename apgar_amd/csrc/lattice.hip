@@ -1375,29 +1375,25 @@ __device__ void WalkConnection(const SearchArgs& a, uint32_t r, const NetStatic&
     // cells were visited marks a corrupted chain.
     if (lane == 0) {
       WalkState u = w;
-      uint32_t lx = u.x - x_lo, ly = u.y - y_lo, cnt = 0, word = 0;
-      bool cycle = false;
+      // The step is computed without branches (selects on the code), so a
+      // cell costs the LDS read and one exit test.
+      const uint32_t end = ny * TX;
+      uint32_t i = (u.y - y_lo) * TX + (u.x - x_lo), cnt = 0, word = 0, next = kNever;
       for (;;) {
-        const uint32_t i = ly * TX + lx;
         word = s_seg[i];
         s_list[cnt++] = static_cast<uint16_t>(i);
-        if ((word >> 3) == 0) break;
-        const uint32_t code = word & 7u;
-        uint32_t nlx = lx, nly = ly;
-        if (code == kFromSouth && ly > 0) --nly;
-        else if (code == kFromNorth && ly + 1 < ny) ++nly;
-        else if (code == kFromWest && lx > 0) --nlx;
-        else if (code == kFromEast && lx + 1 < nx) ++nlx;
-        else break;
-        if (cnt == TX * TY) {
-          cycle = true;
-          break;
-        }
-        lx = nlx;
-        ly = nly;
+        const uint32_t code = word & 7u, col = i % TX;
+        next = kNever;  // the step leaves the segment (a via, an edge or a bad code)
+        next = (code == kFromSouth && i >= TX) ? i - TX : next;
+        next = (code == kFromNorth && i + TX < end) ? i + TX : next;
+        next = (code == kFromWest && col > 0) ? i - 1 : next;
+        next = (code == kFromEast && col + 1 < nx) ? i + 1 : next;
+        if ((word >> 3) == 0 || next == kNever || cnt == TX * TY) break;
+        i = next;
       }
-      u.x = x_lo + lx;
-      u.y = y_lo + ly;
+      const bool cycle = (word >> 3) != 0 && next != kNever;  // only at cnt == TX * TY
+      u.x = x_lo + i % TX;
+      u.y = y_lo + i / TX;
       // Path cells written and cells reseeded (each cell that takes a valid
       // step onward): the walk writes at most `bound` cells (its reservation).
       const uint64_t room = w.len < bound ? bound - w.len : 0;
