@@ -383,8 +383,9 @@ extern "C" int apgx_negotiate(apgx_lattice* lat, const uint8_t* capacity, const 
     // the rounds (tile visits they waste are nearly free there).  A large
     // batch keeps the lattice search's default (visits cost throughput).
     // six_layer_1536, an exact-order pass of 1-2 nets per search: 17.8 s at
-    // the default width, 9.6 s at 16 times it; the 2000-net first pass:
-    // 0.33 s at the default, 0.75 s at 16 times it.
+    // the default width, 9.6 s at 16 times it (wider buckets change nothing
+    // more: the rounds are then the propagation depth); the 2000-net first
+    // pass: 0.33 s at the default, 0.75 s at 16 times it.
     apgx_route_options opt = sopt;
     if (opt.frontier_delta == 0 && count <= kLatencyBatchNets)
       opt.frontier_delta = static_cast<uint32_t>(std::max(1.0, kLatencyDeltaCells * lat->mean_cost));
