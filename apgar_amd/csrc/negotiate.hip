@@ -42,9 +42,11 @@ constexpr int kBlock = 256;
 constexpr uint32_t kGrid = 2048;  // grid-stride kernels: 8 waves' worth per CU
 // Searches of at most this many nets use a bucket of kLatencyDeltaCells mean
 // cell costs (16 times the lattice search's default of 48) unless the caller
-// sets frontier_delta.
+// sets frontier_delta, and check for their end every kLatencySyncRounds
+// rounds unless the caller sets rounds_per_sync.
 constexpr uint32_t kLatencyBatchNets = 8;
 constexpr double kLatencyDeltaCells = 768.0;
+constexpr uint32_t kLatencySyncRounds = 16;
 
 // Parameters of the captured batch-prep graph, in device memory.
 struct BatchParams {
@@ -386,9 +388,15 @@ extern "C" int apgx_negotiate(apgx_lattice* lat, const uint8_t* capacity, const 
     // the default width, 9.6 s at 16 times it (wider buckets change nothing
     // more: the rounds are then the propagation depth); the 2000-net first
     // pass: 0.33 s at the default, 0.75 s at 16 times it.
+    // Such a search also checks for its end every kLatencySyncRounds rounds
+    // rather than every 8: a check leaves the GPU idle while the host reads
+    // the round's count, and the rounds launched past the end are empty
+    // (six_layer_1536 pass 1: 8.05 s at 8, 7.89 s at 16, 8.32 s at 4; the
+    // same trees).
     apgx_route_options opt = sopt;
     if (opt.frontier_delta == 0 && count <= kLatencyBatchNets)
       opt.frontier_delta = static_cast<uint32_t>(std::max(1.0, kLatencyDeltaCells * lat->mean_cost));
+    if (opt.rounds_per_sync == 0 && count <= kLatencyBatchNets) opt.rounds_per_sync = kLatencySyncRounds;
     apgx_search* s = nullptr;
     if (int rc = RouteNets(lat, &ov, batch_nets.data(), count, pins, pin_count, &opt, &s); rc != APGX_OK)
       return rc;
