@@ -410,7 +410,10 @@ typedef struct apgx_negotiation apgx_negotiation;
 void apgx_negotiation_policy_defaults(apgx_negotiation_policy* policy);
 /* capacity: one byte per cell (NULL: capacity 1 everywhere); initial_history:
  * one u32 per cell seeding the history plane (NULL: zero, the reference's
- * pass-zero state); search_options: passed to every search (NULL: defaults). */
+ * pass-zero state); search_options: passed to every search (NULL: defaults).
+ * Searches of at most 8 nets, which leave the GPU nearly idle, take a bucket
+ * 16 times the default width and check for their end every 16 rounds where
+ * frontier_delta and rounds_per_sync are 0; neither changes a route. */
 int apgx_negotiate(apgx_lattice* lattice, const uint8_t* capacity, const apgx_net_query* nets,
                    uint32_t net_count, const uint32_t* pins, uint64_t pin_count,
                    const apgx_negotiation_policy* policy, const uint32_t* initial_history,
