@@ -752,21 +752,15 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
     const uint64_t cbase_index = ns.cost_offset + (static_cast<uint64_t>(l) * a.H + ns.y0 + by) * a.W + ns.x0 + bx;
     const uint16_t* cbase16 = static_cast<const uint16_t*>(a.cost) + cbase_index;
     const uint32_t* cbase32 = static_cast<const uint32_t*>(a.cost) + cbase_index;
-    // The init rounds of the tile and its four neighbours, read together: a
-    // missing neighbour reads the tile's own entry (a valid address, result
-    // unused), so no read waits behind the test of another.
-    const uint32_t* tinit = a.tile_init + ns.tile_offset;
-    const uint32_t init_self = tinit[t];
-    const uint32_t init_w = tinit[tx > 0 ? t - 1 : t];
-    const uint32_t init_e = tinit[tx + 1 < ns.tiles_x ? t + 1 : t];
-    const uint32_t init_s = tinit[ty > 0 ? t - ns.tiles_x : t];
-    const uint32_t init_n = tinit[ty + 1 < ns.tiles_y ? t + ns.tiles_x : t];
-    const bool initd = init_self != kNever;
-    auto ready = [&](uint32_t v) { return v != kNever && v < r; };
-    const bool has_w = tx > 0 && ready(init_w);
-    const bool has_e = tx + 1 < ns.tiles_x && ready(init_e);
-    const bool has_s = ty > 0 && ready(init_s);
-    const bool has_n = ty + 1 < ns.tiles_y && ready(init_n);
+    const bool initd = a.tile_init[g] != kNever;
+    auto ready = [&](uint32_t ntx, uint32_t nty) {
+      const uint32_t v = a.tile_init[ns.tile_offset + nty * ns.tiles_x + ntx];
+      return v != kNever && v < r;
+    };
+    const bool has_w = tx > 0 && ready(tx - 1, ty);
+    const bool has_e = tx + 1 < ns.tiles_x && ready(tx + 1, ty);
+    const bool has_s = ty > 0 && ready(tx, ty - 1);
+    const bool has_n = ty + 1 < ns.tiles_y && ready(tx, ty + 1);
 
     const bool xin = lane < nx;
     uint32_t lab[kRT], cst[kRT / 2], cstr[kRT / 2];
@@ -849,18 +843,10 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
     const uint32_t orig_n = T[(ny - 1) * kRP + lane];
     const uint32_t orig_w = T[lane * kRP];
     const uint32_t orig_e = T[lane * kRP + nx - 1];
-    // Halos, the four loads issued together: a lane without a halo cell reads
-    // a cell of the tile itself (column and row clamped into the tile).
-    const int64_t hx = min(lane, nx - 1), hy = min(lane, ny - 1);
-    const int64_t wi = static_cast<int64_t>(w);
-    const uint32_t hv_s = lbase[has_s ? hx - wi : hx];
-    const uint32_t hv_n = lbase[has_n ? hx + kRT * wi : hx];
-    const uint32_t hv_w = lbase[has_w ? hy * wi - 1 : hy * wi];
-    const uint32_t hv_e = lbase[has_e ? hy * wi + kRT : hy * wi];
-    const uint32_t halo_s = (has_s && xin) ? hv_s : kInf;
-    const uint32_t halo_n = (has_n && xin) ? hv_n : kInf;
-    const uint32_t halo_w = (has_w && lane < ny) ? hv_w : kInf;
-    const uint32_t halo_e = (has_e && lane < ny) ? hv_e : kInf;
+    const uint32_t halo_s = (has_s && xin) ? lbase[static_cast<int64_t>(lane) - static_cast<int64_t>(w)] : kInf;
+    const uint32_t halo_n = (has_n && xin) ? lbase[lane + kRT * w] : kInf;
+    const uint32_t halo_w = (has_w && lane < ny) ? lbase[static_cast<int64_t>(lane) * w - 1] : kInf;
+    const uint32_t halo_e = (has_e && lane < ny) ? lbase[lane * w + kRT] : kInf;
     WaveSync();
     const uint32_t via8 = a.via << 3;
 
@@ -1038,27 +1024,21 @@ void RegRoundKernel(SearchArgs a, uint32_t cost_bytes) {
       // misses a change the neighbour's fixed point depends on.
       const uint32_t ws = T[lane], wn = T[(ny - 1) * kRP + lane];
       const uint32_t ww = T[lane * kRP], we = T[lane * kRP + nx - 1];
-      // The costs of the four neighbour cells, loaded together (a missing
-      // neighbour reads a cell of the tile itself, result unused).
-      const int64_t Wi = static_cast<int64_t>(a.W);
-      const int64_t off_s = ty > 0 ? hx - Wi : hx;
-      const int64_t off_n = ty + 1 < ns.tiles_y ? kRT * Wi + hx : hx;
-      const int64_t off_w = tx > 0 ? hy * Wi - 1 : hy * Wi;
-      const int64_t off_e = tx + 1 < ns.tiles_x ? hy * Wi + kRT : hy * Wi;
-      uint32_t c_s, c_n, c_w, c_e;
-      if (cost_bytes == 2) {
-        c_s = cbase16[off_s]; c_n = cbase16[off_n]; c_w = cbase16[off_w]; c_e = cbase16[off_e];
-      } else {
-        c_s = cbase32[off_s]; c_n = cbase32[off_n]; c_w = cbase32[off_w]; c_e = cbase32[off_e];
-      }
-      auto cost8 = [](uint32_t c) -> uint32_t { return c ? c << 3 : kBlocked8; };
-      if (xin && ty > 0 && (ws >> 3) != (orig_s >> 3) && Relax8(ws, cost8(c_s), kFromNorth) < halo_s)
+      auto cost8 = [&](int64_t off) -> uint32_t {
+        const uint32_t c = cost_bytes == 2 ? cbase16[off] : cbase32[off];
+        return c ? c << 3 : kBlocked8;
+      };
+      if (xin && ty > 0 && (ws >> 3) != (orig_s >> 3) &&
+          Relax8(ws, cost8(static_cast<int64_t>(lane) - static_cast<int64_t>(a.W)), kFromNorth) < halo_s)
         atomicMin(&s_side[2], ws >> 3);
-      if (xin && ty + 1 < ns.tiles_y && (wn >> 3) != (orig_n >> 3) && Relax8(wn, cost8(c_n), kFromSouth) < halo_n)
+      if (xin && ty + 1 < ns.tiles_y && (wn >> 3) != (orig_n >> 3) &&
+          Relax8(wn, cost8(static_cast<int64_t>(kRT) * a.W + lane), kFromSouth) < halo_n)
         atomicMin(&s_side[3], wn >> 3);
-      if (lane < ny && tx > 0 && (ww >> 3) != (orig_w >> 3) && Relax8(ww, cost8(c_w), kFromEast) < halo_w)
+      if (lane < ny && tx > 0 && (ww >> 3) != (orig_w >> 3) &&
+          Relax8(ww, cost8(static_cast<int64_t>(lane) * a.W - 1), kFromEast) < halo_w)
         atomicMin(&s_side[0], ww >> 3);
-      if (lane < ny && tx + 1 < ns.tiles_x && (we >> 3) != (orig_e >> 3) && Relax8(we, cost8(c_e), kFromWest) < halo_e)
+      if (lane < ny && tx + 1 < ns.tiles_x && (we >> 3) != (orig_e >> 3) &&
+          Relax8(we, cost8(static_cast<int64_t>(lane) * a.W + kRT), kFromWest) < halo_e)
         atomicMin(&s_side[1], we >> 3);
     }
     if (xin) {
