@@ -203,8 +203,9 @@ def replay_exact_pass(cost, via, nets, pins, incumbents, occupancy, history, pas
     (canonical multi-pin growth from pin 0, vias paying the entered cell), and
     install the new tree, or restore the incumbent when the net is
     disconnected.  Stops after `first` nets, or once `budget_s` seconds have
-    passed.  nets: (pin_offset, pin_count, window) per net.  Returns the trees
-    of the nets routed, in order, and the seconds spent."""
+    passed.  nets: (pin_offset, pin_count, window) per net; capacity: one value
+    for every cell or a plane of the lattice's shape.  Returns the trees of
+    the nets routed, in order, and the seconds spent."""
     import time
 
     base = np.ascontiguousarray(cost, dtype=np.uint32)
@@ -212,6 +213,8 @@ def replay_exact_pass(cost, via, nets, pins, incumbents, occupancy, history, pas
     occ = np.array(occupancy, dtype=np.int64).reshape(L, H, W)
     hist = np.asarray(history, dtype=np.int64).reshape(L, H, W)
     pf = initial_present_cost + pass_index * present_cost_increment
+    cap = np.broadcast_to(np.asarray(capacity, dtype=np.int64).reshape(
+        (L, H, W) if np.ndim(capacity) else ()), (L, H, W))
     snap = base.copy()
     trees = []
     t0 = time.perf_counter()
@@ -226,7 +229,7 @@ def replay_exact_pass(cost, via, nets, pins, incumbents, occupancy, history, pas
         x0, y0, w, h = win if win is not None else (0, 0, W, H)
         sl = (slice(None), slice(y0, y0 + h), slice(x0, x0 + w))
         b = base[sl].astype(np.int64)
-        cong = hist[sl] + pf * np.maximum(occ[sl] + 1 - capacity, 0)
+        cong = hist[sl] + pf * np.maximum(occ[sl] + 1 - cap[sl], 0)
         if (np.where(b > 0, cong, 0) > maximum_congestion_cost_value).any() or \
                 (np.where(b > 0, b + cong, 0) > maximum_cell_cost).any():
             raise ValueError(f"net {i}: congestion cost above the value bound")

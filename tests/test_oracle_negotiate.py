@@ -187,3 +187,33 @@ def test_replayed_pass_equals_the_oracle_pass(oracle, seed):
 
     assert differs(oracle.replay_exact_pass(cost, 5, nets, pins, a["routes"], a["occupancy"], 0 * hist, 1)[0])
     assert differs(oracle.replay_exact_pass(cost, 5, nets, pins, [None] * len(nets), a["occupancy"], hist, 1)[0])
+
+
+def test_replayed_pass_with_capacity_plane_and_seeded_history(oracle):
+    """The replay with a capacity plane and a seeded history equals pass 1 of
+    the oracle run with the same planes."""
+    rng = np.random.default_rng(731)
+    L, H, W = 2, 20, 24
+    cost = random_lattice(rng, L, H, W, blocked=0.1, cmax=6)
+    free = np.flatnonzero(cost.reshape(-1))
+    pins = rng.choice(free, size=36, replace=False).astype(np.uint32)
+    nets = [(2 * i, 2, None) for i in range(18)]
+    cap = rng.integers(0, 3, cost.size).astype(np.uint8).reshape(cost.shape)
+    cap.reshape(-1)[pins] = 1  # pins keep room for their own net
+    seed = rng.integers(0, 30, cost.size).astype(np.uint32)
+    kw = dict(capacity=cap, initial_history=seed)
+    a = oracle.negotiate(cost, 5, nets, pins, maximum_passes=1, **kw)
+    b = oracle.negotiate(cost, 5, nets, pins, maximum_passes=2, **kw)
+    assert a["status"] == b["status"] == 0 and b["passes"] == 2 and a["overuse_per_pass"][0] > 0
+    over = np.maximum(a["occupancy"].astype(np.int64) - cap.astype(np.int64), 0)
+    hist = seed.reshape(cost.shape).astype(np.int64) + 10 * over
+    assert np.array_equal(hist, b["history"])
+    trees, _ = oracle.replay_exact_pass(cost, 5, nets, pins, a["routes"], a["occupancy"], hist, 1, capacity=cap)
+    for x, y in zip(trees, b["routes"]):
+        assert (x is None) == (y is None)
+        if x is not None:
+            assert np.array_equal(x, y)
+    # Capacity 1 everywhere instead of the plane gives other trees.
+    other, _ = oracle.replay_exact_pass(cost, 5, nets, pins, a["routes"], a["occupancy"], hist, 1)
+    assert any((x is None) != (y is None) or (x is not None and not np.array_equal(x, y))
+               for x, y in zip(other, trees))
